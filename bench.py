@@ -1,0 +1,192 @@
+"""Throughput of the lesgo time step on one MI355X.
+
+Workload (default): wind-farm LES, 256 x 256 x 192 grid on Lx = 2 pi,
+Ly = pi, Lz = 1, 4 x 4 aligned actuator-disk array (D = 0.1, hub 0.1,
+7D x 5D, Ct' = 1.33), Lagrangian scale-dependent SGS, equilibrium wall
+model z0 = 1e-4, dt = 1e-4, seeded log-law start (seed 7).  Both the plain
+FP64 step and the mixed-precision step are timed; `value` is the mixed step's
+grid-pts * steps / s, the FP64 figure and the ratio are reported beside it.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs one case
+per GPU with no sharding.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpts·steps/s at 256³-class, mixed vs FP64; ensemble p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (nx, ny, Nz, dt, seed, turbines)
+    "windfarm256": (256, 256, 192, 1e-4, 7, True),
+    "channel128": (128, 128, 128, 2e-4, 0, False),
+    "channel32": (32, 32, 32, 2e-4, 0, False),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(name, precision):
+    import lesgo_amd
+    nx, ny, Nz, dt, seed, turb = WORKLOADS[name]
+    cfg = lesgo_amd.Config(nx, ny, Nz, dt=dt, precision=precision)
+    c = lesgo_amd.Case(cfg)
+    for n, a in lesgo_amd.seeded_state(cfg, seed).items():
+        c.upload(n, a)
+    c.set_counters(0, 0)
+    if turb:
+        c.turbines_init(lesgo_amd.aligned_farm(cfg))
+    return cfg, c
+
+
+def time_precision(name, precision, steps, warmup):
+    cfg, c = make_case(name, precision)
+    npts = cfg.nx * cfg.ny * cfg.Nz
+    c.step(warmup)
+    c.synchronize()
+    c.timing(True)
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.synchronize()
+    t1 = time.perf_counter()
+    rep = c.timing_report()
+    c.timing(False)
+    rms = c.rmsdiv()
+    c.close()
+    dt = t1 - t0
+    res = {
+        "value": npts * steps / dt / 1e6,
+        "ms_per_step": 1e3 * dt / steps,
+        "rmsdiv": rms,
+        "kernels": {k: {"ms_total": v[0], "launches": v[1], "gbytes": v[2] / 1e9}
+                    for k, v in rep.items()},
+    }
+    dom = max(rep.items(), key=lambda kv: kv[1][0])
+    ms, nl, by = dom[1]
+    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    res["roofline"] = {
+        "kernel": dom[0],
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "launches": nl,
+        "avg_launch_us": 1e3 * ms / nl if nl else 0.0,
+        "share_of_step": ms / (1e3 * dt),
+    }
+    return res
+
+
+def cpu_baseline(name):
+    """The numpy oracle on this host: one plain step of the full workload,
+    scaled by the dynamic/plain step-time ratio measured at 1/8 the points
+    (every 5th step runs the Lagrangian update)."""
+    import numpy as np  # noqa: F401
+
+    from oracle import lesgo_oracle as o
+    nx, ny, Nz, dt, seed, turb = WORKLOADS[name]
+
+    def steps_time(nx_, ny_, Nz_, with_dyn):
+        p = o.Params(nx_, ny_, Nz_, dt=dt)
+        sp = o.Spectral(p)
+        s = o.initial_condition(p, seed)
+        if turb:
+            s.turbines = o.Turbines(p, o.aligned_farm(p))
+        s.jt_total = 3
+        t0 = time.perf_counter()
+        o.step(sp, s)
+        t_plain = time.perf_counter() - t0
+        t_dyn = None
+        if with_dyn:
+            t0 = time.perf_counter()
+            o.step(sp, s)
+            t_dyn = time.perf_counter() - t0
+        return t_plain, t_dyn
+
+    sp_, sd_ = steps_time(nx // 2, ny // 2, Nz // 2, True)
+    ratio = sd_ / sp_
+    tp, _ = steps_time(nx, ny, Nz, False)
+    t_avg = tp * (4 + ratio) / 5
+    return {
+        "value": nx * ny * Nz / t_avg / 1e6,
+        "unit": "Mpts*steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"numpy oracle, 1 plain step at {nx}x{ny}x{Nz} ({tp:.1f} s) scaled by the "
+                   f"dynamic/plain ratio {ratio:.2f} measured at {nx//2}x{ny//2}x{Nz//2} "
+                   f"(1 of every 5 steps is dynamic)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="windfarm256", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs one case per GPU with no sharding; "
+                                   "run with --gpus 1 (multi-GPU scaling is not built)"}))
+        sys.exit(2)
+    nx, ny, Nz, dt, seed, turb = WORKLOADS[args.workload]
+    out = {}
+    if args.precision in ("both", "fp64"):
+        log(f"[bench] fp64 {args.workload}: warmup {args.warmup}, steps {args.steps}")
+        out["fp64"] = time_precision(args.workload, "fp64", args.steps, args.warmup)
+        log(f"[bench] fp64 done: {out['fp64']['value']:.1f} Mpts*steps/s")
+    if args.precision in ("both", "mixed"):
+        log(f"[bench] mixed {args.workload}: warmup {args.warmup}, steps {args.steps}")
+        out["mixed"] = time_precision(args.workload, "mixed", args.steps, args.warmup)
+        log(f"[bench] mixed done: {out['mixed']['value']:.1f} Mpts*steps/s")
+    head = out.get("mixed") or out["fp64"]
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "Mpts*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if "mixed" in out else "fp64",
+        "data": f"synthetic: seeded log-law profile + 10% uniform noise (seed {seed})",
+        "config": {
+            "workload": (f"{args.workload}: {nx}x{ny}x{Nz}, Lx=2pi Ly=pi Lz=1, dt={dt}, "
+                         "Lagrangian scale-dependent SGS, equilibrium wall z0=1e-4"
+                         + (", 4x4 ADM D=0.1 hub=0.1 7Dx5D Ct'=1.33" if turb else "")),
+            "grid": [nx, ny, Nz],
+            "precision_of_value": "mixed" if "mixed" in out else "fp64",
+        },
+        "roofline": {k: v for k, v in head["roofline"].items()},
+    }
+    if "fp64" in out:
+        line["fp64"] = {k: out["fp64"][k] for k in ("value", "ms_per_step", "roofline")}
+    if "fp64" in out and "mixed" in out:
+        line["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    line["kernels"] = head["kernels"]
+    if not args.no_cpu:
+        log("[bench] cpu baseline (numpy oracle)")
+        line["cpu_baseline"] = cpu_baseline(args.workload)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

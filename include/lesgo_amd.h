@@ -1,0 +1,157 @@
+/*
+ * lesgo_amd C ABI: MI355X (gfx950) implementation of one lesgo LES time step.
+ *
+ * The reference's time step is a set of Fortran subroutines acting on module
+ * globals (sim_param.f90:31-93, sgs_param.f90:29-184, param.f90).  This ABI
+ * keeps that shape: a case handle owns device-resident copies of those module
+ * arrays, `lg_upload`/`lg_download` move them to and from host memory in the
+ * reference's own layout (Fortran (ld, ny, 0:nz) column-major == C
+ * [nz+1][ny][ld]), and one entry point per replaced subroutine operates on
+ * the device state.  Every function returns 0 on success or a positive
+ * LG_ERR_* code; lg_last_error() returns a message (the reference prints and
+ * stops, messages.f90:228-241; callers of this ABI decide).
+ *
+ * Only plain C types cross this boundary.
+ */
+#ifndef LESGO_AMD_H
+#define LESGO_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LG_OK = 0,
+    LG_ERR_ARG = 1,
+    LG_ERR_HIP = 2,
+    LG_ERR_ALLOC = 3,
+    LG_ERR_STATE = 4,
+    LG_ERR_UNSUPPORTED = 5,
+    LG_ERR_PIVOT = 6
+};
+
+/* Precision of a case. */
+enum { LG_FP64 = 0, LG_MIXED = 1 };
+
+/* Field identifiers: the sim_param / sgs_param arrays, all (ld, ny, 0:nz). */
+enum {
+    LG_U = 0, LG_V, LG_W,
+    LG_DUDX, LG_DUDY, LG_DUDZ, LG_DVDX, LG_DVDY, LG_DVDZ,
+    LG_DWDX, LG_DWDY, LG_DWDZ,
+    LG_RHSX, LG_RHSY, LG_RHSZ, LG_RHSX_F, LG_RHSY_F, LG_RHSZ_F,
+    LG_DPDX, LG_DPDY, LG_DPDZ,
+    LG_TXX, LG_TXY, LG_TYY, LG_TXZ, LG_TYZ, LG_TZZ,
+    LG_DIVTX, LG_DIVTY, LG_DIVTZ,
+    LG_P, LG_FXA, LG_FYA, LG_FZA,
+    LG_S11, LG_S12, LG_S13, LG_S22, LG_S23, LG_S33,
+    LG_NU_T, LG_CS_OPT2, LG_F_LM, LG_F_MM, LG_F_QN, LG_F_NN, LG_BETA, LG_TN_ALL,
+    LG_NFIELDS
+};
+
+/*
+ * Run parameters (param.f90).  Nz is the number of uv levels in the domain;
+ * the one-rank layout has nz = Nz + 1 levels above the ghost level 0
+ * (input_util.f90:196-235).  Physics: sgs_model = 5, lbc_mom = 2 (equilibrium
+ * wall model), ubc_mom = 0 (stress-free lid), static dt with AB2
+ * (tadv1 = 1.5), inflow_type = 0.
+ */
+typedef struct {
+    int nx, ny, Nz;
+    double lx, ly, lz;
+    double dt;
+    double zo;
+    double vonk;
+    int cs_count;
+    int use_mean_p_force;
+    double mean_p_force_x, mean_p_force_y;
+    double nu;
+    int precision;          /* LG_FP64 or LG_MIXED */
+} lg_params;
+
+typedef struct lg_case lg_case;
+
+/* Lifetime (replaces sim_param_init/sgs_param_init/init_fft/test_filter_init:
+ * sim_param.f90:44-99, sgs_param.f90:79-195, fft.f90:102-127,
+ * test_filtermodule.f90:38-123).  Fields start at zero. */
+int lg_create(const lg_params* prm, lg_case** out);
+int lg_destroy(lg_case* c);
+const char* lg_last_error(void);
+int lg_sizes(const lg_case* c, int* nz, int* ld, int* ny);
+
+/* Host <-> device copies of one field, host layout double [nz+1][ny][ld]. */
+int lg_upload(lg_case* c, int field, const double* host);
+int lg_download(lg_case* c, int field, double* host);
+/* Time counters jt, jt_total (param.f90:146-147). */
+int lg_set_counters(lg_case* c, int jt, int jt_total);
+int lg_get_counters(const lg_case* c, int* jt, int* jt_total);
+int lg_synchronize(lg_case* c);
+
+/* --- replaced subroutines (device state in, device state out) ---------- */
+/* derivatives::filt_da (derivatives.f90:166-211), levels 0..nz */
+int lg_filt_da(lg_case* c, int f, int dfdx, int dfdy);
+/* derivatives::ddx / ddy / ddxy (derivatives.f90:37-163), levels 0..nz */
+int lg_ddx(lg_case* c, int f, int dfdx);
+int lg_ddy(lg_case* c, int f, int dfdy);
+int lg_ddxy(lg_case* c, int f, int dfdx, int dfdy);
+/* derivatives::ddz_uv / ddz_w (derivatives.f90:214-311) */
+int lg_ddz_uv(lg_case* c, int f, int dfdz);
+int lg_ddz_w(lg_case* c, int f, int dfdz);
+/* wallstress (wallstress.f90:21-257): lbc_mom = 2, ubc_mom = 0 */
+int lg_wallstress(lg_case* c);
+/* sgs_stag (sgs_stag_util.f90:33-464) incl. lagrange_Sdep when it is due */
+int lg_sgs_stag(lg_case* c);
+/* lagrange_Sdep (lagrange_Sdep.f90:21-432) incl. interpolag_Sdep */
+int lg_lagrange_sdep(lg_case* c);
+/* interpolag_Sdep (interpolag_Sdep.f90:21-270) alone */
+int lg_interpolag_sdep(lg_case* c);
+/* divstress_uv / divstress_w (divstress_uv.f90:21-85, divstress_w.f90:21-116) */
+int lg_divstress_uv(lg_case* c);
+int lg_divstress_w(lg_case* c);
+/* convec (convec.f90:21-334) */
+int lg_convec(lg_case* c);
+/* press_stag_array + tridag_array (press_stag_array.f90:21-290,
+ * tridag_array.f90:165-246) */
+int lg_press_stag_array(lg_case* c);
+/* forcing::project (forcing.f90:149-244) */
+int lg_project(lg_case* c);
+/* One or more whole time steps (main.f90:146-344). */
+int lg_step(lg_case* c, int nsteps);
+
+/* Diagnostics: rmsdiv.f90:21-60, io.f90:99-141 (energy), cfl_util.f90:35-70,
+ * functions.f90:882-905 (tau_wall_bot). */
+int lg_rmsdiv(lg_case* c, double* out);
+int lg_energy(lg_case* c, double* out);
+int lg_max_cfl(lg_case* c, double* out);
+int lg_tau_wall_bot(lg_case* c, double* out);
+
+/* --- actuator disks (turbines.f90:123-638, turbine_indicator.f90) ------- */
+/* Each turbine: xloc, yloc, height, dia, thk, theta1, theta2, Ct_prime
+ * (the columns of input_turbines/param.dat, nondimensional). */
+int lg_turbines_init(lg_case* c, int nturb, const double* spec8,
+                     double alpha1, double alpha2, double filter_cutoff,
+                     int adm_correction);
+int lg_turbines_forcing(lg_case* c);
+/* u_d_T per turbine (turbines.f90:657-679 checkpoint content). */
+int lg_turbines_get(lg_case* c, double* u_d_T, double* f_n, int nturb);
+
+/* --- per-kernel timing (no reference counterpart: the reference only has
+ * whole-iteration clocks, clocks.f90:47-83).  When enabled, HIP events are
+ * recorded on the case's stream around every launch of the kernel classes
+ * below; lg_timing_report returns total milliseconds, launch count and the
+ * algorithmic bytes (inputs read + outputs written once) of one class. */
+enum {
+    LG_K_ROWS_R2C = 0,
+    LG_K_ROWS_C2R = 1,
+    LG_K_COLS_FWD = 2,
+    LG_K_COLS_INV = 3,
+    LG_K_TRIDAG = 4,
+    LG_K_NKINDS = 5
+};
+int lg_timing_enable(lg_case* c, int on);
+int lg_timing_report(lg_case* c, int kind, double* ms, double* launches, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LESGO_AMD_H */

@@ -1,0 +1,103 @@
+"""ctypes binding of liblesgo_amd.so (the C ABI declared in include/lesgo_amd.h).
+
+The library is loaded from this package directory only; if it is missing
+the import fails loudly -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblesgo_amd.so")
+
+FIELDS = ("u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz",
+          "dwdx", "dwdy", "dwdz", "RHSx", "RHSy", "RHSz", "RHSx_f", "RHSy_f",
+          "RHSz_f", "dpdx", "dpdy", "dpdz", "txx", "txy", "tyy", "txz", "tyz",
+          "tzz", "divtx", "divty", "divtz", "p", "fxa", "fya", "fza",
+          "S11", "S12", "S13", "S22", "S23", "S33", "Nu_t", "Cs_opt2",
+          "F_LM", "F_MM", "F_QN", "F_NN", "Beta", "Tn_all")
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+
+LG_FP64, LG_MIXED = 0, 1
+
+
+class LgParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("Nz", C.c_int),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("dt", C.c_double), ("zo", C.c_double), ("vonk", C.c_double),
+                ("cs_count", C.c_int), ("use_mean_p_force", C.c_int),
+                ("mean_p_force_x", C.c_double), ("mean_p_force_y", C.c_double),
+                ("nu", C.c_double), ("precision", C.c_int)]
+
+
+# name -> (restype, argtypes)
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+SIGNATURES = {
+    "lg_create": (C.c_int, [C.POINTER(LgParams), C.POINTER(C.c_void_p)]),
+    "lg_destroy": (C.c_int, [_P]),
+    "lg_last_error": (C.c_char_p, []),
+    "lg_sizes": (C.c_int, [_P, _I, _I, _I]),
+    "lg_upload": (C.c_int, [_P, C.c_int, _D]),
+    "lg_download": (C.c_int, [_P, C.c_int, _D]),
+    "lg_set_counters": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_get_counters": (C.c_int, [_P, _I, _I]),
+    "lg_synchronize": (C.c_int, [_P]),
+    "lg_filt_da": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "lg_ddx": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_ddy": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_ddxy": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "lg_ddz_uv": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_ddz_w": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_wallstress": (C.c_int, [_P]),
+    "lg_sgs_stag": (C.c_int, [_P]),
+    "lg_lagrange_sdep": (C.c_int, [_P]),
+    "lg_interpolag_sdep": (C.c_int, [_P]),
+    "lg_divstress_uv": (C.c_int, [_P]),
+    "lg_divstress_w": (C.c_int, [_P]),
+    "lg_convec": (C.c_int, [_P]),
+    "lg_press_stag_array": (C.c_int, [_P]),
+    "lg_project": (C.c_int, [_P]),
+    "lg_step": (C.c_int, [_P, C.c_int]),
+    "lg_rmsdiv": (C.c_int, [_P, _D]),
+    "lg_energy": (C.c_int, [_P, _D]),
+    "lg_max_cfl": (C.c_int, [_P, _D]),
+    "lg_tau_wall_bot": (C.c_int, [_P, _D]),
+    "lg_turbines_init": (C.c_int, [_P, C.c_int, _D, C.c_double, C.c_double, C.c_double, C.c_int]),
+    "lg_turbines_forcing": (C.c_int, [_P]),
+    "lg_turbines_get": (C.c_int, [_P, _D, _D, C.c_int]),
+    "lg_timing_enable": (C.c_int, [_P, C.c_int]),
+    "lg_timing_report": (C.c_int, [_P, C.c_int, _D, _D, _D]),
+}
+
+KERNEL_KINDS = ("rows_r2c", "rows_c2r", "cols_fwd", "cols_inv", "tridag")
+
+
+class LesgoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load liblesgo_amd.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not found: build it with `python -m lesgo_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().lg_last_error().decode(errors="replace")
+        raise LesgoError(f"{what} failed (code {rc}): {msg}")

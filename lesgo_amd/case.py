@@ -1,0 +1,276 @@
+"""Host-side mirror of the reference's time-step interface.
+
+``Case`` owns one device-resident LES case (the sim_param / sgs_param
+module state of the reference, sim_param.f90:31-93, sgs_param.f90:29-184)
+and exposes one method per replaced subroutine, under the reference's names:
+``filt_da``, ``ddz_uv``, ``ddz_w``, ``wallstress``, ``sgs_stag``,
+``lagrange_Sdep``, ``interpolag_Sdep``, ``divstress_uv``, ``divstress_w``,
+``convec``, ``press_stag_array``, ``project``, ``rmsdiv``, ``energy`` and
+``turbines_forcing``; ``step`` is one pass of the time loop
+(main.f90:146-344).  Host arrays use the reference layout: Fortran
+``f(ld, ny, 0:nz)`` == numpy ``[nz+1, ny, ld]`` float64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD_ID, FIELDS, LG_FP64, LG_MIXED, LgParams, check, lib
+
+PI = 3.1415926535897932384626433
+
+
+@dataclass
+class Config:
+    """Run parameters (param.f90 subset used on the time step)."""
+    nx: int
+    ny: int
+    Nz: int
+    lx: float = 2.0 * PI
+    ly: float = PI
+    lz: float = 1.0
+    dt: float = 2.0e-4
+    zo: float = 1.0e-4
+    vonk: float = 0.4
+    cs_count: int = 5
+    use_mean_p_force: bool = True
+    mean_p_force_x: float = 1.0
+    mean_p_force_y: float = 0.0
+    nu: float = 0.0
+    precision: str = "fp64"          # "fp64" | "mixed"
+
+    def to_c(self) -> LgParams:
+        return LgParams(self.nx, self.ny, self.Nz, self.lx, self.ly, self.lz,
+                        self.dt, self.zo, self.vonk, self.cs_count,
+                        int(self.use_mean_p_force), self.mean_p_force_x,
+                        self.mean_p_force_y, self.nu,
+                        LG_FP64 if self.precision == "fp64" else LG_MIXED)
+
+
+def _fid(f) -> int:
+    return f if isinstance(f, int) else FIELD_ID[f]
+
+
+class Case:
+    """One device-resident LES case behind the lesgo_amd C ABI."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().lg_create(C.byref(cfg.to_c()), C.byref(h)), "lg_create")
+        self._h = h
+        nz, ld, ny = C.c_int(), C.c_int(), C.c_int()
+        check(lib().lg_sizes(h, C.byref(nz), C.byref(ld), C.byref(ny)), "lg_sizes")
+        self.nz, self.ld, self.ny = nz.value, ld.value, ny.value
+        self.shape = (self.nz + 1, self.ny, self.ld)
+        self.nturb = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- module state ----------------------------------------------------
+    def upload(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"{name}: expected shape {self.shape}, got {a.shape}")
+        check(lib().lg_upload(self._h, _fid(name), a.ctypes.data_as(_lib._D)), f"upload {name}")
+
+    def download(self, name):
+        a = np.empty(self.shape)
+        check(lib().lg_download(self._h, _fid(name), a.ctypes.data_as(_lib._D)), f"download {name}")
+        return a
+
+    def set_counters(self, jt: int, jt_total: int):
+        check(lib().lg_set_counters(self._h, jt, jt_total), "set_counters")
+
+    def counters(self):
+        a, b = C.c_int(), C.c_int()
+        check(lib().lg_get_counters(self._h, C.byref(a), C.byref(b)), "get_counters")
+        return a.value, b.value
+
+    def synchronize(self):
+        check(lib().lg_synchronize(self._h), "synchronize")
+
+    def upload_state(self, fields: dict, jt: int = 0, jt_total: int = 0):
+        for n, a in fields.items():
+            if n in FIELD_ID:
+                self.upload(n, a)
+        self.set_counters(jt, jt_total)
+
+    # -- replaced subroutines ------------------------------------------------
+    def filt_da(self, f, dfdx, dfdy):
+        check(lib().lg_filt_da(self._h, _fid(f), _fid(dfdx), _fid(dfdy)), "filt_da")
+
+    def ddx(self, f, dfdx):
+        check(lib().lg_ddx(self._h, _fid(f), _fid(dfdx)), "ddx")
+
+    def ddy(self, f, dfdy):
+        check(lib().lg_ddy(self._h, _fid(f), _fid(dfdy)), "ddy")
+
+    def ddxy(self, f, dfdx, dfdy):
+        check(lib().lg_ddxy(self._h, _fid(f), _fid(dfdx), _fid(dfdy)), "ddxy")
+
+    def ddz_uv(self, f, dfdz):
+        check(lib().lg_ddz_uv(self._h, _fid(f), _fid(dfdz)), "ddz_uv")
+
+    def ddz_w(self, f, dfdz):
+        check(lib().lg_ddz_w(self._h, _fid(f), _fid(dfdz)), "ddz_w")
+
+    def wallstress(self):
+        check(lib().lg_wallstress(self._h), "wallstress")
+
+    def sgs_stag(self):
+        check(lib().lg_sgs_stag(self._h), "sgs_stag")
+
+    def lagrange_Sdep(self):
+        check(lib().lg_lagrange_sdep(self._h), "lagrange_Sdep")
+
+    def interpolag_Sdep(self):
+        check(lib().lg_interpolag_sdep(self._h), "interpolag_Sdep")
+
+    def divstress_uv(self):
+        check(lib().lg_divstress_uv(self._h), "divstress_uv")
+
+    def divstress_w(self):
+        check(lib().lg_divstress_w(self._h), "divstress_w")
+
+    def convec(self):
+        check(lib().lg_convec(self._h), "convec")
+
+    def press_stag_array(self):
+        check(lib().lg_press_stag_array(self._h), "press_stag_array")
+
+    def project(self):
+        check(lib().lg_project(self._h), "project")
+
+    def step(self, nsteps: int = 1):
+        check(lib().lg_step(self._h, nsteps), "step")
+
+    # -- diagnostics --------------------------------------------------------
+    def _scalar(self, fn, what):
+        out = C.c_double()
+        check(fn(self._h, C.byref(out)), what)
+        return out.value
+
+    def rmsdiv(self):
+        return self._scalar(lib().lg_rmsdiv, "rmsdiv")
+
+    def energy(self):
+        return self._scalar(lib().lg_energy, "energy")
+
+    def max_cfl(self):
+        return self._scalar(lib().lg_max_cfl, "max_cfl")
+
+    def tau_wall_bot(self):
+        return self._scalar(lib().lg_tau_wall_bot, "tau_wall_bot")
+
+    # -- per-kernel timing --------------------------------------------------
+    def timing(self, on: bool):
+        check(lib().lg_timing_enable(self._h, int(on)), "timing_enable")
+
+    def timing_report(self):
+        """{kind: (ms_total, launches, algorithmic_bytes)} for the timed launches."""
+        out = {}
+        for i, name in enumerate(_lib.KERNEL_KINDS):
+            a, b, e = C.c_double(), C.c_double(), C.c_double()
+            check(lib().lg_timing_report(self._h, i, C.byref(a), C.byref(b), C.byref(e)),
+                  "timing_report")
+            out[name] = (a.value, b.value, e.value)
+        return out
+
+    # -- actuator disks -----------------------------------------------------
+    def turbines_init(self, specs, alpha1=1.5, alpha2=1.5, filter_cutoff=1e-2,
+                      adm_correction=True):
+        """specs: iterable of (xloc, yloc, height, dia, thk, theta1, theta2, Ct')."""
+        a = np.ascontiguousarray(np.asarray(specs, dtype=np.float64).reshape(-1, 8))
+        check(lib().lg_turbines_init(self._h, a.shape[0], a.ctypes.data_as(_lib._D), alpha1,
+                                     alpha2, filter_cutoff, int(adm_correction)), "turbines_init")
+        self.nturb = a.shape[0]
+
+    def turbines_forcing(self):
+        check(lib().lg_turbines_forcing(self._h), "turbines_forcing")
+
+    def turbines_state(self):
+        ud = np.empty(self.nturb)
+        fn = np.empty(self.nturb)
+        check(lib().lg_turbines_get(self._h, ud.ctypes.data_as(_lib._D),
+                                    fn.ctypes.data_as(_lib._D), self.nturb), "turbines_get")
+        return ud, fn
+
+
+def seeded_state(cfg: Config, seed: int, noise: float = 0.1):
+    """The seeded vel.out-equivalent start (the ic_file path,
+    initial.f90:226-239) as host arrays in the reference layout: log-law u
+    plus counter-based uniform noise (splitmix64 keyed by seed, field,
+    i, j, k), w = 0 at the wall and lid, Cs_opt2 = 0.03, F_* = 0."""
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.Nz + 1
+    ld = 2 * (nx // 2 + 1)
+    dz = cfg.lz / (nz - 1)
+    shp = (nz + 1, ny, ld)
+    out = {n: np.zeros(shp) for n in ("u", "v", "w", "Cs_opt2")}
+    k = np.arange(1, nz)[:, None, None]
+    j = np.arange(ny)[None, :, None]
+    i = np.arange(nx)[None, None, :]
+    zk = (k - 0.5) * dz
+    ubar = (1.0 / cfg.vonk) * np.log(zk / cfg.zo)
+    amp = noise * ubar
+    for fid, name in enumerate(("u", "v", "w")):
+        r = uniform01(seed, fid, i, j, k)
+        base = ubar if name == "u" else 0.0
+        out[name][1:nz, :, :nx] = base + amp * (2.0 * r - 1.0)
+    out["w"][1] = 0.0
+    out["w"][nz] = 0.0
+    out["u"][nz] = out["u"][nz - 1]
+    out["v"][nz] = out["v"][nz - 1]
+    out["Cs_opt2"][1:nz + 1] = 0.03
+    for name in ("u", "v", "w"):
+        out[name][0] = -1234567890.0
+    return out
+
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+    z = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed, fid, i, j, k):
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0x100000001B3)) & _M64
+        key = _splitmix64(key ^ np.uint64(fid))
+        key = _splitmix64(key ^ (np.asarray(k, np.uint64) << np.uint64(42)
+                                 ^ np.asarray(j, np.uint64) << np.uint64(21)
+                                 ^ np.asarray(i, np.uint64)))
+    return (key >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def aligned_farm(cfg: Config, nxt=4, nyt=4, dia=0.1, hub=0.1, sx_d=7.0, sy_d=5.0,
+                 ct_prime=1.33, thk=0.0):
+    """Regular aligned actuator-disk array centred in the domain
+    (x spacing sx_d D, y spacing sy_d D), rows of (x, y, h, D, thk, th1, th2, Ct')."""
+    x0 = 0.5 * cfg.lx - 0.5 * (nxt - 1) * sx_d * dia
+    y0 = 0.5 * cfg.ly - 0.5 * (nyt - 1) * sy_d * dia
+    return [(x0 + i * sx_d * dia, y0 + j * sy_d * dia, hub, dia, thk, 0.0, 0.0, ct_prime)
+            for i in range(nxt) for j in range(nyt)]

@@ -1,0 +1,337 @@
+// extern "C" boundary: see include/lesgo_amd.h for the per-function
+// reference citations.
+#include <new>
+#include <string>
+#include <vector>
+
+#include "case.hpp"
+
+namespace lg {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace lg
+
+struct lg_case {
+    int precision = LG_FP64;
+    lg::Case<double>* c64 = nullptr;
+    lg::Case<float>* c32 = nullptr;
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return LG_OK;
+    } catch (const lg::ArgError&) {
+        return LG_ERR_ARG;
+    } catch (const lg::HipError&) {
+        return LG_ERR_HIP;
+    } catch (const lg::PivotError&) {
+        return LG_ERR_PIVOT;
+    } catch (const std::bad_alloc&) {
+        lg::set_error("host allocation failed");
+        return LG_ERR_ALLOC;
+    } catch (...) {
+        lg::set_error("unknown error");
+        return LG_ERR_STATE;
+    }
+}
+
+#define REQUIRE(cond, msg)          \
+    do {                            \
+        if (!(cond)) {              \
+            lg::set_error(msg);     \
+            throw lg::ArgError();   \
+        }                           \
+    } while (0)
+
+// Run fn(Case<T>&) on whichever precision the handle holds.
+template <typename F>
+int dispatch(lg_case* h, F&& fn) {
+    if (!h) {
+        lg::set_error("null case handle");
+        return LG_ERR_ARG;
+    }
+    return guarded([&] {
+        if (h->c64) fn(*h->c64);
+        else fn(*h->c32);
+    });
+}
+
+bool smooth(int n) {
+    if (n <= 0) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+void check_field(int id) { REQUIRE(id >= 0 && id < LG_NFIELDS, "field id out of range"); }
+
+}  // namespace
+
+extern "C" {
+
+const char* lg_last_error(void) { return lg::g_err.c_str(); }
+
+int lg_create(const lg_params* p, lg_case** out) {
+    return guarded([&] {
+        REQUIRE(p && out, "null argument");
+        REQUIRE(p->nx >= 4 && p->ny >= 4 && p->nx % 4 == 0 && p->ny % 4 == 0,
+                "nx and ny must be multiples of 4 (3/2-rule grids must stay even)");
+        REQUIRE(smooth(p->nx) && smooth(p->ny), "nx and ny must be of the form 2^a 3^b");
+        REQUIRE(p->Nz >= 3, "Nz must be at least 3");
+        REQUIRE(p->lx > 0 && p->ly > 0 && p->lz > 0 && p->dt > 0 && p->zo > 0,
+                "lengths, dt and zo must be positive");
+        REQUIRE(p->cs_count >= 1, "cs_count must be >= 1");
+        REQUIRE(p->precision == LG_FP64 || p->precision == LG_MIXED, "bad precision");
+        auto* h = new lg_case();
+        h->precision = p->precision;
+        try {
+            if (p->precision == LG_FP64) h->c64 = new lg::Case<double>(*p);
+            else h->c32 = new lg::Case<float>(*p);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int lg_destroy(lg_case* h) {
+    if (!h) return LG_OK;
+    int rc = guarded([&] {
+        delete h->c64;
+        delete h->c32;
+    });
+    delete h;
+    return rc;
+}
+
+int lg_sizes(const lg_case* h, int* nz, int* ld, int* ny) {
+    if (!h) return LG_ERR_ARG;
+    const lg::Dims& d = h->c64 ? h->c64->d : h->c32->d;
+    if (nz) *nz = d.nz;
+    if (ld) *ld = d.ld;
+    if (ny) *ny = d.ny;
+    return LG_OK;
+}
+
+int lg_upload(lg_case* h, int field, const double* host) {
+    return dispatch(h, [&](auto& c) {
+        check_field(field);
+        REQUIRE(host, "null host pointer");
+        using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
+        if constexpr (std::is_same<T, double>::value) {
+            LG_HIP(hipMemcpyAsync(c.fld[field], host, c.d.field * sizeof(double),
+                                  hipMemcpyHostToDevice, c.stream));
+        } else {
+            std::vector<T> tmp((size_t)c.d.field);
+            for (long i = 0; i < c.d.field; ++i) tmp[i] = (T)host[i];
+            LG_HIP(hipMemcpyAsync(c.fld[field], tmp.data(), c.d.field * sizeof(T),
+                                  hipMemcpyHostToDevice, c.stream));
+        }
+        LG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int lg_download(lg_case* h, int field, double* host) {
+    return dispatch(h, [&](auto& c) {
+        check_field(field);
+        REQUIRE(host, "null host pointer");
+        using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
+        if constexpr (std::is_same<T, double>::value) {
+            LG_HIP(hipMemcpyAsync(host, c.fld[field], c.d.field * sizeof(double),
+                                  hipMemcpyDeviceToHost, c.stream));
+            LG_HIP(hipStreamSynchronize(c.stream));
+        } else {
+            std::vector<T> tmp((size_t)c.d.field);
+            LG_HIP(hipMemcpyAsync(tmp.data(), c.fld[field], c.d.field * sizeof(T),
+                                  hipMemcpyDeviceToHost, c.stream));
+            LG_HIP(hipStreamSynchronize(c.stream));
+            for (long i = 0; i < c.d.field; ++i) host[i] = (double)tmp[i];
+        }
+    });
+}
+
+int lg_set_counters(lg_case* h, int jt, int jt_total) {
+    return dispatch(h, [&](auto& c) {
+        c.jt = jt;
+        c.jt_total = jt_total;
+    });
+}
+
+int lg_get_counters(const lg_case* h, int* jt, int* jt_total) {
+    if (!h) return LG_ERR_ARG;
+    if (h->c64) {
+        if (jt) *jt = h->c64->jt;
+        if (jt_total) *jt_total = h->c64->jt_total;
+    } else {
+        if (jt) *jt = h->c32->jt;
+        if (jt_total) *jt_total = h->c32->jt_total;
+    }
+    return LG_OK;
+}
+
+int lg_synchronize(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        LG_HIP(hipStreamSynchronize(c.stream));
+        c.check_device_error();
+    });
+}
+
+int lg_filt_da(lg_case* h, int f, int dfdx, int dfdy) {
+    return dispatch(h, [&](auto& c) {
+        check_field(f);
+        check_field(dfdx);
+        check_field(dfdy);
+        c.filt_da(f, dfdx, dfdy);
+    });
+}
+
+int lg_ddx(lg_case* h, int f, int dfdx) {
+    return dispatch(h, [&](auto& c) {
+        check_field(f);
+        check_field(dfdx);
+        c.ddx(f, dfdx);
+    });
+}
+
+int lg_ddy(lg_case* h, int f, int dfdy) {
+    return dispatch(h, [&](auto& c) {
+        check_field(f);
+        check_field(dfdy);
+        c.ddy(f, dfdy);
+    });
+}
+
+int lg_ddxy(lg_case* h, int f, int dfdx, int dfdy) {
+    return dispatch(h, [&](auto& c) {
+        check_field(f);
+        check_field(dfdx);
+        check_field(dfdy);
+        c.ddxy(f, dfdx, dfdy);
+    });
+}
+
+int lg_ddz_uv(lg_case* h, int f, int dfdz) {
+    return dispatch(h, [&](auto& c) {
+        check_field(f);
+        check_field(dfdz);
+        c.ddz_uv(f, dfdz);
+    });
+}
+
+int lg_ddz_w(lg_case* h, int f, int dfdz) {
+    return dispatch(h, [&](auto& c) {
+        check_field(f);
+        check_field(dfdz);
+        c.ddz_w(f, dfdz);
+    });
+}
+
+int lg_wallstress(lg_case* h) { return dispatch(h, [&](auto& c) { c.wallstress(); }); }
+int lg_sgs_stag(lg_case* h) { return dispatch(h, [&](auto& c) { c.sgs_stag(); }); }
+int lg_lagrange_sdep(lg_case* h) { return dispatch(h, [&](auto& c) { c.lagrange_sdep(); }); }
+int lg_interpolag_sdep(lg_case* h) { return dispatch(h, [&](auto& c) { c.interpolag_sdep(); }); }
+int lg_divstress_uv(lg_case* h) { return dispatch(h, [&](auto& c) { c.divstress_uv(); }); }
+int lg_divstress_w(lg_case* h) { return dispatch(h, [&](auto& c) { c.divstress_w(); }); }
+int lg_convec(lg_case* h) { return dispatch(h, [&](auto& c) { c.convec(); }); }
+
+int lg_press_stag_array(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        c.press_stag_array();
+        c.check_device_error();
+    });
+}
+
+int lg_project(lg_case* h) { return dispatch(h, [&](auto& c) { c.project(); }); }
+
+int lg_step(lg_case* h, int nsteps) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+        for (int n = 0; n < nsteps; ++n) c.step();
+        LG_HIP(hipGetLastError());
+    });
+}
+
+int lg_rmsdiv(lg_case* h, double* out) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(out, "null output");
+        *out = c.rmsdiv();
+    });
+}
+
+int lg_energy(lg_case* h, double* out) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(out, "null output");
+        *out = c.energy();
+    });
+}
+
+int lg_max_cfl(lg_case* h, double* out) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(out, "null output");
+        *out = c.max_cfl();
+    });
+}
+
+int lg_tau_wall_bot(lg_case* h, double* out) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(out, "null output");
+        *out = c.tau_wall_bot();
+    });
+}
+
+int lg_turbines_init(lg_case* h, int nturb, const double* spec8, double alpha1, double alpha2,
+                     double filter_cutoff, int adm_correction) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(nturb >= 0 && (nturb == 0 || spec8), "bad turbine list");
+        REQUIRE(alpha1 > 0 && alpha2 > 0 && filter_cutoff >= 0, "bad turbine filter parameters");
+        for (int s = 0; s < nturb; ++s) REQUIRE(spec8[8 * s + 3] > 0, "turbine diameter must be > 0");
+        lg::turbines_init(c, nturb, spec8, alpha1, alpha2, filter_cutoff, adm_correction);
+    });
+}
+
+int lg_turbines_forcing(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(c.turb, "turbines not initialised");
+        c.turbines_forcing();
+    });
+}
+
+int lg_turbines_get(lg_case* h, double* u_d_T, double* f_n, int nturb) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(c.turb && nturb == c.turb->n, "turbine count mismatch");
+        std::vector<double> st(3 * nturb);
+        LG_HIP(hipMemcpyAsync(st.data(), c.turb->state, 3 * nturb * sizeof(double),
+                              hipMemcpyDeviceToHost, c.stream));
+        LG_HIP(hipStreamSynchronize(c.stream));
+        for (int s = 0; s < nturb; ++s) {
+            if (u_d_T) u_d_T[s] = st[3 * s + 1];
+            if (f_n) f_n[s] = st[3 * s + 2];
+        }
+    });
+}
+
+int lg_timing_enable(lg_case* h, int on) {
+    return dispatch(h, [&](auto& c) {
+        LG_HIP(hipStreamSynchronize(c.stream));
+        c.timer.reset();
+        c.timer.on = on != 0;
+    });
+}
+
+int lg_timing_report(lg_case* h, int kind, double* ms, double* launches, double* bytes) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(kind >= 0 && kind < lg::KK_NKINDS, "timing kind out of range");
+        double a, b, e;
+        c.timer.report(kind, a, b, e);
+        if (ms) *ms = a;
+        if (launches) *launches = b;
+        if (bytes) *bytes = e;
+    });
+}
+
+}  // extern "C"

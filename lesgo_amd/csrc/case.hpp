@@ -1,0 +1,143 @@
+// Device-resident state of one LES case and the subsystem drivers.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/lesgo_amd.h"
+#include "common.hpp"
+#include "fft.hpp"
+#include "timer.hpp"
+
+namespace lg {
+
+// Run parameters plus the derived sizes of the one-rank layout
+// (input_util.f90:196-235, 403-409; sgs_param.f90:187).
+struct Dims {
+    lg_params prm;
+    int nx, ny, nz, lh, ld, nx2, ny2, ld_big, NK;
+    double dx, dy, dz, tadv1, tadv2, delta, lagran_dt;
+    long plane;       // ny * ld
+    long field;       // (nz+1) * ny * ld
+    long splane;      // ny * NK (complex)
+    long bplane;      // ny2 * ld_big
+    long bsplane;     // ny2 * NK (complex)
+    explicit Dims(const lg_params& p);
+};
+
+// Device twiddle / permutation tables for one transform length.
+struct PlanTables {
+    FftPlan plan;
+    int* perm = nullptr;
+    double2* tw64 = nullptr;
+    float2* tw32 = nullptr;
+    PlanTables() = default;
+    PlanTables(const PlanTables&) = delete;
+    PlanTables& operator=(const PlanTables&) = delete;
+    ~PlanTables();
+    void build(int N);
+};
+
+// Actuator-disk node lists on the device (turbines.f90:275-462 output).
+struct TurbineSet {
+    int n = 0;
+    int total_nodes = 0;
+    bool adm_correction = true;
+    std::vector<double> spec;        // 8 per turbine
+    std::vector<double> M;           // indicator M per turbine
+    std::vector<int> h_off;
+    int* node_lin = nullptr;         // linear offset (k*ny + j)*ld + i
+    double* node_ind = nullptr;      // normalised indicator weight
+    int* off = nullptr;              // n+1 offsets
+    int* node_tid = nullptr;         // turbine of each node
+    bool has_fz = false;
+    double* coef = nullptr;          // per turbine: n1,n2,n3,ct',dia,corrfac
+    double* state = nullptr;         // per turbine: u_d, u_d_T, f_n
+    ~TurbineSet();
+};
+
+template <typename T>
+struct Case {
+    Dims d;
+    hipStream_t stream = nullptr;
+    int jt = 0, jt_total = 0;
+
+    T* fld[LG_NFIELDS] = {};
+    // spectral scratch, each (nz+1) small spectral planes
+    static constexpr int NSPEC = 8;
+    cplx<T>* spec[NSPEC] = {};
+    // big-grid scratch (levels 0..nz)
+    static constexpr int NBIG = 9;
+    T* big[NBIG] = {};
+    cplx<T>* bigspec[6] = {};
+    // physical scratch fields
+    static constexpr int NTMP = 9;
+    T* tmp[NTMP] = {};
+    // pressure solve
+    double* gam = nullptr;           // (nz+1) * ny * NK
+    double2* usc = nullptr;          // (nz+1) * ny * NK
+    int* dev_err = nullptr;
+    // Lagrangian model scratch (chunked over levels)
+    int lag_chunk = 0;
+    T* lag_phys = nullptr;           // 15 * chunk planes
+    cplx<T>* lag_spec = nullptr;     // 21 * chunk planes
+    T* lag_filt = nullptr;           // 42 * chunk planes
+    T* Ftmp = nullptr;               // 4 fields (F_* copies for interpolation)
+    double* red = nullptr;           // reduction scratch
+    T* G1 = nullptr;                 // [ny][NK]
+    T* G2 = nullptr;
+    PlanTables px, py, pbx, pby;     // nx, ny, nx2, ny2
+    std::unique_ptr<TurbineSet> turb;
+    KernelTimer timer;
+
+    explicit Case(const lg_params& p);
+    ~Case();
+    Case(const Case&) = delete;
+    Case& operator=(const Case&) = delete;
+
+    T* F(int id) { return fld[id]; }
+    T* lev(int id, int k) { return fld[id] + (long)k * d.plane; }
+
+    // 2-D transform helpers over planes [lev0, lev0+nlev)
+    void fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
+               cplx<T>* const* dst, long dst_plane, int dst_lev0, int nlev);
+    void inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane, int src_lev0,
+               T* const* dst, long dst_plane, int dst_lev0, int nlev, cplx<T>* const* tmpspec,
+               long tmp_plane);
+    void pad_inv_big(int njobs, const cplx<T>* const* src, int lev0, int nlev);
+    void fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst, int lev0, int nlev);
+
+    // subsystems
+    void filt_da(int f, int dfdx, int dfdy);
+    void ddx(int f, int dfdx);
+    void ddy(int f, int dfdy);
+    void ddxy(int f, int dfdx, int dfdy);
+    void ddz_uv(int f, int dfdz);
+    void ddz_w(int f, int dfdz);
+    void test_filter_plane(const T* in, T* out, int op);
+    void wallstress();
+    void calc_sij();
+    void sgs_stag();
+    void interpolag_sdep();
+    void lagrange_sdep();
+    void divstress_uv();
+    void divstress_w();
+    void convec();
+    void press_stag_array();
+    void project();
+    void rhs_ab2();
+    void step();
+    void turbines_forcing();
+    double rmsdiv();
+    double energy();
+    double max_cfl();
+    double tau_wall_bot();
+    void fill_level(int id, int k, double v);
+    void check_device_error();
+};
+
+template <typename T>
+void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, double alpha2,
+                   double filter_cutoff, int adm_correction);
+
+}  // namespace lg

@@ -1,0 +1,426 @@
+// Pointwise / stencil kernels of the time step.  Each kernel cites the
+// reference loop it replaces.  Index (k, j, i) of a field lives at
+// ((k * ny) + j) * ld + i; k = 0..nz (lbz = 0, the one-rank PPMPI layout).
+#pragma once
+
+#include "common.hpp"
+
+namespace lg {
+
+struct Geo {
+    int nx, ny, nz, ld;
+    long plane;
+};
+
+__device__ __forceinline__ bool ij_of(int W, int ny, int& j, int& i) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= W * ny) return false;
+    j = idx / W;
+    i = idx - j * W;
+    return true;
+}
+
+template <typename T>
+__global__ void k_fill_planes(T* f, long plane, long n, T v) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) f[idx] = v;
+}
+
+// ddz_uv (derivatives.f90:214-264): dfdz(k) = (1/dz)(f(k) - f(k-1)), k=1..nz.
+template <typename T>
+__global__ void k_ddz_uv(const T* f, T* out, Geo g, T rdz) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    out[o] = rdz * (f[o] - f[o - g.plane]);
+}
+
+// ddz_w (derivatives.f90:267-311): dfdz(k) = (1/dz)(f(k+1) - f(k)), k=0..nz-1.
+template <typename T>
+__global__ void k_ddz_w(const T* f, T* out, Geo g, T rdz) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    out[o] = rdz * (f[o + g.plane] - f[o]);
+}
+
+// ws_equilibrium_lbc (wallstress.f90:171-219) on the k=1 plane.
+template <typename T>
+__global__ void k_wall_lbc(const T* u1f, const T* v1f, const T* u1, const T* v1, T* txz1,
+                           T* tyz1, T* dudz1, T* dvdz1, Geo g, double denom, double vonk,
+                           double dz) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const long o = (long)j * g.ld + i;
+    const double uf = u1f[o], vf = v1f[o];
+    const double ua = sqrt(uf * uf + vf * vf);
+    const double us = ua * vonk / denom;
+    const double c = -(us * us) / ua;
+    txz1[o] = (T)(c * uf);
+    tyz1[o] = (T)(c * vf);
+    const double uu = u1[o], vv = v1[o];
+    const double a = us / (0.5 * dz * vonk);
+    dudz1[o] = (uu == 0.0) ? (T)0 : (T)(a * uu / ua);
+    dvdz1[o] = (vv == 0.0) ? (T)0 : (T)(a * vv / ua);
+}
+
+// calc_Sij (sgs_stag_util.f90:467-634), one rank, lbc_mom >= 1, ubc_mom = 0.
+template <typename T>
+struct SijArgs {
+    const T *dudx, *dudy, *dudz, *dvdx, *dvdy, *dvdz, *dwdx, *dwdy, *dwdz;
+    T *S11, *S12, *S13, *S22, *S23, *S33;
+};
+
+template <typename T>
+__global__ void k_calc_sij(SijArgs<T> a, Geo g) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long P = g.plane;
+    const long o = (long)k * P + (long)j * g.ld + i;
+    const T h = (T)0.5;
+    if (k == 1) {
+        a.S11[o] = a.dudx[o];
+        a.S12[o] = h * (a.dudy[o] + a.dvdx[o]);
+        const T wx = h * (a.dwdx[o] + a.dwdx[o + P]);
+        a.S13[o] = h * (a.dudz[o] + wx);
+        a.S22[o] = a.dvdy[o];
+        const T wy = h * (a.dwdy[o] + a.dwdy[o + P]);
+        a.S23[o] = h * (a.dvdz[o] + wy);
+        a.S33[o] = a.dwdz[o];
+    } else if (k == g.nz) {
+        const long m = o - P;
+        a.S11[o] = a.dudx[m];
+        a.S12[o] = h * (a.dudy[m] + a.dvdx[m]);
+        a.S13[o] = h * (a.dudz[o] + a.dwdx[o]);
+        a.S22[o] = a.dvdy[m];
+        a.S23[o] = h * (a.dvdz[o] + a.dwdy[o]);
+        a.S33[o] = h * (a.dwdz[m] + (T)0);
+    } else {
+        const long m = o - P;
+        a.S11[o] = h * (a.dudx[o] + a.dudx[m]);
+        const T uy = a.dudy[o] + a.dudy[m];
+        const T vx = a.dvdx[o] + a.dvdx[m];
+        a.S12[o] = (T)0.25 * (uy + vx);
+        a.S13[o] = h * (a.dudz[o] + a.dwdx[o]);
+        a.S22[o] = h * (a.dvdy[o] + a.dvdy[m]);
+        a.S23[o] = h * (a.dvdz[o] + a.dwdy[o]);
+        a.S33[o] = h * (a.dwdz[o] + a.dwdz[m]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T smag_norm(T s11, T s12, T s13, T s22, T s23, T s33) {
+    return sqrt((T)2 * (s11 * s11 + s22 * s22 + s33 * s33 +
+                        (T)2 * (s12 * s12 + s13 * s13 + s23 * s23)));
+}
+
+// sgs_stag: Nu_t and tau_ij (sgs_stag_util.f90:226-235, 237-428), one rank.
+template <typename T>
+struct TauArgs {
+    const T *S11, *S12, *S13, *S22, *S23, *S33, *Cs;
+    T *Nu_t, *txx, *txy, *tyy, *txz, *tyz, *tzz;
+    T l2, nu;
+};
+
+template <typename T>
+__device__ __forceinline__ T nut_at(const TauArgs<T>& a, long o) {
+    return smag_norm(a.S11[o], a.S12[o], a.S13[o], a.S22[o], a.S23[o], a.S33[o]) * a.Cs[o] * a.l2;
+}
+
+template <typename T>
+__global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long P = g.plane;
+    const long o = (long)k * P + (long)j * g.ld + i;
+    const T nk = nut_at(a, o);
+    a.Nu_t[o] = nk;
+    const T nu = a.nu;
+    if (k == 1) {
+        const T c = (T)-2 * (nk + nu);
+        a.txx[o] = c * a.S11[o];
+        a.txy[o] = c * a.S12[o];
+        a.tyy[o] = c * a.S22[o];
+        a.tzz[o] = c * a.S33[o];
+    } else if (k == g.nz - 1) {
+        const long q = o + P;
+        const T nk1 = nut_at(a, q);
+        const T c = (T)0.5 * (nk + nk1) + nu;
+        const T c2 = (T)2 * (nk + nu);
+        a.txx[o] = -c * (a.S11[o] + a.S11[q]);
+        a.txy[o] = -c * (a.S12[o] + a.S12[q]);
+        a.tyy[o] = -c * (a.S22[o] + a.S22[q]);
+        a.tzz[o] = -c * (a.S33[o] + a.S33[q]);
+        a.txz[o] = -c2 * a.S13[o];
+        a.tyz[o] = -c2 * a.S23[o];
+    } else if (k < g.nz - 1) {
+        const long q = o + P;
+        const T nk1 = nut_at(a, q);
+        const T c = (T)-0.5 * (nk + nk1);
+        const T c2 = (T)-2 * nk;
+        const T c3 = (T)-2 * nu * (T)0.5;
+        const T c4 = (T)-2 * nu;
+        a.txx[o] = (c + c3) * (a.S11[o] + a.S11[q]);
+        a.txy[o] = (c + c3) * (a.S12[o] + a.S12[q]);
+        a.tyy[o] = (c + c3) * (a.S22[o] + a.S22[q]);
+        a.tzz[o] = (c + c3) * (a.S33[o] + a.S33[q]);
+        a.txz[o] = (c2 + c4) * a.S13[o];
+        a.tyz[o] = (c2 + c4) * a.S23[o];
+    }
+}
+
+// divstress_uv sum (divstress_uv.f90:58-83): k = 1..nz-1, all ld columns.
+template <typename T>
+__global__ void k_divt_uv(const T* ddx_a, const T* ddy_b, const T* tz, T* out, Geo g, T rdz) {
+    int j, i;
+    if (!ij_of(g.ld, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    if (i >= g.nx) {
+        out[o] = (T)0;
+        return;
+    }
+    const T dz = rdz * (tz[o + g.plane] - tz[o]);
+    out[o] = ddx_a[o] + ddy_b[o] + dz;
+}
+
+// divstress_w sum (divstress_w.f90:63-114): k = 1..nz.
+template <typename T>
+__global__ void k_divt_w(const T* dtxdx, const T* dtydy, const T* tzz, T* out, Geo g, T rdz) {
+    int j, i;
+    if (!ij_of(g.ld, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    if (i >= g.nx) {
+        if (k < g.nz) out[o] = (T)0;
+        return;
+    }
+    if (k == 1 || k == g.nz) {
+        out[o] = dtxdx[o] + dtydy[o];
+    } else {
+        const T dz = rdz * (tzz[o] - tzz[o - g.plane]);
+        out[o] = dtxdx[o] + dtydy[o] + dz;
+    }
+}
+
+// convec vorticity (convec.f90:97-168), scaled by 1/(nx ny), k = 1..nz.
+template <typename T>
+struct VortArgs {
+    const T *dudy, *dudz, *dvdx, *dvdz, *dwdx, *dwdy;
+    T *o1, *o2, *o3;
+    T c;
+};
+
+template <typename T>
+__global__ void k_vort(VortArgs<T> a, Geo g) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long P = g.plane;
+    const long o = (long)k * P + (long)j * g.ld + i;
+    if (k == 1) {
+        a.o1[o] = a.c * ((T)0.5 * (a.dwdy[o] + a.dwdy[o + P]) - a.dvdz[o]);
+        a.o2[o] = a.c * (a.dudz[o] - (T)0.5 * (a.dwdx[o] + a.dwdx[o + P]));
+    } else {
+        a.o1[o] = a.c * (a.dwdy[o] - a.dvdz[o]);
+        a.o2[o] = a.c * (a.dudz[o] - a.dwdx[o]);
+    }
+    a.o3[o] = a.c * (a.dvdx[o] - a.dudy[o]);
+}
+
+// convec products on the 3/2 grid (convec.f90:172-305), k = 1..nz.
+template <typename T>
+struct ProdArgs {
+    const T *ub, *vb, *wb, *o1, *o2, *o3;
+    T *cx, *cy, *cz;
+    T c;
+};
+
+template <typename T>
+__global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz) {
+    int j, i;
+    if (!ij_of(nx2, ny2, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long P = (long)ny2 * ldb;
+    const long o = (long)k * P + (long)j * ldb + i;
+    const T h = (T)0.5;
+    const T c = a.c;
+    if (k <= nz - 1) {
+        T x, y;
+        if (k == 1) {
+            x = c * (a.vb[o] * (-a.o3[o]) + h * a.wb[o + P] * a.o2[o + P]);
+            y = c * (a.ub[o] * a.o3[o] + h * a.wb[o + P] * (-a.o1[o + P]));
+        } else if (k == nz - 1) {
+            x = c * (a.vb[o] * (-a.o3[o]) + h * a.wb[o] * a.o2[o]);
+            y = c * (a.ub[o] * a.o3[o] + h * a.wb[o] * (-a.o1[o]));
+        } else {
+            x = c * (a.vb[o] * (-a.o3[o]) + h * (a.wb[o + P] * a.o2[o + P] + a.wb[o] * a.o2[o]));
+            y = c * (a.ub[o] * a.o3[o] +
+                     h * (a.wb[o + P] * (-a.o1[o + P]) + a.wb[o] * (-a.o1[o])));
+        }
+        a.cx[o] = x;
+        a.cy[o] = y;
+    }
+    T z = (T)0;
+    if (k >= 2 && k <= nz - 1) {
+        z = (c * h) * ((a.ub[o] + a.ub[o - P]) * (-a.o2[o]) + (a.vb[o] + a.vb[o - P]) * a.o1[o]);
+    }
+    a.cz[o] = z;
+}
+
+// RHS assembly, forcing and AB2 (main.f90:211-214, 228-232, 264-266, 287-296).
+template <typename T>
+struct Ab2Args {
+    T *u, *v, *w, *rx, *ry, *rz;
+    const T *rxf, *ryf, *rzf, *dx, *dy, *dz, *fx, *fy, *fz;
+    T dt, t1, t2, mpx, mpy;
+    int use_mpf;
+};
+
+template <typename T>
+__global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
+    int j, i;
+    if (!ij_of(g.ld, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    if (k <= g.nz - 1) {
+        T rx = -a.rx[o] - a.dx[o];
+        T ry = -a.ry[o] - a.dy[o];
+        T rz = -a.rz[o] - a.dz[o];
+        if (a.use_mpf) {
+            rx = rx + a.mpx;
+            ry = ry + a.mpy;
+        }
+        if (a.fx) {
+            rx = rx + a.fx[o];
+            ry = ry + a.fy[o];
+            rz = rz + a.fz[o];
+        }
+        a.rx[o] = rx;
+        a.ry[o] = ry;
+        a.rz[o] = rz;
+        a.u[o] = a.u[o] + a.dt * (a.t1 * rx + a.t2 * a.rxf[o]);
+        a.v[o] = a.v[o] + a.dt * (a.t1 * ry + a.t2 * a.ryf[o]);
+        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * a.rzf[o]);
+    } else {
+        const T rz = -a.rz[o] - a.dz[o];
+        a.rz[o] = rz;
+        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * a.rzf[o]);
+    }
+}
+
+// RHS -= grad p (main.f90:321-326), k = 1..nz (z only at nz).
+template <typename T>
+__global__ void k_rhs_gradp(T* rx, T* ry, T* rz, const T* px, const T* py, const T* pz, Geo g) {
+    int j, i;
+    if (!ij_of(g.ld, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    if (k <= g.nz - 1) {
+        rx[o] = rx[o] - px[o];
+        ry[o] = ry[o] - py[o];
+    }
+    rz[o] = rz[o] - pz[o];
+}
+
+// dpdz by differences (press_stag_array.f90:284-288), k = 1..nz.
+template <typename T>
+__global__ void k_dpdz(const T* p, T* dpdz, Geo g, T dzv) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    dpdz[o] = (p[o] - p[o - g.plane]) / dzv;
+}
+
+// project (forcing.f90:149-244), interior update; lid/wall BCs follow.
+template <typename T>
+__global__ void k_project(T* u, T* v, T* w, const T* px, const T* py, const T* pz, Geo g, T dt,
+                          T t1) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    u[o] = u[o] + dt * (-(t1 * px[o]));
+    v[o] = v[o] + dt * (-(t1 * py[o]));
+    if (k >= 2) w[o] = w[o] + dt * (-(t1 * pz[o]));
+}
+
+template <typename T>
+__global__ void k_copy_plane(const T* src, T* dst, long n) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) dst[idx] = src[idx];
+}
+
+// Block-partial sums for the diagnostics (rmsdiv.f90:39-50, io.f90:115-124).
+template <typename T>
+__global__ void k_rmsdiv_part(const T* dudx, const T* dvdy, const T* dwdz, Geo g, double* part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long n = (long)(g.nz - 1) * g.ny * g.nx;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long k = idx / ((long)g.ny * g.nx) + 1;
+        const long r = idx % ((long)g.ny * g.nx);
+        const long j = r / g.nx, i = r % g.nx;
+        const long o = k * g.plane + j * g.ld + i;
+        acc += fabs((double)dudx[o] + (double)dvdy[o] + (double)dwdz[o]);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+template <typename T>
+__global__ void k_energy_part(const T* u, const T* v, const T* w, Geo g, double* part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long n = (long)(g.nz - 1) * g.ny * g.nx;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long k = idx / ((long)g.ny * g.nx) + 1;
+        const long r = idx % ((long)g.ny * g.nx);
+        const long j = r / g.nx, i = r % g.nx;
+        const long o = k * g.plane + j * g.ld + i;
+        const double tw = 0.5 * ((double)w[o] + (double)w[o + g.plane]);
+        acc += (double)u[o] * u[o] + (double)v[o] * v[o] + tw * tw;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+template <typename T>
+__global__ void k_maxabs_part(const T* f, Geo g, double* part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long n = (long)(g.nz - 1) * g.ny * g.nx;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long k = idx / ((long)g.ny * g.nx) + 1;
+        const long r = idx % ((long)g.ny * g.nx);
+        const long j = r / g.nx, i = r % g.nx;
+        acc = fmax(acc, fabs((double)f[k * g.plane + j * g.ld + i]));
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+}  // namespace lg

@@ -1,0 +1,107 @@
+// Pressure Poisson solve in (kx, ky) space: RHS assembly, batched Thomas
+// and the (0,0) mode (press_stag_array.f90:149-250, tridag_array.f90:165-246).
+// Always computed in FP64, whatever the storage type of the spectra.
+#pragma once
+
+#include "common.hpp"
+
+namespace lg {
+
+struct TriGeo {
+    int ny, NK, nz;
+    long splane;         // ny * NK complex per level
+    double kxs, kys;     // 2 pi / Lx, 2 pi / Ly
+    double dz, const3, const4;
+};
+
+// One thread per (ky, kx < nx/2).  Hx, Hy hold levels 1..nz-1, Hz levels
+// 1..nz (all already scaled by 1/(nx ny tadv1 dt)); rb, rt the bottom/top
+// Neumann planes scaled by 1/(nx ny).  Writes p_hat for rows 0..nz and
+// uses gam and usc (FP64, (nz+1) * ny * NK each) as the Thomas scratch.
+template <typename T>
+__global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz,
+                         const cplx<T>* rb, const cplx<T>* rt, cplx<T>* ph, double* gam,
+                         double2* usc, TriGeo g, int* err) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.ny * g.NK) return;
+    const int jy = idx / g.NK;
+    const int jx = idx - jy * g.NK;
+    const long S = g.splane;
+    const int n = g.nz + 1;
+    auto ld = [](const cplx<T>* p, long o) -> double2 {
+        const cplx<T> v = p[o];
+        return make_double2((double)v.x, (double)v.y);
+    };
+    auto st = [&](long o, double2 v) { ph[o] = cplx<T>{(T)v.x, (T)v.y}; };
+    if (jy == g.ny / 2) {
+        for (int m = 0; m < n; ++m) st((long)m * S + idx, make_double2(0.0, 0.0));
+        return;
+    }
+    if (jx == 0 && jy == 0) {
+        // zero-wavenumber mode: p(0) = 0, p(1) = -dz rb, p(k) = p(k-1) + Hz(k) dz
+        double2 p0 = make_double2(0.0, 0.0);
+        st(idx, p0);
+        const double2 b = ld(rb, idx);
+        double2 pk = make_double2(p0.x - g.dz * b.x, p0.y - g.dz * b.y);
+        st(S + idx, pk);
+        for (int m = 2; m <= g.nz; ++m) {
+            const double2 h = ld(Hz, (long)m * S + idx);
+            pk = make_double2(pk.x + h.x * g.dz, pk.y + h.y * g.dz);
+            st((long)m * S + idx, pk);
+        }
+        return;
+    }
+    const double kx = g.kxs * (double)jx;
+    const int my = (jy < g.ny / 2) ? jy : jy - g.ny;
+    const double ky = g.kys * (double)my;
+    const double a_in = g.const3, c_in = g.const3;
+    const double b_in = -(kx * kx + ky * ky + 2.0 * g.const3);
+    // row 0: b = -1, c = 1, r = -dz rb
+    double2 r0 = ld(rb, idx);
+    double2 u = make_double2((-g.dz * r0.x) / -1.0, (-g.dz * r0.y) / -1.0);
+    usc[idx] = u;
+    double bet = -1.0;
+    double cprev = 1.0;
+    for (int m = 1; m <= g.nz; ++m) {
+        double a, b, cc;
+        double2 r;
+        if (m < g.nz) {
+            a = a_in;
+            b = b_in;
+            cc = c_in;
+            const double2 hx = ld(Hx, (long)m * S + idx);
+            const double2 hy = ld(Hy, (long)m * S + idx);
+            const double2 z1 = ld(Hz, (long)(m + 1) * S + idx);
+            const double2 z0 = ld(Hz, (long)m * S + idx);
+            // i kx Hx + i ky Hy + (Hz(m+1) - Hz(m)) / dz
+            const double ahx_r = -hx.y * kx, ahx_i = hx.x * kx;
+            const double ahy_r = -hy.y * ky, ahy_i = hy.x * ky;
+            r.x = (ahx_r + ahy_r) + (z1.x - z0.x) * g.const4;
+            r.y = (ahx_i + ahy_i) + (z1.y - z0.y) * g.const4;
+        } else {
+            a = -1.0;
+            b = 1.0;
+            cc = 0.0;
+            const double2 t = ld(rt, idx);
+            r = make_double2(-g.dz * t.x, -g.dz * t.y);
+        }
+        const double gm = cprev / bet;
+        gam[(long)m * S + idx] = gm;
+        bet = b - a * gm;
+        if (bet == 0.0) atomicExch(err, 1);
+        u = make_double2((r.x - a * u.x) / bet, (r.y - a * u.y) / bet);
+        usc[(long)m * S + idx] = u;
+        cprev = cc;
+    }
+    st((long)g.nz * S + idx, u);
+    double2 up = u;
+    for (int m = g.nz - 1; m >= 0; --m) {
+        const double gm = gam[(long)(m + 1) * S + idx];
+        double2 um = usc[(long)m * S + idx];
+        um = make_double2(um.x - gm * up.x, um.y - gm * up.y);
+        st((long)m * S + idx, um);
+        up = um;
+    }
+}
+
+}  // namespace lg

@@ -1,0 +1,77 @@
+// Optional per-kernel-class timing with HIP events recorded on the case's
+// stream around each launch (used by bench.py for the roofline figure).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace lg {
+
+enum KernelKind : int {
+    KK_ROWS_R2C = 0,
+    KK_ROWS_C2R = 1,
+    KK_COLS_FWD = 2,
+    KK_COLS_INV = 3,
+    KK_TRIDAG = 4,
+    KK_NKINDS = 5,
+};
+
+struct KernelTimer {
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool on = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            LG_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+
+    template <class F>
+    void run(hipStream_t s, int kind, double bytes, F&& launch) {
+        if (!on) {
+            launch();
+            return;
+        }
+        hipEvent_t a = get(), b = get();
+        LG_HIP(hipEventRecord(a, s));
+        launch();
+        LG_HIP(hipEventRecord(b, s));
+        recs.push_back({kind, a, b, bytes});
+    }
+
+    void reset() {
+        recs.clear();
+        used = 0;
+    }
+
+    // total milliseconds, launch count and algorithmic bytes of one kind
+    void report(int kind, double& ms, double& n, double& bytes) {
+        ms = n = bytes = 0.0;
+        for (auto& r : recs) {
+            if (r.kind != kind) continue;
+            float t = 0.f;
+            LG_HIP(hipEventSynchronize(r.b));
+            LG_HIP(hipEventElapsedTime(&t, r.a, r.b));
+            ms += t;
+            n += 1.0;
+            bytes += r.bytes;
+        }
+    }
+
+    ~KernelTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace lg

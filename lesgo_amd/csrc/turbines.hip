@@ -1,0 +1,290 @@
+// Actuator-disk model (turbines.f90:123-638, turbine_indicator.f90:43-169).
+// Initialisation (indicator table, node lists) runs once on the host; the
+// per-step disk average, thrust and force scatter run on the device.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "case.hpp"
+#include "pointwise.hpp"
+
+namespace lg {
+
+static constexpr double PI_T = 3.1415926535897932384626433;
+
+// turb_ind_func_t (turbine_indicator.f90:43-169).  R2 is row 0 of the
+// circular convolution of the Gaussian g (width delta2) with the unit-disk
+// indicator h on an N x N grid.  The Gaussian factorises, so the row is
+// evaluated as sum_l e_l H1[(j - l) mod N] with H1[m] = sum_k e_k
+// h[(-k) mod N][m]: exact in real arithmetic, O(N^2).
+struct Indicator {
+    double ell, d1, d2, thk, s6d1, M;
+    std::vector<double> r, R2;
+
+    Indicator(double delta1, double delta2, double thk_in, double dia, int N = 2048) {
+        ell = 0.5 * dia;
+        d1 = delta1 / ell;
+        d2 = delta2 / ell;
+        thk = thk_in / ell;
+        s6d1 = std::sqrt(6.0) / d1;
+        const double L = 1 + 10 * d2 / std::sqrt(12.0);
+        const double d = 2 * L / N;
+        std::vector<double> yz(N), e(N);
+        for (int i = 0; i < N; ++i) {
+            yz[i] = -L + d * ((i + 1) - 0.5);
+            e[i] = std::exp(-6 * (yz[i] * yz[i]) / (d2 * d2));
+        }
+        const double cg = 6.0 / (PI_T * d2 * d2);
+        std::vector<double> H1(N, 0.0);
+        for (int k = 0; k < N; ++k) {
+            const int a = (N - k) % N;
+            const double ya2 = yz[a] * yz[a];
+            double* out = H1.data();
+            for (int m = 0; m < N; ++m) {
+                if (std::sqrt(ya2 + yz[m] * yz[m]) < 1) out[m] += e[k] * (1.0 / PI_T);
+            }
+        }
+        r.resize(N / 2);
+        R2.resize(N / 2);
+        for (int j = 0; j < N / 2; ++j) {
+            double s = 0.0;
+            for (int l = 0; l < N; ++l) s += e[l] * H1[(j - l + N) % N];
+            R2[j] = d * d * cg * s;
+            r[j] = yz[j] + L;
+        }
+        const double dr = r[1] - r[0];
+        double norm = 0.0;
+        for (int j = 0; j < N / 2; ++j) norm += 2 * PI_T * r[j] * R2[j] * dr;
+        for (auto& v : R2) v /= norm;
+        double m = 0.0;
+        for (int j = 0; j < N / 2; ++j) m += 2 * PI_T * r[j] * R2[j] * R2[j] * dr;
+        M = m * PI_T;
+    }
+
+    // linear_interp_sa + binary_search (functions.f90:711-860)
+    double interp(double xq) const {
+        const int N = (int)r.size();
+        if (xq < r[0]) return R2[0];
+        if (xq > r[N - 1]) return R2[N - 1];
+        int low = 1, high = N;          // 1-based, as binary_search
+        while (high - low > 1) {
+            const int mid = (low + high) / 2;
+            if (r[mid - 1] > xq) high = mid;
+            else if (r[mid - 1] < xq) low = mid;
+            else {
+                low = mid;
+                break;
+            }
+        }
+        if (low == N) return R2[N - 1];
+        const int i = low - 1;
+        return R2[i] + (xq - r[i]) * (R2[i + 1] - R2[i]) / (r[i + 1] - r[i]);
+    }
+
+    double val(double rr_in, double x_in) const {
+        const double rr = rr_in / ell;
+        const double xx = x_in / ell;
+        const double r2 = interp(rr);
+        const double R1 = 0.5 / thk * (std::erf(s6d1 * (xx + 0.5 * thk)) -
+                                       std::erf(s6d1 * (xx - 0.5 * thk)));
+        return R1 * r2 / (ell * ell * ell);
+    }
+};
+
+static int fnint(double x) {
+    return (int)std::floor(std::fabs(x) + 0.5) * (x >= 0 ? 1 : -1);
+}
+
+// Disk-averaged velocity, thrust (turbines.f90:517-580): one block per
+// turbine; T_avg_dim <= 0 so eps = 1.
+template <typename T>
+__global__ void k_turb_reduce(const T* u, const T* v, const T* w, const int* node_lin,
+                              const double* node_ind, const int* off, const double* coef,
+                              double* state, long plane, double dV) {
+    __shared__ double sh[256];
+    const int s = blockIdx.x;
+    const double n1 = coef[6 * s + 0], n2 = coef[6 * s + 1], n3 = coef[6 * s + 2];
+    double acc = 0.0;
+    for (int l = off[s] + threadIdx.x; l < off[s + 1]; l += blockDim.x) {
+        const long o = node_lin[l];
+        const double wuv = 0.5 * ((double)w[o + plane] + (double)w[o]);
+        acc += dV * node_ind[l] * (n1 * (double)u[o] + n2 * (double)v[o] + n3 * wuv);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double ct = coef[6 * s + 3], dia = coef[6 * s + 4], corr = coef[6 * s + 5];
+        double ud = sh[0];
+        ud = ud / corr;
+        const double eps = 1.0;
+        const double udt = (1. - eps) * state[3 * s + 1] + eps * ud;
+        const double fn = -0.5 * ct * fabs(udt) * udt * 0.25 * 3.1415926535897932384626433 * (dia * dia);
+        state[3 * s + 0] = ud;
+        state[3 * s + 1] = udt;
+        state[3 * s + 2] = fn;
+    }
+}
+
+// Scatter f_n n ind to the nodes (turbines.f90:597-617).
+template <typename T>
+__global__ void k_turb_scatter(T* fx, T* fy, T* fzu, const int* node_lin, const double* node_ind,
+                               const int* node_tid, const double* coef, const double* state,
+                               int total) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= total) return;
+    const int s = node_tid[l];
+    const double fn = state[3 * s + 2];
+    const long o = node_lin[l];
+    fx[o] = (T)(fn * coef[6 * s + 0] * node_ind[l]);
+    fy[o] = (T)(fn * coef[6 * s + 1] * node_ind[l]);
+    fzu[o] = (T)(fn * coef[6 * s + 2] * node_ind[l]);
+}
+
+// fza = interp_to_w_grid(fza) (functions.f90:131-185), k = 1..nz.
+template <typename T>
+__global__ void k_interp_w(const T* src, T* dst, Geo g) {
+    int j, i;
+    if (!ij_of(g.ld, g.ny, j, i)) return;
+    const int k = blockIdx.y + 1;
+    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    dst[o] = (T)0.5 * (src[o - g.plane] + src[o]);
+}
+
+template <typename T>
+void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, double alpha2,
+                   double filter_cutoff, int adm_correction) {
+    const Dims& d = c.d;
+    auto ts = std::make_unique<TurbineSet>();
+    ts->n = nturb;
+    ts->adm_correction = adm_correction != 0;
+    ts->spec.assign(spec8, spec8 + 8 * nturb);
+    const double h = std::sqrt(d.dx * d.dx + d.dy * d.dy + d.dz * d.dz);
+    const double delta1 = alpha1 * h, delta2 = alpha2 * h;
+    std::vector<int> lin, tid, off(1, 0);
+    std::vector<double> ind, coef;
+    const int nx = d.nx, ny = d.ny;
+    for (int s = 0; s < nturb; ++s) {
+        const double* t = spec8 + 8 * s;
+        const double xloc = t[0], yloc = t[1], height = t[2], dia = t[3];
+        const double thk = std::max(t[4], d.dx * 1.01);
+        const double th1 = t[5], th2 = t[6], ctp = t[7];
+        Indicator itf(delta1, delta2, thk, dia);
+        ts->M.push_back(itf.M);
+        const double search_rad = 0.5 * dia + 3 * std::max(alpha1, alpha2) * h;
+        const int imax = std::min((int)(search_rad / d.dx + 2), nx / 2);
+        const int jmax = std::min((int)(search_rad / d.dy + 2), ny / 2);
+        const double n1 = -std::cos(PI_T * th1 / 180.) * std::cos(PI_T * th2 / 180.);
+        const double n2 = -std::sin(PI_T * th1 / 180.) * std::cos(PI_T * th2 / 180.);
+        const double n3 = std::sin(PI_T * th2 / 180.);
+        const int icp = fnint(xloc / d.dx);
+        const int jcp = fnint(yloc / d.dy);
+        const double filt_max = itf.val(0.0, 0.0);
+        std::vector<double> filt_l;
+        std::vector<int> lin_l;
+        double sumA = 0.0;
+        for (int k = 1; k <= d.nz - 1; ++k) {
+            for (int j = jcp - jmax + 1; j <= jcp + jmax; ++j) {
+                for (int i = icp - imax + 1; i <= icp + imax; ++i) {
+                    int i2, j2;
+                    double rx, ry;
+                    if (i < 1) {
+                        i2 = ((i + nx - 1) % nx + nx) % nx + 1;
+                        rx = ((i2 - 1) * d.dx - d.prm.lx) - xloc;
+                    } else if (i > nx) {
+                        i2 = (i + nx - 1) % nx + 1;
+                        rx = (d.prm.lx + (i2 - 1) * d.dx) - xloc;
+                    } else {
+                        i2 = i;
+                        rx = (i - 1) * d.dx - xloc;
+                    }
+                    if (j < 1) {
+                        j2 = ((j + ny - 1) % ny + ny) % ny + 1;
+                        ry = ((j2 - 1) * d.dy - d.prm.ly) - yloc;
+                    } else if (j > ny) {
+                        j2 = (j + ny - 1) % ny + 1;
+                        ry = (d.prm.ly + (j2 - 1) * d.dy) - yloc;
+                    } else {
+                        j2 = j;
+                        ry = (j - 1) * d.dy - yloc;
+                    }
+                    const double rz = (k - 0.5) * d.dz - height;
+                    const double r = std::sqrt(rx * rx + ry * ry + rz * rz);
+                    const double r_norm = std::fabs(rx * n1 + ry * n2 + rz * n3);
+                    const double r_disk = std::sqrt(r * r - r_norm * r_norm);
+                    const double f = itf.val(r_disk, r_norm);
+                    if (f > filter_cutoff * filt_max) {
+                        filt_l.push_back(f);
+                        lin_l.push_back((int)(((long)k * ny + (j2 - 1)) * d.ld + (i2 - 1)));
+                        sumA = sumA + f * d.dx * d.dy * d.dz;
+                    }
+                }
+            }
+        }
+        for (size_t l = 0; l < filt_l.size(); ++l) {
+            ind.push_back(filt_l[l] / sumA);
+            lin.push_back(lin_l[l]);
+            tid.push_back(s);
+        }
+        off.push_back((int)lin.size());
+        const double corr = ts->adm_correction ? (1 + 0.25 * (1 - itf.M) * ctp) : 1.0;
+        coef.insert(coef.end(), {n1, n2, n3, ctp, dia, corr});
+    }
+    ts->total_nodes = (int)lin.size();
+    ts->h_off = off;
+    auto up = [&](auto*& dptr, const auto& hv) {
+        using E = typename std::decay<decltype(hv[0])>::type;
+        LG_HIP(hipMalloc(&dptr, std::max<size_t>(1, hv.size()) * sizeof(E)));
+        if (!hv.empty())
+            LG_HIP(hipMemcpy(dptr, hv.data(), hv.size() * sizeof(E), hipMemcpyHostToDevice));
+    };
+    up(ts->node_lin, lin);
+    up(ts->node_ind, ind);
+    up(ts->node_tid, tid);
+    up(ts->off, off);
+    up(ts->coef, coef);
+    std::vector<double> st(3 * nturb);
+    for (int s = 0; s < nturb; ++s) {
+        st[3 * s + 0] = 0.0;
+        st[3 * s + 1] = -1.0;      // u_d_T without a u_d_T.dat (turbines.f90:245-250)
+        st[3 * s + 2] = 0.0;
+    }
+    up(ts->state, st);
+    ts->has_fz = false;
+    for (int s = 0; s < nturb; ++s)
+        if (coef[6 * s + 2] != 0.0) ts->has_fz = true;
+    LG_HIP(hipStreamSynchronize(c.stream));
+    for (int id : {LG_FXA, LG_FYA, LG_FZA})
+        LG_HIP(hipMemsetAsync(c.fld[id], 0, d.field * sizeof(T), c.stream));
+    c.turb = std::move(ts);
+}
+
+template <typename T>
+void Case<T>::turbines_forcing() {
+    if (!turb) return;
+    TurbineSet& t = *turb;
+    if (t.total_nodes == 0) return;
+    hipLaunchKernelGGL(k_turb_reduce<T>, dim3(t.n), dim3(256), 0, stream, fld[LG_U], fld[LG_V],
+                       fld[LG_W], t.node_lin, t.node_ind, t.off, t.coef, t.state, d.plane,
+                       d.dx * d.dy * d.dz);
+    // the node set is static, so fxa/fya/fza are zero off the nodes after init
+    T* fz_uv = t.has_fz ? tmp[8] : fld[LG_FZA];
+    hipLaunchKernelGGL(k_turb_scatter<T>, dim3((t.total_nodes + 255) / 256), dim3(256), 0, stream,
+                       fld[LG_FXA], fld[LG_FYA], fz_uv, t.node_lin, t.node_ind, t.node_tid,
+                       t.coef, t.state, t.total_nodes);
+    if (t.has_fz) {
+        Geo g{d.nx, d.ny, d.nz, d.ld, d.plane};
+        hipLaunchKernelGGL(k_interp_w<T>, dim3((unsigned)((d.ld * d.ny + 255) / 256), d.nz),
+                           dim3(256), 0, stream, tmp[8], fld[LG_FZA], g);
+    }
+}
+
+template void turbines_init<double>(Case<double>&, int, const double*, double, double, double, int);
+template void turbines_init<float>(Case<float>&, int, const double*, double, double, double, int);
+template void Case<double>::turbines_forcing();
+template void Case<float>::turbines_forcing();
+
+}  // namespace lg

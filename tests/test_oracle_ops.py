@@ -245,3 +245,30 @@ def test_config0_step_is_divergence_free():
     s, snaps = o.run(p, 12, seed=0, every=5)
     assert snaps[10][4] < 1e-12
     assert np.isfinite(s.u[1:p.nz]).all()
+
+
+def test_lagrange_floor_sensitivity_is_intrinsic():
+    """Documents the conditioning of the reference algorithm that bounds
+    achievable parity in the actuator-disk case: a 1e-14 relative
+    perturbation of u at step 5 stays ~1e-14 until the next dynamic update
+    (step 10), where Tn = 1.5 delta max(F_LM F_MM, 1e-24)^(-1/8) near the
+    clip floor amplifies it by orders of magnitude."""
+    p = o.Params(48, 32, 24)
+    specs = o.aligned_farm(p, 2, 2, dia=0.3, hub=0.3, sx_d=7, sy_d=5)
+    sp = o.Spectral(p)
+    s = o.initial_condition(p, 0)
+    s.turbines = o.Turbines(p, specs)
+    for _ in range(5):
+        o.step(sp, s)
+    t = s.copy()
+    rng = np.random.default_rng(0)
+    t.u[1:p.nz] *= 1 + 1e-14 * rng.standard_normal(t.u[1:p.nz].shape)
+    X = np.s_[1:p.nz, :, :p.nx]
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    for _ in range(4):
+        o.step(sp, s)
+        o.step(sp, t)
+    assert rel(t.u[X], s.u[X]) < 1e-13
+    o.step(sp, s)
+    o.step(sp, t)
+    assert rel(t.u[X], s.u[X]) > 1e-11
