@@ -139,7 +139,18 @@ def main():
     ap.add_argument("--workload", default="windfarm256", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import ctypes
+        from lesgo_amd import _lib
+        ms = ctypes.c_double()
+        nbytes = args.calibrate_copy * (1 << 20)
+        _lib.check(_lib.lib().lg_calibrate_copy(nbytes, 5, ctypes.byref(ms)), "calibrate_copy")
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms.value,
+                          "copy_GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one case per GPU with no sharding; "
                                    "run with --gpus 1 (multi-GPU scaling is not built)"}))

@@ -81,6 +81,13 @@ int lg_sizes(const lg_case* c, int* nz, int* ld, int* ny);
 /* Host <-> device copies of one field, host layout double [nz+1][ny][ld]. */
 int lg_upload(lg_case* c, int field, const double* host);
 int lg_download(lg_case* c, int field, double* host);
+/* Seeded start written straight into the case (replaces the /dev/urandom
+ * ic_les, initial.f90:554, with the reproducible vel.out-style state):
+ * u = ln(z/z0)/vonk + U(-noise, noise) ln(z/z0)/vonk on uv levels 1..nz-1,
+ * v, w get the same-amplitude noise, w = 0 at the wall and the lid,
+ * u(nz) = u(nz-1), v(nz) = v(nz-1), Cs_opt2 = 0.03, everything else 0, level 0
+ * BOGUS.  Uniforms: splitmix64 of (seed, field, i, j, k); counters reset. */
+int lg_seed_state(lg_case* c, long seed, double noise);
 /* Time counters jt, jt_total (param.f90:146-147). */
 int lg_set_counters(lg_case* c, int jt, int jt_total);
 int lg_get_counters(const lg_case* c, int* jt, int* jt_total);
@@ -114,6 +121,15 @@ int lg_convec(lg_case* c);
 int lg_press_stag_array(lg_case* c);
 /* forcing::project (forcing.f90:149-244) */
 int lg_project(lg_case* c);
+/* Inline blocks of the time loop in main.f90, exposed so a host driver can
+ * mirror the loop call by call:
+ *   lg_copy_field        RHSx_f = RHSx etc. (main.f90:155-157)
+ *   lg_rhs_ab2           RHS = -RHS - divt + mean_p_force + f_adm; AB2 u*;
+ *                        BOGUS fills (main.f90:211-214,228-232,264-308)
+ *   lg_rhs_gradp         RHS -= grad p (main.f90:321-326) */
+int lg_copy_field(lg_case* c, int src, int dst);
+int lg_rhs_ab2(lg_case* c);
+int lg_rhs_gradp(lg_case* c);
 /* One or more whole time steps (main.f90:146-344). */
 int lg_step(lg_case* c, int nsteps);
 
@@ -149,6 +165,11 @@ enum {
 };
 int lg_timing_enable(lg_case* c, int on);
 int lg_timing_report(lg_case* c, int kind, double* ms, double* launches, double* bytes);
+
+/* Plain device-to-device copy of nbytes (a multiple of 16) repeated reps
+ * times: calibrates HBM byte counters and measures copy bandwidth.  No
+ * counterpart in the reference. */
+int lg_calibrate_copy(long nbytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

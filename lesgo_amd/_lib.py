@@ -42,6 +42,7 @@ SIGNATURES = {
     "lg_sizes": (C.c_int, [_P, _I, _I, _I]),
     "lg_upload": (C.c_int, [_P, C.c_int, _D]),
     "lg_download": (C.c_int, [_P, C.c_int, _D]),
+    "lg_seed_state": (C.c_int, [_P, C.c_long, C.c_double]),
     "lg_set_counters": (C.c_int, [_P, C.c_int, C.c_int]),
     "lg_get_counters": (C.c_int, [_P, _I, _I]),
     "lg_synchronize": (C.c_int, [_P]),
@@ -60,6 +61,9 @@ SIGNATURES = {
     "lg_convec": (C.c_int, [_P]),
     "lg_press_stag_array": (C.c_int, [_P]),
     "lg_project": (C.c_int, [_P]),
+    "lg_copy_field": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_rhs_ab2": (C.c_int, [_P]),
+    "lg_rhs_gradp": (C.c_int, [_P]),
     "lg_step": (C.c_int, [_P, C.c_int]),
     "lg_rmsdiv": (C.c_int, [_P, _D]),
     "lg_energy": (C.c_int, [_P, _D]),
@@ -70,6 +74,7 @@ SIGNATURES = {
     "lg_turbines_get": (C.c_int, [_P, _D, _D, C.c_int]),
     "lg_timing_enable": (C.c_int, [_P, C.c_int]),
     "lg_timing_report": (C.c_int, [_P, C.c_int, _D, _D, _D]),
+    "lg_calibrate_copy": (C.c_int, [C.c_long, C.c_int, _D]),
 }
 
 KERNEL_KINDS = ("rows_r2c", "rows_c2r", "cols_fwd", "cols_inv", "tridag")

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblesgo_amd.so")
-SOURCES = ["case.hip", "turbines.hip", "capi.hip"]
+SOURCES = ["case.hip", "turbines.hip", "capi.hip", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -32,6 +32,28 @@ def _compile(src: str) -> str:
     return obj
 
 
+FSRC = os.path.join(HERE, "fortran")
+FORTRAN = ["lesgo_amd_c.f90", "lesgo_state.f90", "lesgo_gpu.f90", "main.f90"]
+DRIVER = os.path.join(HERE, "lesgo_amd_main")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_driver(verbose: bool = True) -> str:
+    """Fortran driver (amdflang) linked against liblesgo_amd.so."""
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [FC, "-O2", "-module-dir", mod, *[os.path.join(FSRC, f) for f in FORTRAN],
+           "-L" + HERE, "-llesgo_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
+           "-o", DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr[-6000:]}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -44,6 +66,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_driver(verbose)
     return LIB
 
 

@@ -98,6 +98,10 @@ class Case:
         check(lib().lg_download(self._h, _fid(name), a.ctypes.data_as(_lib._D)), f"download {name}")
         return a
 
+    def seed_state(self, seed: int, noise: float = 0.1):
+        """Write the seeded start on the device (lg_seed_state)."""
+        check(lib().lg_seed_state(self._h, seed, noise), "seed_state")
+
     def set_counters(self, jt: int, jt_total: int):
         check(lib().lg_set_counters(self._h, jt, jt_total), "set_counters")
 
@@ -160,6 +164,16 @@ class Case:
 
     def project(self):
         check(lib().lg_project(self._h), "project")
+
+    def copy_field(self, src, dst):
+        check(lib().lg_copy_field(self._h, _fid(src), _fid(dst)), "copy_field")
+
+    def rhs_ab2(self):
+        """RHS assembly, forcing and AB2 (the inline block of main.f90:211-308)."""
+        check(lib().lg_rhs_ab2(self._h), "rhs_ab2")
+
+    def rhs_gradp(self):
+        check(lib().lg_rhs_gradp(self._h), "rhs_gradp")
 
     def step(self, nsteps: int = 1):
         check(lib().lg_step(self._h, nsteps), "step")

@@ -810,10 +810,15 @@ void Case<T>::step() {
     if (turb) turbines_forcing();
     rhs_ab2();
     press_stag_array();
+    rhs_gradp();
+    project();
+}
+
+template <typename T>
+void Case<T>::rhs_gradp() {
     hipLaunchKernelGGL(k_rhs_gradp<T>, grid_ij(d.ld, d.ny, d.nz), dim3(BLK), 0, stream,
                        fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ], fld[LG_DPDX], fld[LG_DPDY],
                        fld[LG_DPDZ], geo_of(d));
-    project();
 }
 
 // ---------------------------------------------------------------------------

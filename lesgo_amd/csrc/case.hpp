@@ -126,6 +126,7 @@ struct Case {
     void press_stag_array();
     void project();
     void rhs_ab2();
+    void rhs_gradp();
     void step();
     void turbines_forcing();
     double rmsdiv();

@@ -1,0 +1,68 @@
+!*******************************************************************************
+module types
+!*******************************************************************************
+! Precision kinds, as the reference's types module (types.f90:21-37).
+implicit none
+integer, parameter :: rprec = kind(1.d0)
+end module types
+
+!*******************************************************************************
+module param
+!*******************************************************************************
+! The subset of the reference's param module (param.f90) the time step reads,
+! with the same names; set by the driver's input, derived sizes follow
+! input_util.f90:196-235 for one process (lbz = 0, nz = Nz + 1).
+use types, only : rprec
+implicit none
+save
+integer, parameter :: lbz = 0
+real(rprec), parameter :: BOGUS = -1234567890._rprec
+real(rprec), parameter :: pi = 3.1415926535897932384626433_rprec
+integer :: Nx = 32, Ny = 32, Nz = 32       ! Nz: uv levels in the input; nz below
+integer :: nz_tot, nx2, ny2, lh, ld, lh_big, ld_big
+real(rprec) :: L_x = 2._rprec*pi, L_y = pi, L_z = 1._rprec
+real(rprec) :: dx, dy, dz
+real(rprec) :: dt = 2.0e-4_rprec, zo = 1.0e-4_rprec, vonk = 0.4_rprec
+integer :: cs_count = 5
+logical :: use_mean_p_force = .true.
+real(rprec) :: mean_p_force_x = 1._rprec, mean_p_force_y = 0._rprec
+real(rprec) :: tadv1 = 1.5_rprec, tadv2 = -0.5_rprec
+integer :: nsteps = 100, wbase = 10, domain_nskip = 10
+integer :: jt = 0, jt_total = 0
+real(rprec) :: total_time = 0._rprec
+integer :: precision = 0                     ! 0 FP64, 1 mixed
+integer :: seed = 0
+logical :: fused = .false.                   ! .true.: whole steps via lg_step
+character(len=256) :: path = './'
+end module param
+
+!*******************************************************************************
+module sim_param
+!*******************************************************************************
+! Host mirrors of the reference's sim_param arrays (sim_param.f90:31-93),
+! (ld, ny, lbz:nz).  The device copies are the working state; these are
+! filled on demand by gpu_download.
+use types, only : rprec
+implicit none
+save
+real(rprec), dimension(:,:,:), allocatable, target :: u, v, w,                 &
+    dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz,                      &
+    RHSx, RHSy, RHSz, RHSx_f, RHSy_f, RHSz_f, dpdx, dpdy, dpdz,                &
+    txx, txy, tyy, txz, tyz, tzz, divtx, divty, divtz, p
+contains
+subroutine sim_param_init()
+use param, only : ld, ny, nz, lbz
+allocate(u(ld,ny,lbz:nz), v(ld,ny,lbz:nz), w(ld,ny,lbz:nz))
+allocate(dudx(ld,ny,lbz:nz), dudy(ld,ny,lbz:nz), dudz(ld,ny,lbz:nz))
+allocate(dvdx(ld,ny,lbz:nz), dvdy(ld,ny,lbz:nz), dvdz(ld,ny,lbz:nz))
+allocate(dwdx(ld,ny,lbz:nz), dwdy(ld,ny,lbz:nz), dwdz(ld,ny,lbz:nz))
+allocate(RHSx(ld,ny,lbz:nz), RHSy(ld,ny,lbz:nz), RHSz(ld,ny,lbz:nz))
+allocate(RHSx_f(ld,ny,lbz:nz), RHSy_f(ld,ny,lbz:nz), RHSz_f(ld,ny,lbz:nz))
+allocate(dpdx(ld,ny,lbz:nz), dpdy(ld,ny,lbz:nz), dpdz(ld,ny,lbz:nz))
+allocate(txx(ld,ny,lbz:nz), txy(ld,ny,lbz:nz), tyy(ld,ny,lbz:nz))
+allocate(txz(ld,ny,lbz:nz), tyz(ld,ny,lbz:nz), tzz(ld,ny,lbz:nz))
+allocate(divtx(ld,ny,lbz:nz), divty(ld,ny,lbz:nz), divtz(ld,ny,lbz:nz))
+allocate(p(ld,ny,0:nz))
+u = 0; v = 0; w = 0; p = 0
+end subroutine sim_param_init
+end module sim_param
