@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "case.hpp"
+#include "fft2.hpp"
 #include "lagrange.hpp"
 #include "pointwise.hpp"
 #include "pressure.hpp"
@@ -14,6 +15,70 @@ namespace lg {
 static constexpr double PI_D = 3.1415926535897932384626433;   // param.f90:94
 static constexpr int BLK = 256;
 static constexpr size_t LDS_MAX = 64 * 1024;
+
+#define LG_SPLITS(X) X(4, 4) X(4, 6) X(4, 8) X(6, 8) X(8, 8) X(8, 9) X(8, 12) X(8, 16) \
+    X(12, 12) X(12, 16) X(16, 16) X(16, 24)
+
+template <typename T>
+static bool v2_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        hipLaunchKernelGGL((k_rows_r2c_v2<T, A, B>), g, dim3(BLK), lds, s, a);           \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+template <typename T>
+static bool v2_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        hipLaunchKernelGGL((k_rows_c2r_v2<T, A, B>), g, dim3(BLK), lds, s, a);           \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+template <typename T, int DIR>
+static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        hipLaunchKernelGGL((k_cols_v2<T, A, B, DIR>), g, dim3(BLK), lds, s, a);          \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+// Launch a column pass: the register four-step kernel when the length has a
+// split, the generic LDS kernel otherwise.  Sets c.TC / c.stride.
+template <typename T, int DIR>
+static void launch_cols(ColArgs<T>& c, int njobs, int nlev, hipStream_t s) {
+    const int N = c.plan.N;
+    int n1, n2;
+    if (split_of(N, n1, n2)) {
+        const int YS = n1 * (n2 + 1) + 1;
+        int TC = 16;
+        while (TC > 1 && (size_t)TC * YS * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
+        c.TC = std::min(TC, c.ncols);
+        c.stride = YS;
+        const size_t lds = (size_t)c.TC * YS * sizeof(cplx<T>);
+        dim3 grid(njobs, (c.ncols + c.TC - 1) / c.TC, nlev);
+        if (v2_cols<T, DIR>(n1, n2, grid, lds, s, c)) return;
+    }
+    int TC = 16;
+    while (TC > 1 && (size_t)TC * (N + 1) * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
+    c.TC = std::min(TC, c.ncols);
+    c.stride = N + 1;
+    const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
+    dim3 grid(njobs, (c.ncols + c.TC - 1) / c.TC, nlev);
+    hipLaunchKernelGGL((k_cols<T, DIR>), grid, dim3(BLK), lds, s, c);
+}
 
 Dims::Dims(const lg_params& p) : prm(p) {
     nx = p.nx;
@@ -228,14 +293,28 @@ static void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const
         a.ncols = ncols;
         a.lev0_src = src_lev0;
         a.lev0_dst = dst_lev0;
-        a.stride = pl.N + 1;
-        a.S = rows_per_wg<T>(pl.N, nrows / 2);
-        const size_t lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        int n1, n2;
+        const bool v2 = split_of(pl.N, n1, n2);
+        size_t lds;
+        if (v2) {
+            const int YS = n1 * (n2 + 1) + 1;
+            int S = std::min(std::min(BLK / n1, 16), nrows / 2);
+            while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_MAX) --S;
+            a.S = S;
+            a.stride = YS;
+            lds = (size_t)a.S * YS * sizeof(cplx<T>);
+        } else {
+            a.stride = pl.N + 1;
+            a.S = rows_per_wg<T>(pl.N, nrows / 2);
+            lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        }
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev, nj);
         const double bytes = (double)nj * nlev * nrows * ((double)pl.N * sizeof(T) +
                                                            (double)ncols * sizeof(cplx<T>));
-        tm.run(s, KK_ROWS_R2C, bytes,
-               [&] { hipLaunchKernelGGL(k_rows_r2c<T>, grid, dim3(BLK), lds, s, a); });
+        tm.run(s, KK_ROWS_R2C, bytes, [&] {
+            if (!(v2 && v2_r2c<T>(n1, n2, grid, lds, s, a)))
+                hipLaunchKernelGGL(k_rows_r2c<T>, grid, dim3(BLK), lds, s, a);
+        });
     }
 }
 
@@ -259,14 +338,28 @@ static void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs,
         a.ncols = ncols;
         a.lev0_src = src_lev0;
         a.lev0_dst = dst_lev0;
-        a.stride = pl.N + 1;
-        a.S = rows_per_wg<T>(pl.N, nrows / 2);
-        const size_t lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        int n1, n2;
+        const bool v2 = split_of(pl.N, n1, n2);
+        size_t lds;
+        if (v2) {
+            const int YS = n1 * (n2 + 1) + 1;
+            int S = std::min(16, nrows / 2);
+            while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_MAX) --S;
+            a.S = S;
+            a.stride = YS;
+            lds = (size_t)a.S * YS * sizeof(cplx<T>);
+        } else {
+            a.stride = pl.N + 1;
+            a.S = rows_per_wg<T>(pl.N, nrows / 2);
+            lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        }
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev, nj);
         const double bytes = (double)nj * nlev * nrows * ((double)ncols * sizeof(cplx<T>) +
                                                            (double)dst_ld * sizeof(T));
-        tm.run(s, KK_ROWS_C2R, bytes,
-               [&] { hipLaunchKernelGGL(k_rows_c2r<T>, grid, dim3(BLK), lds, s, a); });
+        tm.run(s, KK_ROWS_C2R, bytes, [&] {
+            if (!(v2 && v2_c2r<T>(n1, n2, grid, lds, s, a)))
+                hipLaunchKernelGGL(k_rows_c2r<T>, grid, dim3(BLK), lds, s, a);
+        });
     }
 }
 
@@ -301,7 +394,7 @@ void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_pl
         dim3 grid(nj, (d.NK + c.TC - 1) / c.TC, nlev);
         const int nj_ = nj, nlev_ = nlev;
         const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny + d.ny);
-        timer.run(stream, KK_COLS_FWD, cbytes, [&] { hipLaunchKernelGGL((k_cols<T, -1>), grid, dim3(BLK), lds, stream, c); });
+        timer.run(stream, KK_COLS_FWD, cbytes, [&] { launch_cols<T, -1>(c, nj_, nlev_, stream); });
     }
 }
 
@@ -337,7 +430,7 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         dim3 grid(nj, (d.NK + c.TC - 1) / c.TC, nlev);
         const int nj_ = nj, nlev_ = nlev;
         const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny + d.ny);
-        timer.run(stream, KK_COLS_INV, cbytes, [&] { hipLaunchKernelGGL((k_cols<T, +1>), grid, dim3(BLK), lds, stream, c); });
+        timer.run(stream, KK_COLS_INV, cbytes, [&] { launch_cols<T, +1>(c, nj_, nlev_, stream); });
         std::vector<const cplx<T>*> ts(tmpspec, tmpspec + nj);
         launch_rows_c2r<T>(timer, px.plan, nj, ts.data(), tmp_plane, 0, dst + j0, dst_plane, d.ld,
                            dst_lev0, d.ny, d.NK, nlev, stream);
@@ -369,7 +462,7 @@ void Case<T>::pad_inv_big(int njobs, const cplx<T>* const* src, int lev0, int nl
     dim3 grid(njobs, (d.NK + c.TC - 1) / c.TC, nlev);
     const int nj_ = njobs, nlev_ = nlev;
         const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny + d.ny2);
-        timer.run(stream, KK_COLS_INV, cbytes, [&] { hipLaunchKernelGGL((k_cols<T, +1>), grid, dim3(BLK), lds, stream, c); });
+        timer.run(stream, KK_COLS_INV, cbytes, [&] { launch_cols<T, +1>(c, nj_, nlev_, stream); });
     std::vector<const cplx<T>*> bs(bigspec, bigspec + njobs);
     launch_rows_c2r<T>(timer, pbx.plan, njobs, bs.data(), d.bsplane, lev0, big, d.bplane, d.ld_big, lev0,
                        d.ny2, d.NK, nlev, stream);
@@ -403,7 +496,7 @@ void Case<T>::fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst,
     dim3 grid(njobs, (d.NK + c.TC - 1) / c.TC, nlev);
     const int nj_ = njobs, nlev_ = nlev;
         const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny2 + d.ny);
-        timer.run(stream, KK_COLS_FWD, cbytes, [&] { hipLaunchKernelGGL((k_cols<T, -1>), grid, dim3(BLK), lds, stream, c); });
+        timer.run(stream, KK_COLS_FWD, cbytes, [&] { launch_cols<T, -1>(c, nj_, nlev_, stream); });
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,363 @@
+// Register-resident plane FFT passes for the common transform lengths.
+//
+// A length N = N1 * N2 transform is done as a four-step FFT: every task
+// first takes the N1 points x[n2 + N2 n1] (n1 < N1) into registers and does
+// a length-N1 DFT there, multiplies by W_N^(n2 k1), and writes them to LDS;
+// after one barrier every task takes the N2 points of one k1 and does the
+// length-N2 DFT in registers, giving X[k1 + N1 k2].  Both small DFTs are
+// unrolled at compile time (recursive radix 4/3/2 with constant twiddles),
+// so one LDS transpose replaces the log(N) LDS stages of the generic pass.
+// LDS rows are padded so that consecutive lanes fall in different banks.
+// Same conventions and arguments as the generic passes in fft.hpp.
+#pragma once
+
+#include "fft.hpp"
+#include "twiddle144.hpp"
+
+namespace lg {
+
+// W_n^j for the small DFTs (n divides 144); DIR = -1 forward, +1 inverse.
+template <typename T, int n, int j, int DIR>
+__device__ __forceinline__ cplx<T> wsmall() {
+    constexpr int m = ((j % n) * (144 / n)) % 144;
+    return cplx<T>{(T)kCos144[m], (T)(DIR * kSin144[m])};
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void bf2(cplx<T>& a, cplx<T>& b) {
+    const cplx<T> t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void bf3(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2) {
+    const T c = (T)-0.5;
+    const T sn = (T)(DIR * 0.86602540378443864676);
+    const cplx<T> s = cadd(a1, a2), d = csub(a1, a2);
+    const cplx<T> m{a0.x + c * s.x, a0.y + c * s.y};
+    const cplx<T> id{-sn * d.y, sn * d.x};
+    a0 = cadd(a0, s);
+    a1 = cadd(m, id);
+    a2 = csub(m, id);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void bf4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3) {
+    const cplx<T> s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    const cplx<T> s13 = cadd(a1, a3), d13 = csub(a1, a3);
+    const cplx<T> jd13 = (DIR < 0) ? cplx<T>{d13.y, -d13.x} : cplx<T>{-d13.y, d13.x};
+    a0 = cadd(s02, s13);
+    a1 = cadd(d02, jd13);
+    a2 = csub(s02, s13);
+    a3 = csub(d02, jd13);
+}
+
+template <typename T, int N, int R, int M, int DIR, int K1 = 0>
+__device__ __forceinline__ void combine(cplx<T> (&x)[N], cplx<T> (&sub)[R][M]);
+
+// In-place DFT of N register values (natural order in and out).
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void dft_reg(cplx<T> (&x)[N]) {
+    if constexpr (N == 1) {
+        return;
+    } else if constexpr (N == 2) {
+        bf2<T, DIR>(x[0], x[1]);
+    } else if constexpr (N == 3) {
+        bf3<T, DIR>(x[0], x[1], x[2]);
+    } else if constexpr (N == 4) {
+        bf4<T, DIR>(x[0], x[1], x[2], x[3]);
+    } else {
+        constexpr int R = (N % 4 == 0) ? 4 : ((N % 3 == 0) ? 3 : 2);
+        constexpr int M = N / R;
+        cplx<T> sub[R][M];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int j = 0; j < M; ++j) sub[q][j] = x[q + R * j];
+#pragma unroll
+        for (int q = 0; q < R; ++q) dft_reg<T, M, DIR>(sub[q]);
+        // X[k1 + M k2] = sum_q W_N^(q k1) W_R^(q k2) sub[q][k1]
+        combine<T, N, R, M, DIR>(x, sub);
+    }
+}
+
+template <typename T, int N, int R, int M, int DIR, int K1>
+__device__ __forceinline__ void combine(cplx<T> (&x)[N], cplx<T> (&sub)[R][M]) {
+    if constexpr (K1 < M) {
+        cplx<T> t[R];
+        t[0] = sub[0][K1];
+        if constexpr (R > 1) t[1] = (K1 == 0) ? sub[1][K1] : cmul(sub[1][K1], wsmall<T, N, 1 * K1, DIR>());
+        if constexpr (R > 2) t[2] = (K1 == 0) ? sub[2][K1] : cmul(sub[2][K1], wsmall<T, N, 2 * K1, DIR>());
+        if constexpr (R > 3) t[3] = (K1 == 0) ? sub[3][K1] : cmul(sub[3][K1], wsmall<T, N, 3 * K1, DIR>());
+        dft_reg<T, R, DIR>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) x[K1 + M * k2] = t[k2];
+        combine<T, N, R, M, DIR, K1 + 1>(x, sub);
+    }
+}
+
+// outer twiddle W_N^(e) from the plan table (e < N)
+template <typename T, int DIR>
+__device__ __forceinline__ cplx<T> wbig(const typename TwT<T>::type* tw, int e) {
+    const auto w = tw[e];
+    return cplx<T>{(T)w.x, (T)(-DIR) * (T)w.y};
+}
+
+template <int N1, int N2>
+struct Split {
+    static constexpr int n1 = N1, n2 = N2, N = N1 * N2;
+    static constexpr int YR = N2 + 1;          // LDS row stride (complex)
+    static constexpr int YS = N1 * YR + 1;     // LDS stride per sequence / column
+};
+
+// --------------------------------------------------------------------------
+// Row pass r2c (paired rows), register four-step.
+// --------------------------------------------------------------------------
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(256) k_rows_r2c_v2(RowR2CArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    extern __shared__ __align__(16) unsigned char smem_r2[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2);
+    const auto* tw = plan_tw<T>(a.plan);
+    const int job = blockIdx.z;
+    const int plane = blockIdx.y;
+    const int npairs = a.nrows / 2;
+    const int pair0 = blockIdx.x * a.S;
+    const int S = min(a.S, npairs - pair0);
+    const T* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const T sc = a.scale[job];
+    for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
+        const int s = t / N2, n2 = t - s * N2;
+        const int j = 2 * (pair0 + s);
+        const T* ra = src + (long)j * a.src_ld;
+        const T* rb = ra + a.src_ld;
+        cplx<T> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) x[n1] = cplx<T>{sc * ra[n2 + N2 * n1], sc * rb[n2 + N2 * n1]};
+        dft_reg<T, N1, -1>(x);
+        cplx<T>* Y = lds + s * SP::YS;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+            Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, -1>(tw, n2 * k1));
+    }
+    __syncthreads();
+    // step 2 in one round (the launcher keeps S * N1 <= blockDim.x): all LDS
+    // reads finish before the barrier, then X overwrites Y in natural order
+    cplx<T> y[N2];
+    const int t2 = threadIdx.x;
+    const bool act = t2 < S * N1;
+    const int s2 = act ? t2 / N1 : 0;
+    const int k1 = act ? t2 - s2 * N1 : 0;
+    if (act) {
+        const cplx<T>* Y = lds + s2 * SP::YS + k1 * SP::YR;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+        dft_reg<T, N2, -1>(y);
+    }
+    __syncthreads();
+    if (act) {
+        cplx<T>* X = lds + s2 * SP::YS;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) X[k1 + N1 * k2] = y[k2];
+    }
+    __syncthreads();
+    const int nc = a.ncols;
+    for (int idx = threadIdx.x; idx < S * nc; idx += blockDim.x) {
+        const int s = idx / nc;
+        const int k = idx - s * nc;
+        const int j = 2 * (pair0 + s);
+        const cplx<T>* X = lds + s * SP::YS;
+        const cplx<T> z = X[k];
+        const cplx<T> zc = X[(N - k) % N];
+        const cplx<T> A{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y - zc.y)};
+        const cplx<T> Bm{(T)0.5 * (z.x - zc.x), (T)0.5 * (z.y + zc.y)};
+        dst[(long)j * nc + k] = A;
+        dst[(long)(j + 1) * nc + k] = cplx<T>{Bm.y, -Bm.x};
+    }
+}
+
+// --------------------------------------------------------------------------
+// Row pass c2r (paired rows), register four-step.
+// --------------------------------------------------------------------------
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    constexpr int half = N / 2;
+    extern __shared__ __align__(16) unsigned char smem_c2[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2);
+    const auto* tw = plan_tw<T>(a.plan);
+    const int job = blockIdx.z;
+    const int plane = blockIdx.y;
+    const int npairs = a.nrows / 2;
+    const int pair0 = blockIdx.x * a.S;
+    const int S = min(a.S, npairs - pair0);
+    const cplx<T>* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
+    T* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const int nc = a.ncols;
+    for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
+        const int s = t / N2, n2 = t - s * N2;
+        const int j = 2 * (pair0 + s);
+        const cplx<T>* ra = src + (long)j * nc;
+        const cplx<T>* rb = ra + nc;
+        cplx<T> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int k = n2 + N2 * n1;
+            cplx<T> z{(T)0, (T)0};
+            if (k == 0) {
+                z = cplx<T>{ra[0].x, rb[0].x};
+            } else if (k < half) {
+                if (k < nc) {
+                    const cplx<T> A = ra[k], B = rb[k];
+                    z = cplx<T>{A.x - B.y, A.y + B.x};
+                }
+            } else if (k > half) {
+                const int kk = N - k;
+                if (kk < nc) {
+                    const cplx<T> A = ra[kk], B = rb[kk];
+                    z = cplx<T>{A.x + B.y, B.x - A.y};
+                }
+            }
+            x[n1] = z;
+        }
+        dft_reg<T, N1, +1>(x);
+        cplx<T>* Y = lds + s * SP::YS;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+            Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, +1>(tw, n2 * k1));
+    }
+    __syncthreads();
+    const int ld = a.dst_ld;
+    for (int t = threadIdx.x; t < S * N1; t += blockDim.x) {
+        const int s = t / N1, k1 = t - s * N1;
+        const int j = 2 * (pair0 + s);
+        const cplx<T>* Y = lds + s * SP::YS + k1 * SP::YR;
+        cplx<T> y[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+        dft_reg<T, N2, +1>(y);
+        T* oa = dst + (long)j * ld;
+        T* ob = oa + ld;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            oa[k1 + N1 * k2] = y[k2].x;
+            ob[k1 + N1 * k2] = y[k2].y;
+        }
+    }
+    // padding columns N..ld-1
+    const int npad = ld - N;
+    for (int idx = threadIdx.x; idx < S * npad; idx += blockDim.x) {
+        const int s = idx / npad, q = idx - s * npad;
+        const int j = 2 * (pair0 + s);
+        dst[(long)j * ld + N + q] = (T)0;
+        dst[(long)(j + 1) * ld + N + q] = (T)0;
+    }
+}
+
+// --------------------------------------------------------------------------
+// Column pass, register four-step; lanes run over columns for coalescing.
+// --------------------------------------------------------------------------
+template <typename T, int N1, int N2, int DIR>
+__global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    extern __shared__ __align__(16) unsigned char smem_cv[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_cv);
+    const auto* tw = plan_tw<T>(a.plan);
+    const int job = blockIdx.x;
+    const int TC = a.TC;
+    const int c0 = blockIdx.y * TC;
+    const int tc = min(TC, a.ncols - c0);
+    const int plane = blockIdx.z;
+    const cplx<T>* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const int op = a.op[job];
+    const int ny = a.ny;
+    const int nyh = ny / 2;
+    const int CS = SP::YS;                     // per-column LDS block (odd-padded)
+    for (int t = threadIdx.x; t < TC * N2; t += blockDim.x) {
+        const int c = t % TC, n2 = t / TC;
+        if (c >= tc) continue;
+        cplx<T> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int jb = n2 + N2 * n1;
+            cplx<T> v{(T)0, (T)0};
+            if (a.map == MAP_PAD) {
+                int j = -1;
+                if (jb < nyh) j = jb;
+                else if (jb > N - nyh) j = jb - (N - ny);
+                if (j >= 0) v = apply_op<T>(op, src[(long)j * a.ncols + c0 + c], c0 + c, j, a);
+            } else {
+                v = src[(long)jb * a.ncols + c0 + c];
+                if (DIR > 0 && a.map == MAP_ID) v = apply_op<T>(op, v, c0 + c, jb, a);
+            }
+            x[n1] = v;
+        }
+        dft_reg<T, N1, DIR>(x);
+        cplx<T>* Y = lds + c * CS;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+            Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, DIR>(tw, n2 * k1));
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < TC * N1; t += blockDim.x) {
+        const int c = t % TC, k1 = t / TC;
+        if (c >= tc) continue;
+        const cplx<T>* Y = lds + c * CS + k1 * SP::YR;
+        cplx<T> y[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+        dft_reg<T, N2, DIR>(y);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int jb = k1 + N1 * k2;
+            cplx<T> v = y[k2];
+            if (a.map == MAP_UNPAD) {
+                int j;
+                if (jb < nyh) j = jb;
+                else if (jb > N - nyh) j = jb - (N - ny);
+                else continue;
+                if (DIR < 0) v = apply_op<T>(op, v, c0 + c, j, a);
+                dst[(long)j * a.ncols + c0 + c] = v;
+            } else {
+                if (DIR < 0 && a.map == MAP_ID) {
+                    if (a.zero_nyq && jb == nyh) v = cplx<T>{(T)0, (T)0};
+                    v = apply_op<T>(op, v, c0 + c, jb, a);
+                }
+                dst[(long)jb * a.ncols + c0 + c] = v;
+            }
+        }
+    }
+    if (a.map == MAP_UNPAD) {
+        // the ky = ny/2 row of the small output is zero (fft.f90:84-86)
+        for (int c = threadIdx.x; c < tc; c += blockDim.x)
+            dst[(long)nyh * a.ncols + c0 + c] = cplx<T>{(T)0, (T)0};
+    }
+}
+
+// Splits used for the transform lengths on the step: N -> (N1, N2).
+// Returns false when N has no specialised kernel (generic passes are used).
+inline bool split_of(int N, int& n1, int& n2) {
+    switch (N) {
+        case 16: n1 = 4; n2 = 4; return true;
+        case 24: n1 = 4; n2 = 6; return true;
+        case 32: n1 = 4; n2 = 8; return true;
+        case 48: n1 = 6; n2 = 8; return true;
+        case 64: n1 = 8; n2 = 8; return true;
+        case 72: n1 = 8; n2 = 9; return true;
+        case 96: n1 = 8; n2 = 12; return true;
+        case 128: n1 = 8; n2 = 16; return true;
+        case 144: n1 = 12; n2 = 12; return true;
+        case 192: n1 = 12; n2 = 16; return true;
+        case 256: n1 = 16; n2 = 16; return true;
+        case 384: n1 = 16; n2 = 24; return true;
+        default: return false;
+    }
+}
+
+}  // namespace lg
