@@ -194,6 +194,9 @@ Case<T>::Case(const lg_params& p) : d(p) {
     for (int i = 0; i < NBIG; ++i) big[i] = dalloc<T>((long)(d.nz + 1) * d.bplane);
     for (int i = 0; i < 6; ++i) bigspec[i] = dalloc<cplx<T>>((long)(d.nz + 1) * d.bsplane);
     for (int i = 0; i < NTMP; ++i) tmp[i] = dalloc<T>(d.field);
+    sU = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
+    sV = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
+    sW = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
     gam = dalloc<double>((long)(d.nz + 1) * d.splane);
     usc = dalloc<double2>((long)(d.nz + 1) * d.splane);
     dev_err = dalloc<int>(1);
@@ -242,6 +245,9 @@ Case<T>::~Case() {
     for (auto p : big) fr(p);
     for (auto p : bigspec) fr(p);
     for (auto p : tmp) fr(p);
+    fr(sU);
+    fr(sV);
+    fr(sW);
     fr(gam);
     fr(usc);
     fr(dev_err);
@@ -401,7 +407,7 @@ void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_pl
 template <typename T>
 void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane,
                     int src_lev0, T* const* dst, long dst_plane, int dst_lev0, int nlev,
-                    cplx<T>* const* tmpspec, long tmp_plane) {
+                    cplx<T>* const* tmpspec, long tmp_plane, const cplx<T>* const* src2) {
     if (nlev <= 0) return;
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
@@ -409,6 +415,7 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         c.plan = py.plan;
         for (int i = 0; i < nj; ++i) {
             c.src[i] = src[j0 + i];
+            c.src2[i] = src2 ? src2[j0 + i] : nullptr;
             c.dst[i] = tmpspec[i];
             c.op[i] = ops[j0 + i];
         }
@@ -525,12 +532,13 @@ static void fill_ptr(T* p, long n, double v, hipStream_t s) {
 // derivatives.f90
 // ---------------------------------------------------------------------------
 template <typename T>
-void Case<T>::filt_da(int f, int dfdx, int dfdy) {
+void Case<T>::filt_da(int f, int dfdx, int dfdy, cplx<T>* keep) {
     const T sc = (T)(1.0 / (d.nx * d.ny));
     const T* src[1] = {fld[f]};
-    cplx<T>* S0[1] = {spec[0]};
+    cplx<T>* sp = keep ? keep : spec[0];
+    cplx<T>* S0[1] = {sp};
     fwd2d(1, src, &sc, d.plane, 0, S0, d.splane, 0, d.nz + 1);
-    const cplx<T>* s3[3] = {spec[0], spec[0], spec[0]};
+    const cplx<T>* s3[3] = {sp, sp, sp};
     const int ops[3] = {OP_NONE, OP_IKX, OP_IKY};
     T* dst[3] = {fld[f], fld[dfdx], fld[dfdy]};
     cplx<T>* tsp[3] = {spec[1], spec[2], spec[3]};
@@ -719,23 +727,26 @@ void Case<T>::lagrange_sdep() {
 // ---------------------------------------------------------------------------
 template <typename T>
 void Case<T>::divstress_uv() {
+    // ddx txx + ddy txy and ddx txy + ddy tyy are summed in spectral space
+    // (one inverse each); the z term is added in physical space.
     const T sc = (T)(1.0 / (d.nx * d.ny));
     const int nl = d.nz - 1;
     const T* src[3] = {fld[LG_TXX], fld[LG_TXY], fld[LG_TYY]};
     const T scs[3] = {sc, sc, sc};
     cplx<T>* S[3] = {spec[0], spec[1], spec[2]};
     fwd2d(3, src, scs, d.plane, 1, S, d.splane, 1, nl);
-    const cplx<T>* is[4] = {spec[0], spec[1], spec[1], spec[2]};
-    const int ops[4] = {OP_IKX, OP_IKY, OP_IKX, OP_IKY};
-    T* dst[4] = {tmp[0], tmp[1], tmp[2], tmp[3]};
-    cplx<T>* tsp[4] = {spec[3], spec[4], spec[5], spec[6]};
-    inv2d(4, is, ops, d.splane, 1, dst, d.plane, 1, nl, tsp, d.splane);
+    const cplx<T>* is[2] = {spec[0], spec[1]};
+    const cplx<T>* is2[2] = {spec[1], spec[2]};
+    const int ops[2] = {OP_DIVXY, OP_DIVXY};
+    T* dst[2] = {tmp[0], tmp[2]};
+    cplx<T>* tsp[2] = {spec[3], spec[4]};
+    inv2d(2, is, ops, d.splane, 1, dst, d.plane, 1, nl, tsp, d.splane, is2);
     const Geo g = geo_of(d);
     const T rdz = (T)(1.0 / d.dz);
     hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl), dim3(BLK), 0, stream, tmp[0],
-                       tmp[1], fld[LG_TXZ], fld[LG_DIVTX], g, rdz);
+                       fld[LG_TXZ], fld[LG_DIVTX], g, rdz);
     hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl), dim3(BLK), 0, stream, tmp[2],
-                       tmp[3], fld[LG_TYZ], fld[LG_DIVTY], g, rdz);
+                       fld[LG_TYZ], fld[LG_DIVTY], g, rdz);
     for (int id : {LG_DIVTX, LG_DIVTY}) {
         fill_level(id, 0, BOGUS);
         fill_level(id, d.nz, BOGUS);
@@ -749,13 +760,14 @@ void Case<T>::divstress_w() {
     const T scs[2] = {sc, sc};
     cplx<T>* S[2] = {spec[0], spec[1]};
     fwd2d(2, src, scs, d.plane, 1, S, d.splane, 1, d.nz);
-    const cplx<T>* is[2] = {spec[0], spec[1]};
-    const int ops[2] = {OP_IKX, OP_IKY};
-    T* dst[2] = {tmp[4], tmp[5]};
-    cplx<T>* tsp[2] = {spec[2], spec[3]};
-    inv2d(2, is, ops, d.splane, 1, dst, d.plane, 1, d.nz, tsp, d.splane);
+    const cplx<T>* is[1] = {spec[0]};
+    const cplx<T>* is2[1] = {spec[1]};
+    const int ops[1] = {OP_DIVXY};
+    T* dst[1] = {tmp[4]};
+    cplx<T>* tsp[1] = {spec[2]};
+    inv2d(1, is, ops, d.splane, 1, dst, d.plane, 1, d.nz, tsp, d.splane, is2);
     hipLaunchKernelGGL(k_divt_w<T>, grid_ij(d.ld, d.ny, d.nz), dim3(BLK), 0, stream, tmp[4],
-                       tmp[5], fld[LG_TZZ], fld[LG_DIVTZ], geo_of(d), (T)(1.0 / d.dz));
+                       fld[LG_TZZ], fld[LG_DIVTZ], geo_of(d), (T)(1.0 / d.dz));
     fill_level(LG_DIVTZ, 0, BOGUS);
 }
 
@@ -763,19 +775,38 @@ void Case<T>::divstress_w() {
 // convec.f90
 // ---------------------------------------------------------------------------
 template <typename T>
-void Case<T>::convec() {
+void Case<T>::convec(bool reuse_spectra) {
     const T c = (T)(1.0 / (d.nx * d.ny));
     const int nl = d.nz;
-    const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
-    const T scs[3] = {c, c, c};
-    cplx<T>* S[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
-    fwd2d(3, src, scs, d.plane, 1, S, d.splane, 1, nl);
-    VortArgs<T> va{fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDZ], fld[LG_DWDX],
-                   fld[LG_DWDY], tmp[0], tmp[1], tmp[2], c};
-    hipLaunchKernelGGL(k_vort<T>, grid_ij(d.nx, d.ny, nl), dim3(BLK), 0, stream, va, geo_of(d));
-    const T* vs[3] = {tmp[0], tmp[1], tmp[2]};
-    fwd2d(3, vs, nullptr, d.plane, 1, S + 3, d.splane, 1, nl);
     const cplx<T>* ps[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
+    if (reuse_spectra) {
+        // u, v, w spectra are the ones filt_da just produced (the filtered
+        // fields' transforms); the vorticity is formed from them in spectral
+        // space, with only the wall-level dudz/dvdz planes transformed.
+        const T* wsrc[2] = {lev(LG_DVDZ, 1), lev(LG_DUDZ, 1)};
+        const T wsc[2] = {c, c};
+        cplx<T>* wd[2] = {spec[6], spec[6] + d.splane};
+        fwd2d(2, wsrc, wsc, d.plane, 0, wd, d.splane, 0, 1);
+        const int n = d.ny * d.NK;
+        hipLaunchKernelGGL(k_vort_spec<T>, dim3((n + BLK - 1) / BLK, nl), dim3(BLK), 0, stream,
+                           sU, sV, sW, spec[6], spec[6] + d.splane, spec[3], spec[4], spec[5],
+                           d.ny, d.NK, d.nz, d.splane, (T)(2.0 * PI_D / d.prm.lx),
+                           (T)(2.0 * PI_D / d.prm.ly), (T)(1.0 / d.dz));
+        ps[0] = sU;
+        ps[1] = sV;
+        ps[2] = sW;
+    } else {
+        const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
+        const T scs[3] = {c, c, c};
+        cplx<T>* S[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
+        fwd2d(3, src, scs, d.plane, 1, S, d.splane, 1, nl);
+        VortArgs<T> va{fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDZ], fld[LG_DWDX],
+                       fld[LG_DWDY], tmp[0], tmp[1], tmp[2], c};
+        hipLaunchKernelGGL(k_vort<T>, grid_ij(d.nx, d.ny, nl), dim3(BLK), 0, stream, va,
+                           geo_of(d));
+        const T* vs[3] = {tmp[0], tmp[1], tmp[2]};
+        fwd2d(3, vs, nullptr, d.plane, 1, S + 3, d.splane, 1, nl);
+    }
     pad_inv_big(6, ps, 1, nl);
     ProdArgs<T> pa{big[0], big[1], big[2], big[3], big[4], big[5], big[6], big[7], big[8],
                    (T)(1.0 / (d.nx2 * d.ny2))};
@@ -889,9 +920,9 @@ void Case<T>::step() {
     std::swap(fld[LG_RHSX], fld[LG_RHSX_F]);
     std::swap(fld[LG_RHSY], fld[LG_RHSY_F]);
     std::swap(fld[LG_RHSZ], fld[LG_RHSZ_F]);
-    filt_da(LG_U, LG_DUDX, LG_DUDY);
-    filt_da(LG_V, LG_DVDX, LG_DVDY);
-    filt_da(LG_W, LG_DWDX, LG_DWDY);
+    filt_da(LG_U, LG_DUDX, LG_DUDY, sU);
+    filt_da(LG_V, LG_DVDX, LG_DVDY, sV);
+    filt_da(LG_W, LG_DWDX, LG_DWDY, sW);
     ddz_uv(LG_U, LG_DUDZ);
     ddz_uv(LG_V, LG_DVDZ);
     ddz_w(LG_W, LG_DWDZ);
@@ -899,7 +930,7 @@ void Case<T>::step() {
     sgs_stag();
     divstress_uv();
     divstress_w();
-    convec();
+    convec(true);
     if (turb) turbines_forcing();
     rhs_ab2();
     press_stag_array();

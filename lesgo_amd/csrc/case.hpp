@@ -74,6 +74,9 @@ struct Case {
     static constexpr int NTMP = 9;
     T* tmp[NTMP] = {};
     // pressure solve
+    cplx<T>* sU = nullptr;           // filtered u, v, w spectra of the current step
+    cplx<T>* sV = nullptr;
+    cplx<T>* sW = nullptr;
     double* gam = nullptr;           // (nz+1) * ny * NK
     double2* usc = nullptr;          // (nz+1) * ny * NK
     int* dev_err = nullptr;
@@ -103,12 +106,12 @@ struct Case {
                cplx<T>* const* dst, long dst_plane, int dst_lev0, int nlev);
     void inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane, int src_lev0,
                T* const* dst, long dst_plane, int dst_lev0, int nlev, cplx<T>* const* tmpspec,
-               long tmp_plane);
+               long tmp_plane, const cplx<T>* const* src2 = nullptr);
     void pad_inv_big(int njobs, const cplx<T>* const* src, int lev0, int nlev);
     void fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst, int lev0, int nlev);
 
     // subsystems
-    void filt_da(int f, int dfdx, int dfdy);
+    void filt_da(int f, int dfdx, int dfdy, cplx<T>* keep = nullptr);
     void ddx(int f, int dfdx);
     void ddy(int f, int dfdy);
     void ddxy(int f, int dfdx, int dfdy);
@@ -122,7 +125,7 @@ struct Case {
     void lagrange_sdep();
     void divstress_uv();
     void divstress_w();
-    void convec();
+    void convec(bool reuse_spectra = false);
     void press_stag_array();
     void project();
     void rhs_ab2();

@@ -280,6 +280,7 @@ enum ColOp : int {
     OP_IKY = 2,        // .MULI. ky
     OP_G1 = 3,         // .MULR. G_test       (test_filtermodule.f90:141)
     OP_G2 = 4,         // .MULR. G_test_test  (test_filtermodule.f90:164)
+    OP_DIVXY = 5,      // (.MULI. kx) src + (.MULI. ky) src2: d/dx + d/dy of two fields
 };
 
 enum ColMap : int {
@@ -292,6 +293,7 @@ template <typename T>
 struct ColArgs {
     FftPlan plan;
     const cplx<T>* src[MAX_JOBS];
+    const cplx<T>* src2[MAX_JOBS];     // second input of OP_DIVXY
     cplx<T>* dst[MAX_JOBS];
     int op[MAX_JOBS];
     long src_plane;
@@ -309,6 +311,16 @@ struct ColArgs {
     const T* G1;       // [ny][ncols] filter tables
     const T* G2;
 };
+
+// i kx v + i ky v2 at small-grid index (c, j) (Nyquist row: both 0)
+template <typename T>
+__device__ __forceinline__ cplx<T> divxy(cplx<T> v, cplx<T> v2, int c, int j, const ColArgs<T>& a) {
+    if (j == a.ny / 2) return cplx<T>{(T)0, (T)0};
+    const T kx = a.kxs * (T)c;
+    const int m = (j < a.ny / 2) ? j : j - a.ny;
+    const T ky = a.kys * (T)m;
+    return cplx<T>{-v.y * kx + -v2.y * ky, v.x * kx + v2.x * ky};
+}
 
 template <typename T>
 __device__ __forceinline__ cplx<T> apply_op(int op, cplx<T> v, int c, int j, const ColArgs<T>& a) {
@@ -356,7 +368,14 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
             if (j >= 0) v = apply_op<T>(op, src[(long)j * a.ncols + c0 + c], c0 + c, j, a);
         } else {
             v = src[(long)jb * a.ncols + c0 + c];
-            if (DIR > 0 && a.map == MAP_ID) v = apply_op<T>(op, v, c0 + c, jb, a);
+            if (DIR > 0 && a.map == MAP_ID) {
+                if (op == OP_DIVXY) {
+                    const cplx<T>* s2 = a.src2[job] + (long)(a.lev0_src + plane) * a.src_plane;
+                    v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);
+                } else {
+                    v = apply_op<T>(op, v, c0 + c, jb, a);
+                }
+            }
         }
         lds[c * a.stride + perm[jb]] = v;
     }

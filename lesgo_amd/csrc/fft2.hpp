@@ -294,7 +294,14 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 if (j >= 0) v = apply_op<T>(op, src[(long)j * a.ncols + c0 + c], c0 + c, j, a);
             } else {
                 v = src[(long)jb * a.ncols + c0 + c];
-                if (DIR > 0 && a.map == MAP_ID) v = apply_op<T>(op, v, c0 + c, jb, a);
+                if (DIR > 0 && a.map == MAP_ID) {
+                    if (op == OP_DIVXY) {
+                        const cplx<T>* s2 = a.src2[job] + (long)(a.lev0_src + plane) * a.src_plane;
+                        v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);
+                    } else {
+                        v = apply_op<T>(op, v, c0 + c, jb, a);
+                    }
+                }
             }
             x[n1] = v;
         }

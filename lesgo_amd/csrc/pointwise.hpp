@@ -173,9 +173,10 @@ __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
     }
 }
 
-// divstress_uv sum (divstress_uv.f90:58-83): k = 1..nz-1, all ld columns.
+// divstress_uv sum (divstress_uv.f90:58-83): k = 1..nz-1, all ld columns;
+// dxy holds ddx(t_x1) + ddy(t_x2) formed in spectral space.
 template <typename T>
-__global__ void k_divt_uv(const T* ddx_a, const T* ddy_b, const T* tz, T* out, Geo g, T rdz) {
+__global__ void k_divt_uv(const T* dxy, const T* tz, T* out, Geo g, T rdz) {
     int j, i;
     if (!ij_of(g.ld, g.ny, j, i)) return;
     const int k = blockIdx.y + 1;
@@ -185,12 +186,12 @@ __global__ void k_divt_uv(const T* ddx_a, const T* ddy_b, const T* tz, T* out, G
         return;
     }
     const T dz = rdz * (tz[o + g.plane] - tz[o]);
-    out[o] = ddx_a[o] + ddy_b[o] + dz;
+    out[o] = dxy[o] + dz;
 }
 
 // divstress_w sum (divstress_w.f90:63-114): k = 1..nz.
 template <typename T>
-__global__ void k_divt_w(const T* dtxdx, const T* dtydy, const T* tzz, T* out, Geo g, T rdz) {
+__global__ void k_divt_w(const T* dxy, const T* tzz, T* out, Geo g, T rdz) {
     int j, i;
     if (!ij_of(g.ld, g.ny, j, i)) return;
     const int k = blockIdx.y + 1;
@@ -200,10 +201,10 @@ __global__ void k_divt_w(const T* dtxdx, const T* dtydy, const T* tzz, T* out, G
         return;
     }
     if (k == 1 || k == g.nz) {
-        out[o] = dtxdx[o] + dtydy[o];
+        out[o] = dxy[o];
     } else {
         const T dz = rdz * (tzz[o] - tzz[o - g.plane]);
-        out[o] = dtxdx[o] + dtydy[o] + dz;
+        out[o] = dxy[o] + dz;
     }
 }
 
@@ -230,6 +231,50 @@ __global__ void k_vort(VortArgs<T> a, Geo g) {
         a.o2[o] = a.c * (a.dudz[o] - a.dwdx[o]);
     }
     a.o3[o] = a.c * (a.dvdx[o] - a.dudy[o]);
+}
+
+// Vorticity in spectral space from the filtered u, v, w spectra (normalised
+// by 1/(nx ny) like the reference's const * vort before its r2c,
+// convec.f90:97-168): w1 = i ky w - dz v, w2 = dz u - i kx w, w3 = i kx v -
+// i ky u; the wall level uses the averaged dw/dx, dw/dy and the wall-model
+// dudz, dvdz planes (Du1, Dv1), the lid level the zero dudz, dvdz of
+// ws_free_ubc.  k = 1..nz.
+template <typename T>
+__global__ void k_vort_spec(const cplx<T>* U, const cplx<T>* V, const cplx<T>* W,
+                            const cplx<T>* Dv1, const cplx<T>* Du1, cplx<T>* O1, cplx<T>* O2,
+                            cplx<T>* O3, int ny, int NK, int nz, long splane, T kxs, T kys,
+                            T rdz) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ny * NK) return;
+    const int k = blockIdx.y + 1;
+    const int j = idx / NK, c = idx - j * NK;
+    T kx = kxs * (T)c, ky = kys * (T)((j < ny / 2) ? j : j - ny);
+    if (j == ny / 2) kx = ky = (T)0;
+    auto muli = [](cplx<T> a, T kk) { return cplx<T>{-a.y * kk, a.x * kk}; };
+    const long o = (long)k * splane + idx;
+    const cplx<T> w = W[o];
+    cplx<T> o1, o2;
+    if (k == 1) {
+        const cplx<T> w2 = W[o + splane];
+        const cplx<T> ay = muli(w, ky), by = muli(w2, ky);
+        const cplx<T> ax = muli(w, kx), bx = muli(w2, kx);
+        o1 = cplx<T>{(T)0.5 * (ay.x + by.x) - Dv1[idx].x, (T)0.5 * (ay.y + by.y) - Dv1[idx].y};
+        o2 = cplx<T>{Du1[idx].x - (T)0.5 * (ax.x + bx.x), Du1[idx].y - (T)0.5 * (ax.y + bx.y)};
+    } else if (k == nz) {
+        o1 = muli(w, ky);
+        const cplx<T> ax = muli(w, kx);
+        o2 = cplx<T>{-ax.x, -ax.y};
+    } else {
+        const cplx<T> v0 = V[o], vm = V[o - splane];
+        const cplx<T> u0 = U[o], um = U[o - splane];
+        const cplx<T> ay = muli(w, ky), ax = muli(w, kx);
+        o1 = cplx<T>{ay.x - rdz * (v0.x - vm.x), ay.y - rdz * (v0.y - vm.y)};
+        o2 = cplx<T>{rdz * (u0.x - um.x) - ax.x, rdz * (u0.y - um.y) - ax.y};
+    }
+    const cplx<T> vx = muli(V[o], kx), uy = muli(U[o], ky);
+    O1[o] = o1;
+    O2[o] = o2;
+    O3[o] = cplx<T>{vx.x - uy.x, vx.y - uy.y};
 }
 
 // convec products on the 3/2 grid (convec.f90:172-305), k = 1..nz.

@@ -1,7 +1,8 @@
 """The Fortran host driver (lesgo_amd/fortran/main.f90) steps the 32^3 half
 channel through the C ABI; its domain output files (the io.f90:786-884 binary
 layout) must match the oracle's u, v, w_uv and p - |u|^2/2 at 1e-10 every 10
-steps, for both the call-by-call loop and the fused lg_step loop."""
+steps, for both the call-by-call loop and the fused lg_step loop (which
+reuses the filt_da spectra inside convec, so the two agree to rounding)."""
 import os
 import subprocess
 
@@ -54,7 +55,9 @@ def test_fortran_driver_matches_oracle(tmp_path):
             lv = slice(0, p.nz - 1)            # levels 1..nz-1 (the lid row copies nz-1)
             e = rel_l2(g[lv], r[lv])
             assert e <= 1e-10, (jt, name, e)
-            assert np.array_equal(g, f), (jt, name, "fused and call-by-call loops differ")
+            # lg_step reuses filt_da's spectra in convec: equal up to rounding
+            assert rel_l2(f[lv], g[lv]) <= 1e-12, (jt, name, "fused vs call-by-call")
+            assert rel_l2(f[lv], r[lv]) <= 1e-10, (jt, name, "fused vs oracle")
         print(f"  step {jt}: ok", flush=True)
 
 
