@@ -171,6 +171,60 @@ int lg_timing_report(lg_case* c, int kind, double* ms, double* launches, double*
  * counterpart in the reference. */
 int lg_calibrate_copy(long nbytes, int reps, double* ms_per_copy);
 
+/* --- batches of independent cases (no counterpart in the reference, which
+ * runs one case per process).  A batch holds `members` cases of one grid,
+ * each with its own fields, z0, mean pressure gradient and disks; every
+ * entry point above acts on all stepped members at once (lg_step,
+ * subsystems) or on member 0 (copies, seeding, diagnostics, disks).  Member
+ * m's arithmetic is identical to the same case run alone. */
+int lg_create_batch(const lg_params* prm, int members, lg_case** out);
+/* Capacity and number of stepped members (members 0..active-1). */
+int lg_batch_info(const lg_case* c, int* members, int* active);
+int lg_set_active(lg_case* c, int active);
+/* Per-member z0 (wallstress.f90:198) and mean_p_force (main.f90:228-232). */
+int lg_member_params(lg_case* c, int m, double zo, double mean_p_force_x,
+                     double mean_p_force_y);
+int lg_upload_member(lg_case* c, int m, int field, const double* host);
+int lg_download_member(lg_case* c, int m, int field, double* host);
+/* lg_seed_state for member m with its own z0; counters untouched. */
+int lg_seed_member(lg_case* c, int m, long seed, double noise);
+/* All fields, z0, mean_p_force and disks of member src copied to dst. */
+int lg_copy_member(lg_case* c, int src, int dst);
+int lg_rmsdiv_member(lg_case* c, int m, double* out);
+int lg_energy_member(lg_case* c, int m, double* out);
+int lg_max_cfl_member(lg_case* c, int m, double* out);
+int lg_tau_wall_bot_member(lg_case* c, int m, double* out);
+/* Replace member m's disks (nturb = 0 removes them). */
+int lg_turbines_init_member(lg_case* c, int m, int nturb, const double* spec8,
+                            double alpha1, double alpha2, double filter_cutoff,
+                            int adm_correction);
+int lg_turbines_get_member(lg_case* c, int m, double* u_d_T, double* f_n, int nturb);
+/* lg_step replayed from captured HIP graphs: the same launches as lg_step,
+ * captured once per (RHS storage parity, dynamic-model step) and replayed;
+ * recaptured after lg_set_active or a disk change. */
+int lg_step_graph(lg_case* c, int nsteps);
+
+/* --- autotuner (no counterpart in the reference; FFTW_PATIENT planning,
+ * fft.f90:114-121, is the nearest analogue).  Pass kinds: */
+enum {
+    LG_PASS_R2C = 0,        /* row pass r2c: N = row length, other = rows */
+    LG_PASS_C2R = 1,        /* row pass c2r */
+    LG_PASS_COLS_FWD = 2,   /* column pass forward: N = column length, other = columns */
+    LG_PASS_COLS_INV = 3,   /* column pass inverse */
+    LG_PASS_TRIDAG = 4,     /* tridiagonal: N = ny, other = nx/2; per_wg = block size */
+    LG_PASS_COLS_UNPAD = 5, /* forward, 3/2 grid -> small */
+    LG_PASS_COLS_PAD = 6    /* inverse, small -> 3/2 grid */
+};
+/* Times every launchable choice (four-step split n1 x n2 or the generic LDS
+ * kernel n1 = 0; rows or columns per workgroup) of each pass of the case's
+ * step at its real shape and records the fastest in a process-wide table
+ * keyed by (kind, precision, N, other).  The table depends on the grid only,
+ * so every case of a grid, alone or batched, then runs the same kernels. */
+int lg_autotune(lg_case* c, int reps, int* npasses);
+int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg);
+int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg);
+int lg_tune_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,7 +75,29 @@ SIGNATURES = {
     "lg_timing_enable": (C.c_int, [_P, C.c_int]),
     "lg_timing_report": (C.c_int, [_P, C.c_int, _D, _D, _D]),
     "lg_calibrate_copy": (C.c_int, [C.c_long, C.c_int, _D]),
+    "lg_create_batch": (C.c_int, [C.POINTER(LgParams), C.c_int, C.POINTER(C.c_void_p)]),
+    "lg_batch_info": (C.c_int, [_P, _I, _I]),
+    "lg_set_active": (C.c_int, [_P, C.c_int]),
+    "lg_member_params": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, C.c_double]),
+    "lg_upload_member": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_download_member": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_seed_member": (C.c_int, [_P, C.c_int, C.c_long, C.c_double]),
+    "lg_copy_member": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_rmsdiv_member": (C.c_int, [_P, C.c_int, _D]),
+    "lg_energy_member": (C.c_int, [_P, C.c_int, _D]),
+    "lg_max_cfl_member": (C.c_int, [_P, C.c_int, _D]),
+    "lg_tau_wall_bot_member": (C.c_int, [_P, C.c_int, _D]),
+    "lg_turbines_init_member": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_double, C.c_double,
+                                          C.c_double, C.c_int]),
+    "lg_turbines_get_member": (C.c_int, [_P, C.c_int, _D, _D, C.c_int]),
+    "lg_step_graph": (C.c_int, [_P, C.c_int]),
+    "lg_autotune": (C.c_int, [_P, C.c_int, _I]),
+    "lg_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _I, _I, _I]),
+    "lg_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "lg_tune_clear": (C.c_int, []),
 }
+
+PASS_KINDS = ("r2c", "c2r", "cols_fwd", "cols_inv", "tridag", "cols_unpad", "cols_pad")
 
 KERNEL_KINDS = ("rows_r2c", "rows_c2r", "cols_fwd", "cols_inv", "tridag")
 

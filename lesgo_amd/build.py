@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblesgo_amd.so")
-SOURCES = ["case.hip", "turbines.hip", "capi.hip", "probe.hip"]
+SOURCES = ["case.hip", "passes_f64.hip", "passes_f32.hip", "tune.hip", "turbines.hip", "capi.hip",
+           "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

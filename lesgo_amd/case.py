@@ -56,18 +56,23 @@ def _fid(f) -> int:
 
 
 class Case:
-    """One device-resident LES case behind the lesgo_amd C ABI."""
+    """Device-resident LES case(s) behind the lesgo_amd C ABI.
 
-    def __init__(self, cfg: Config):
+    ``members > 1`` makes a batch of independent cases of one grid (each
+    with its own fields, z0, mean pressure gradient and disks) that every
+    step advances together; per-member methods take ``member``."""
+
+    def __init__(self, cfg: Config, members: int = 1):
         self.cfg = cfg
         h = C.c_void_p()
-        check(lib().lg_create(C.byref(cfg.to_c()), C.byref(h)), "lg_create")
+        check(lib().lg_create_batch(C.byref(cfg.to_c()), members, C.byref(h)), "lg_create_batch")
         self._h = h
         nz, ld, ny = C.c_int(), C.c_int(), C.c_int()
         check(lib().lg_sizes(h, C.byref(nz), C.byref(ld), C.byref(ny)), "lg_sizes")
         self.nz, self.ld, self.ny = nz.value, ld.value, ny.value
         self.shape = (self.nz + 1, self.ny, self.ld)
-        self.nturb = 0
+        self.members = members
+        self.nturb = [0] * members
 
     def close(self):
         if getattr(self, "_h", None):
@@ -87,20 +92,43 @@ class Case:
         self.close()
 
     # -- module state ----------------------------------------------------
-    def upload(self, name, arr):
+    def upload(self, name, arr, member: int = 0):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if a.shape != self.shape:
             raise ValueError(f"{name}: expected shape {self.shape}, got {a.shape}")
-        check(lib().lg_upload(self._h, _fid(name), a.ctypes.data_as(_lib._D)), f"upload {name}")
+        check(lib().lg_upload_member(self._h, member, _fid(name), a.ctypes.data_as(_lib._D)),
+              f"upload {name}")
 
-    def download(self, name):
+    def download(self, name, member: int = 0):
         a = np.empty(self.shape)
-        check(lib().lg_download(self._h, _fid(name), a.ctypes.data_as(_lib._D)), f"download {name}")
+        check(lib().lg_download_member(self._h, member, _fid(name), a.ctypes.data_as(_lib._D)),
+              f"download {name}")
         return a
 
-    def seed_state(self, seed: int, noise: float = 0.1):
-        """Write the seeded start on the device (lg_seed_state)."""
-        check(lib().lg_seed_state(self._h, seed, noise), "seed_state")
+    def seed_state(self, seed: int, noise: float = 0.1, member=None):
+        """Write the seeded start on the device: member 0 with the counters
+        reset (lg_seed_state), or one member of a batch (lg_seed_member)."""
+        if member is None:
+            check(lib().lg_seed_state(self._h, seed, noise), "seed_state")
+        else:
+            check(lib().lg_seed_member(self._h, member, seed, noise), "seed_member")
+
+    def set_member(self, member: int, zo: float, mean_p_force_x: float,
+                   mean_p_force_y: float = 0.0):
+        check(lib().lg_member_params(self._h, member, zo, mean_p_force_x, mean_p_force_y),
+              "member_params")
+
+    def set_active(self, n: int):
+        check(lib().lg_set_active(self._h, n), "set_active")
+
+    def batch_info(self):
+        a, b = C.c_int(), C.c_int()
+        check(lib().lg_batch_info(self._h, C.byref(a), C.byref(b)), "batch_info")
+        return a.value, b.value
+
+    def copy_member(self, src: int, dst: int):
+        check(lib().lg_copy_member(self._h, src, dst), "copy_member")
+        self.nturb[dst] = self.nturb[src]
 
     def set_counters(self, jt: int, jt_total: int):
         check(lib().lg_set_counters(self._h, jt, jt_total), "set_counters")
@@ -178,23 +206,32 @@ class Case:
     def step(self, nsteps: int = 1):
         check(lib().lg_step(self._h, nsteps), "step")
 
+    def step_graph(self, nsteps: int = 1):
+        """The same steps replayed from captured HIP graphs (lg_step_graph)."""
+        check(lib().lg_step_graph(self._h, nsteps), "step_graph")
+
+    def autotune(self, reps: int = 5) -> int:
+        n = C.c_int()
+        check(lib().lg_autotune(self._h, reps, C.byref(n)), "autotune")
+        return n.value
+
     # -- diagnostics --------------------------------------------------------
-    def _scalar(self, fn, what):
+    def _scalar(self, fn, member, what):
         out = C.c_double()
-        check(fn(self._h, C.byref(out)), what)
+        check(fn(self._h, member, C.byref(out)), what)
         return out.value
 
-    def rmsdiv(self):
-        return self._scalar(lib().lg_rmsdiv, "rmsdiv")
+    def rmsdiv(self, member: int = 0):
+        return self._scalar(lib().lg_rmsdiv_member, member, "rmsdiv")
 
-    def energy(self):
-        return self._scalar(lib().lg_energy, "energy")
+    def energy(self, member: int = 0):
+        return self._scalar(lib().lg_energy_member, member, "energy")
 
-    def max_cfl(self):
-        return self._scalar(lib().lg_max_cfl, "max_cfl")
+    def max_cfl(self, member: int = 0):
+        return self._scalar(lib().lg_max_cfl_member, member, "max_cfl")
 
-    def tau_wall_bot(self):
-        return self._scalar(lib().lg_tau_wall_bot, "tau_wall_bot")
+    def tau_wall_bot(self, member: int = 0):
+        return self._scalar(lib().lg_tau_wall_bot_member, member, "tau_wall_bot")
 
     # -- per-kernel timing --------------------------------------------------
     def timing(self, on: bool):
@@ -212,22 +249,44 @@ class Case:
 
     # -- actuator disks -----------------------------------------------------
     def turbines_init(self, specs, alpha1=1.5, alpha2=1.5, filter_cutoff=1e-2,
-                      adm_correction=True):
-        """specs: iterable of (xloc, yloc, height, dia, thk, theta1, theta2, Ct')."""
+                      adm_correction=True, member: int = 0):
+        """specs: iterable of (xloc, yloc, height, dia, thk, theta1, theta2, Ct');
+        replaces the member's disks (an empty list removes them)."""
         a = np.ascontiguousarray(np.asarray(specs, dtype=np.float64).reshape(-1, 8))
-        check(lib().lg_turbines_init(self._h, a.shape[0], a.ctypes.data_as(_lib._D), alpha1,
-                                     alpha2, filter_cutoff, int(adm_correction)), "turbines_init")
-        self.nturb = a.shape[0]
+        check(lib().lg_turbines_init_member(self._h, member, a.shape[0],
+                                            a.ctypes.data_as(_lib._D), alpha1, alpha2,
+                                            filter_cutoff, int(adm_correction)),
+              "turbines_init")
+        self.nturb[member] = a.shape[0]
 
     def turbines_forcing(self):
         check(lib().lg_turbines_forcing(self._h), "turbines_forcing")
 
-    def turbines_state(self):
-        ud = np.empty(self.nturb)
-        fn = np.empty(self.nturb)
-        check(lib().lg_turbines_get(self._h, ud.ctypes.data_as(_lib._D),
-                                    fn.ctypes.data_as(_lib._D), self.nturb), "turbines_get")
+    def turbines_state(self, member: int = 0):
+        n = self.nturb[member]
+        ud = np.empty(n)
+        fn = np.empty(n)
+        check(lib().lg_turbines_get_member(self._h, member, ud.ctypes.data_as(_lib._D),
+                                           fn.ctypes.data_as(_lib._D), n), "turbines_get")
         return ud, fn
+
+
+def tune_get(kind: str, precision: str, N: int, other: int):
+    """(n1, n2, per_wg) launch choice the library uses for a pass."""
+    a, b, s = C.c_int(), C.c_int(), C.c_int()
+    check(lib().lg_tune_get(_lib.PASS_KINDS.index(kind), LG_FP64 if precision == "fp64"
+                            else LG_MIXED, N, other, C.byref(a), C.byref(b), C.byref(s)),
+          "tune_get")
+    return a.value, b.value, s.value
+
+
+def tune_set(kind: str, precision: str, N: int, other: int, n1: int, n2: int, per_wg: int):
+    check(lib().lg_tune_set(_lib.PASS_KINDS.index(kind), LG_FP64 if precision == "fp64"
+                            else LG_MIXED, N, other, n1, n2, per_wg), "tune_set")
+
+
+def tune_clear():
+    check(lib().lg_tune_clear(), "tune_clear")
 
 
 def seeded_state(cfg: Config, seed: int, noise: float = 0.1):

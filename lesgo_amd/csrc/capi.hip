@@ -71,15 +71,23 @@ bool smooth(int n) {
 
 void check_field(int id) { REQUIRE(id >= 0 && id < LG_NFIELDS, "field id out of range"); }
 
+template <typename C>
+void check_member(const C& c, int m) {
+    REQUIRE(m >= 0 && m < c.M, "member index out of range");
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* lg_last_error(void) { return lg::g_err.c_str(); }
 
-int lg_create(const lg_params* p, lg_case** out) {
+int lg_create_batch(const lg_params* p, int members, lg_case** out) {
     return guarded([&] {
         REQUIRE(p && out, "null argument");
+        REQUIRE(members >= 1, "members must be >= 1");
+        REQUIRE((long)(p->Nz + 2) * members <= 65535,
+                "members * (Nz + 2) must not exceed 65535 (launch grid limit)");
         REQUIRE(p->nx >= 4 && p->ny >= 4 && p->nx % 4 == 0 && p->ny % 4 == 0,
                 "nx and ny must be multiples of 4 (3/2-rule grids must stay even)");
         REQUIRE(smooth(p->nx) && smooth(p->ny), "nx and ny must be of the form 2^a 3^b");
@@ -91,8 +99,8 @@ int lg_create(const lg_params* p, lg_case** out) {
         auto* h = new lg_case();
         h->precision = p->precision;
         try {
-            if (p->precision == LG_FP64) h->c64 = new lg::Case<double>(*p);
-            else h->c32 = new lg::Case<float>(*p);
+            if (p->precision == LG_FP64) h->c64 = new lg::Case<double>(*p, members);
+            else h->c32 = new lg::Case<float>(*p, members);
         } catch (...) {
             delete h;
             throw;
@@ -100,6 +108,8 @@ int lg_create(const lg_params* p, lg_case** out) {
         *out = h;
     });
 }
+
+int lg_create(const lg_params* p, lg_case** out) { return lg_create_batch(p, 1, out); }
 
 int lg_destroy(lg_case* h) {
     if (!h) return LG_OK;
@@ -120,41 +130,51 @@ int lg_sizes(const lg_case* h, int* nz, int* ld, int* ny) {
     return LG_OK;
 }
 
-int lg_upload(lg_case* h, int field, const double* host) {
+int lg_upload_member(lg_case* h, int m, int field, const double* host) {
     return dispatch(h, [&](auto& c) {
         check_field(field);
+        check_member(c, m);
         REQUIRE(host, "null host pointer");
         using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
         if constexpr (std::is_same<T, double>::value) {
-            LG_HIP(hipMemcpyAsync(c.fld[field], host, c.d.field * sizeof(double),
+            LG_HIP(hipMemcpyAsync(c.mem(field, m), host, c.d.field * sizeof(double),
                                   hipMemcpyHostToDevice, c.stream));
         } else {
             std::vector<T> tmp((size_t)c.d.field);
             for (long i = 0; i < c.d.field; ++i) tmp[i] = (T)host[i];
-            LG_HIP(hipMemcpyAsync(c.fld[field], tmp.data(), c.d.field * sizeof(T),
+            LG_HIP(hipMemcpyAsync(c.mem(field, m), tmp.data(), c.d.field * sizeof(T),
                                   hipMemcpyHostToDevice, c.stream));
         }
         LG_HIP(hipStreamSynchronize(c.stream));
     });
 }
 
-int lg_download(lg_case* h, int field, double* host) {
+int lg_download_member(lg_case* h, int m, int field, double* host) {
     return dispatch(h, [&](auto& c) {
         check_field(field);
+        check_member(c, m);
         REQUIRE(host, "null host pointer");
         using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
         if constexpr (std::is_same<T, double>::value) {
-            LG_HIP(hipMemcpyAsync(host, c.fld[field], c.d.field * sizeof(double),
+            LG_HIP(hipMemcpyAsync(host, c.mem(field, m), c.d.field * sizeof(double),
                                   hipMemcpyDeviceToHost, c.stream));
             LG_HIP(hipStreamSynchronize(c.stream));
         } else {
             std::vector<T> tmp((size_t)c.d.field);
-            LG_HIP(hipMemcpyAsync(tmp.data(), c.fld[field], c.d.field * sizeof(T),
+            LG_HIP(hipMemcpyAsync(tmp.data(), c.mem(field, m), c.d.field * sizeof(T),
                                   hipMemcpyDeviceToHost, c.stream));
             LG_HIP(hipStreamSynchronize(c.stream));
             for (long i = 0; i < c.d.field; ++i) host[i] = (double)tmp[i];
         }
     });
+}
+
+int lg_upload(lg_case* h, int field, const double* host) {
+    return lg_upload_member(h, 0, field, host);
+}
+
+int lg_download(lg_case* h, int field, double* host) {
+    return lg_download_member(h, 0, field, host);
 }
 
 namespace {
@@ -173,14 +193,16 @@ inline double uniform01(uint64_t seed, uint64_t fid, uint64_t i, uint64_t j, uin
 }
 }  // namespace
 
-int lg_seed_state(lg_case* h, long seed, double noise) {
+int lg_seed_member(lg_case* h, int m, long seed, double noise) {
     return dispatch(h, [&](auto& c) {
+        check_member(c, m);
         const lg::Dims& d = c.d;
+        const double zo = c.zo[m];
         using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
         std::vector<std::vector<T>> f(4, std::vector<T>((size_t)d.field, (T)0));
         for (int k = 1; k < d.nz; ++k) {
             const double zk = (k - 0.5) * d.dz;
-            const double ubar = (1.0 / d.prm.vonk) * std::log(zk / d.prm.zo);
+            const double ubar = (1.0 / d.prm.vonk) * std::log(zk / zo);
             const double amp = noise * ubar;
             for (int j = 0; j < d.ny; ++j)
                 for (int i = 0; i < d.nx; ++i) {
@@ -202,15 +224,19 @@ int lg_seed_state(lg_case* h, long seed, double noise) {
         }
         for (size_t o = P; o < (size_t)d.field; ++o) f[3][o] = (T)0.03;
         for (int id = 0; id < LG_NFIELDS; ++id)
-            LG_HIP(hipMemsetAsync(c.fld[id], 0, d.field * sizeof(T), c.stream));
+            LG_HIP(hipMemsetAsync(c.mem(id, m), 0, d.field * sizeof(T), c.stream));
         const int ids[4] = {LG_U, LG_V, LG_W, LG_CS_OPT2};
         for (int q = 0; q < 4; ++q)
-            LG_HIP(hipMemcpyAsync(c.fld[ids[q]], f[q].data(), d.field * sizeof(T),
+            LG_HIP(hipMemcpyAsync(c.mem(ids[q], m), f[q].data(), d.field * sizeof(T),
                                   hipMemcpyHostToDevice, c.stream));
         LG_HIP(hipStreamSynchronize(c.stream));
-        c.jt = 0;
-        c.jt_total = 0;
     });
+}
+
+int lg_seed_state(lg_case* h, long seed, double noise) {
+    const int rc = lg_seed_member(h, 0, seed, noise);
+    if (rc != LG_OK) return rc;
+    return lg_set_counters(h, 0, 0);
 }
 
 int lg_set_counters(lg_case* h, int jt, int jt_total) {
@@ -312,7 +338,7 @@ int lg_copy_field(lg_case* h, int src, int dst) {
         check_field(dst);
         using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
         if (src != dst)
-            LG_HIP(hipMemcpyAsync(c.fld[dst], c.fld[src], c.d.field * sizeof(T),
+            LG_HIP(hipMemcpyAsync(c.fld[dst], c.fld[src], c.Ma * c.d.field * sizeof(T),
                                   hipMemcpyDeviceToDevice, c.stream));
     });
 }
@@ -328,42 +354,36 @@ int lg_step(lg_case* h, int nsteps) {
     });
 }
 
-int lg_rmsdiv(lg_case* h, double* out) {
-    return dispatch(h, [&](auto& c) {
-        REQUIRE(out, "null output");
-        *out = c.rmsdiv();
-    });
-}
+#define LG_DIAG(NAME, FN)                                                  \
+    int NAME##_member(lg_case* h, int m, double* out) {                    \
+        return dispatch(h, [&](auto& c) {                                  \
+            REQUIRE(out, "null output");                                   \
+            check_member(c, m);                                            \
+            *out = c.FN(m);                                                \
+        });                                                                \
+    }                                                                      \
+    int NAME(lg_case* h, double* out) { return NAME##_member(h, 0, out); }
+LG_DIAG(lg_rmsdiv, rmsdiv)
+LG_DIAG(lg_energy, energy)
+LG_DIAG(lg_max_cfl, max_cfl)
+LG_DIAG(lg_tau_wall_bot, tau_wall_bot)
+#undef LG_DIAG
 
-int lg_energy(lg_case* h, double* out) {
+int lg_turbines_init_member(lg_case* h, int m, int nturb, const double* spec8, double alpha1,
+                            double alpha2, double filter_cutoff, int adm_correction) {
     return dispatch(h, [&](auto& c) {
-        REQUIRE(out, "null output");
-        *out = c.energy();
-    });
-}
-
-int lg_max_cfl(lg_case* h, double* out) {
-    return dispatch(h, [&](auto& c) {
-        REQUIRE(out, "null output");
-        *out = c.max_cfl();
-    });
-}
-
-int lg_tau_wall_bot(lg_case* h, double* out) {
-    return dispatch(h, [&](auto& c) {
-        REQUIRE(out, "null output");
-        *out = c.tau_wall_bot();
+        check_member(c, m);
+        REQUIRE(nturb >= 0 && (nturb == 0 || spec8), "bad turbine list");
+        REQUIRE(alpha1 > 0 && alpha2 > 0 && filter_cutoff >= 0, "bad turbine filter parameters");
+        for (int s = 0; s < nturb; ++s) REQUIRE(spec8[8 * s + 3] > 0, "turbine diameter must be > 0");
+        lg::turbines_init(c, m, nturb, spec8, alpha1, alpha2, filter_cutoff, adm_correction);
     });
 }
 
 int lg_turbines_init(lg_case* h, int nturb, const double* spec8, double alpha1, double alpha2,
                      double filter_cutoff, int adm_correction) {
-    return dispatch(h, [&](auto& c) {
-        REQUIRE(nturb >= 0 && (nturb == 0 || spec8), "bad turbine list");
-        REQUIRE(alpha1 > 0 && alpha2 > 0 && filter_cutoff >= 0, "bad turbine filter parameters");
-        for (int s = 0; s < nturb; ++s) REQUIRE(spec8[8 * s + 3] > 0, "turbine diameter must be > 0");
-        lg::turbines_init(c, nturb, spec8, alpha1, alpha2, filter_cutoff, adm_correction);
-    });
+    return lg_turbines_init_member(h, 0, nturb, spec8, alpha1, alpha2, filter_cutoff,
+                                   adm_correction);
 }
 
 int lg_turbines_forcing(lg_case* h) {
@@ -373,17 +393,101 @@ int lg_turbines_forcing(lg_case* h) {
     });
 }
 
-int lg_turbines_get(lg_case* h, double* u_d_T, double* f_n, int nturb) {
+int lg_turbines_get_member(lg_case* h, int m, double* u_d_T, double* f_n, int nturb) {
     return dispatch(h, [&](auto& c) {
-        REQUIRE(c.turb && nturb == c.turb->n, "turbine count mismatch");
-        std::vector<double> st(3 * nturb);
-        LG_HIP(hipMemcpyAsync(st.data(), c.turb->state, 3 * nturb * sizeof(double),
-                              hipMemcpyDeviceToHost, c.stream));
-        LG_HIP(hipStreamSynchronize(c.stream));
+        check_member(c, m);
+        REQUIRE(c.turb && nturb == c.mturb[m].n(), "turbine count mismatch");
+        c.pull_turbine_state();
+        const auto& st = c.mturb[m].state;
         for (int s = 0; s < nturb; ++s) {
             if (u_d_T) u_d_T[s] = st[3 * s + 1];
             if (f_n) f_n[s] = st[3 * s + 2];
         }
+    });
+}
+
+int lg_turbines_get(lg_case* h, double* u_d_T, double* f_n, int nturb) {
+    return lg_turbines_get_member(h, 0, u_d_T, f_n, nturb);
+}
+
+int lg_batch_info(const lg_case* h, int* members, int* active) {
+    if (!h) return LG_ERR_ARG;
+    const int M = h->c64 ? h->c64->M : h->c32->M;
+    const int Ma = h->c64 ? h->c64->Ma : h->c32->Ma;
+    if (members) *members = M;
+    if (active) *active = Ma;
+    return LG_OK;
+}
+
+int lg_set_active(lg_case* h, int active) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(active >= 1 && active <= c.M, "active member count out of range");
+        c.Ma = active;
+    });
+}
+
+int lg_member_params(lg_case* h, int m, double zo, double mean_p_force_x,
+                     double mean_p_force_y) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(zo > 0, "zo must be positive");
+        c.set_member_params(m, zo, mean_p_force_x, mean_p_force_y);
+    });
+}
+
+int lg_copy_member(lg_case* h, int src, int dst) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, src);
+        check_member(c, dst);
+        c.copy_member(src, dst);
+        LG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int lg_autotune(lg_case* h, int reps, int* npasses) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(reps >= 1, "reps must be >= 1");
+        const int n = lg::autotune(c, reps);
+        if (npasses) *npasses = n;
+    });
+}
+
+int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg) {
+    return guarded([&] {
+        REQUIRE(kind >= 0 && kind < lg::PK_NKINDS, "pass kind out of range");
+        REQUIRE(precision == LG_FP64 || precision == LG_MIXED, "bad precision");
+        const int ts = precision == LG_FP64 ? 8 : 4;
+        lg::PassChoice c;
+        if (!lg::tuned_choice(kind, ts, N, other, c))
+            c = ts == 8 ? lg::default_choice<double>(kind, N, other)
+                        : lg::default_choice<float>(kind, N, other);
+        if (n1) *n1 = c.n1;
+        if (n2) *n2 = c.n2;
+        if (per_wg) *per_wg = c.S;
+    });
+}
+
+int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg) {
+    return guarded([&] {
+        REQUIRE(kind >= 0 && kind < lg::PK_NKINDS, "pass kind out of range");
+        REQUIRE(precision == LG_FP64 || precision == LG_MIXED, "bad precision");
+        const lg::PassChoice c{n1, n2, per_wg};
+        const bool ok = precision == LG_FP64 ? lg::choice_ok<double>(kind, c, N, other)
+                                             : lg::choice_ok<float>(kind, c, N, other);
+        REQUIRE(ok, "launch choice not launchable for this pass");
+        lg::set_tuned_choice(kind, precision == LG_FP64 ? 8 : 4, N, other, c);
+    });
+}
+
+int lg_tune_clear(void) {
+    return guarded([&] { lg::clear_tuned_choices(); });
+}
+
+int lg_step_graph(lg_case* h, int nsteps) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+        for (int n = 0; n < nsteps; ++n) c.step_graph();
+        LG_HIP(hipGetLastError());
     });
 }
 

@@ -1,11 +1,15 @@
 // Subsystem drivers: each method replaces one reference subroutine and runs
-// on the case's stream.  See the kernel headers for the per-loop citations.
+// on the case's stream over every stepped member of the batch.  See the
+// kernel headers for the per-loop citations.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "case.hpp"
-#include "fft2.hpp"
+#include "passes.hpp"
 #include "lagrange.hpp"
 #include "pointwise.hpp"
 #include "pressure.hpp"
@@ -14,71 +18,6 @@ namespace lg {
 
 static constexpr double PI_D = 3.1415926535897932384626433;   // param.f90:94
 static constexpr int BLK = 256;
-static constexpr size_t LDS_MAX = 64 * 1024;
-
-#define LG_SPLITS(X) X(4, 4) X(4, 6) X(4, 8) X(6, 8) X(8, 8) X(8, 9) X(8, 12) X(8, 16) \
-    X(12, 12) X(12, 16) X(16, 16) X(16, 24)
-
-template <typename T>
-static bool v2_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
-#define LG_CASE(A, B)                                                                    \
-    if (n1 == A && n2 == B) {                                                            \
-        hipLaunchKernelGGL((k_rows_r2c_v2<T, A, B>), g, dim3(BLK), lds, s, a);           \
-        return true;                                                                     \
-    }
-    LG_SPLITS(LG_CASE)
-#undef LG_CASE
-    return false;
-}
-
-template <typename T>
-static bool v2_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
-#define LG_CASE(A, B)                                                                    \
-    if (n1 == A && n2 == B) {                                                            \
-        hipLaunchKernelGGL((k_rows_c2r_v2<T, A, B>), g, dim3(BLK), lds, s, a);           \
-        return true;                                                                     \
-    }
-    LG_SPLITS(LG_CASE)
-#undef LG_CASE
-    return false;
-}
-
-template <typename T, int DIR>
-static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
-#define LG_CASE(A, B)                                                                    \
-    if (n1 == A && n2 == B) {                                                            \
-        hipLaunchKernelGGL((k_cols_v2<T, A, B, DIR>), g, dim3(BLK), lds, s, a);          \
-        return true;                                                                     \
-    }
-    LG_SPLITS(LG_CASE)
-#undef LG_CASE
-    return false;
-}
-
-// Launch a column pass: the register four-step kernel when the length has a
-// split, the generic LDS kernel otherwise.  Sets c.TC / c.stride.
-template <typename T, int DIR>
-static void launch_cols(ColArgs<T>& c, int njobs, int nlev, hipStream_t s) {
-    const int N = c.plan.N;
-    int n1, n2;
-    if (split_of(N, n1, n2)) {
-        const int YS = n1 * (n2 + 1) + 1;
-        int TC = 16;
-        while (TC > 1 && (size_t)TC * YS * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
-        c.TC = std::min(TC, c.ncols);
-        c.stride = YS;
-        const size_t lds = (size_t)c.TC * YS * sizeof(cplx<T>);
-        dim3 grid(njobs, (c.ncols + c.TC - 1) / c.TC, nlev);
-        if (v2_cols<T, DIR>(n1, n2, grid, lds, s, c)) return;
-    }
-    int TC = 16;
-    while (TC > 1 && (size_t)TC * (N + 1) * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
-    c.TC = std::min(TC, c.ncols);
-    c.stride = N + 1;
-    const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
-    dim3 grid(njobs, (c.ncols + c.TC - 1) / c.TC, nlev);
-    hipLaunchKernelGGL((k_cols<T, DIR>), grid, dim3(BLK), lds, s, c);
-}
 
 Dims::Dims(const lg_params& p) : prm(p) {
     nx = p.nx;
@@ -187,25 +126,34 @@ static T* dalloc(long n) {
 }
 
 template <typename T>
-Case<T>::Case(const lg_params& p) : d(p) {
+Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
+    msp = (long)(d.nz + 1) * d.splane;
+    mbp = (long)(d.nz + 1) * d.bplane;
+    mbsp = (long)(d.nz + 1) * d.bsplane;
     LG_HIP(hipStreamCreate(&stream));
-    for (int i = 0; i < LG_NFIELDS; ++i) fld[i] = dalloc<T>(d.field);
-    for (int i = 0; i < NSPEC; ++i) spec[i] = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
-    for (int i = 0; i < NBIG; ++i) big[i] = dalloc<T>((long)(d.nz + 1) * d.bplane);
-    for (int i = 0; i < 6; ++i) bigspec[i] = dalloc<cplx<T>>((long)(d.nz + 1) * d.bsplane);
-    for (int i = 0; i < NTMP; ++i) tmp[i] = dalloc<T>(d.field);
-    sU = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
-    sV = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
-    sW = dalloc<cplx<T>>((long)(d.nz + 1) * d.splane);
-    gam = dalloc<double>((long)(d.nz + 1) * d.splane);
-    usc = dalloc<double2>((long)(d.nz + 1) * d.splane);
+    for (int i = 0; i < LG_NFIELDS; ++i) fld[i] = dalloc<T>(M * d.field);
+    for (int i = 0; i < NSPEC; ++i) spec[i] = dalloc<cplx<T>>(M * msp);
+    for (int i = 0; i < NBIG; ++i) big[i] = dalloc<T>(M * mbp);
+    for (int i = 0; i < 6; ++i) bigspec[i] = dalloc<cplx<T>>(M * mbsp);
+    for (int i = 0; i < NTMP; ++i) tmp[i] = dalloc<T>(M * d.field);
+    sU = dalloc<cplx<T>>(M * msp);
+    sV = dalloc<cplx<T>>(M * msp);
+    sW = dalloc<cplx<T>>(M * msp);
+    gam = dalloc<double>(M * msp);
+    usc = dalloc<double2>(M * msp);
     dev_err = dalloc<int>(1);
     lag_chunk = std::min(d.nz, 64);
-    lag_phys = dalloc<T>(15L * lag_chunk * d.plane);
-    lag_spec = dalloc<cplx<T>>(21L * lag_chunk * d.splane);
-    lag_filt = dalloc<T>(42L * lag_chunk * d.plane);
-    Ftmp = dalloc<T>(4L * d.field);
+    lag_phys = dalloc<T>(15L * M * lag_chunk * d.plane);
+    lag_spec = dalloc<cplx<T>>(21L * M * lag_chunk * d.splane);
+    lag_filt = dalloc<T>(42L * M * lag_chunk * d.plane);
+    Ftmp = dalloc<T>(4L * M * d.field);
     red = dalloc<double>(1024);
+    d_denom = dalloc<double>(M);
+    d_mpf = dalloc<double>(2L * M);
+    zo.assign(M, p.zo);
+    mpf.resize(2 * M);
+    for (int m = 0; m < M; ++m) set_member_params(m, p.zo, p.mean_p_force_x, p.mean_p_force_y);
+    mturb.resize(M);
     px.build(d.nx);
     py.build(d.ny);
     pbx.build(d.nx2);
@@ -237,6 +185,8 @@ Case<T>::Case(const lg_params& p) : d(p) {
 template <typename T>
 Case<T>::~Case() {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    turb.reset();
     auto fr = [](void* p) {
         if (p) (void)hipFree(p);
     };
@@ -256,125 +206,45 @@ Case<T>::~Case() {
     fr(lag_filt);
     fr(Ftmp);
     fr(red);
+    fr(d_denom);
+    fr(d_mpf);
     fr(G1);
     fr(G2);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+void Case<T>::set_member_params(int m, double zo_m, double mpfx, double mpfy) {
+    zo[m] = zo_m;
+    mpf[2 * m] = mpfx;
+    mpf[2 * m + 1] = mpfy;
+    const double den = std::log(0.5 * d.dz / zo_m);
+    LG_HIP(hipMemcpyAsync(d_denom + m, &den, sizeof(double), hipMemcpyHostToDevice, stream));
+    LG_HIP(hipMemcpyAsync(d_mpf + 2 * m, &mpf[2 * m], 2 * sizeof(double), hipMemcpyHostToDevice,
+                          stream));
+    LG_HIP(hipStreamSynchronize(stream));
 }
 
 // ---------------------------------------------------------------------------
 // Transform helpers
 // ---------------------------------------------------------------------------
 template <typename T>
-static int rows_per_wg(int N, int npairs) {
-    int S = std::max(1, 2048 / N);
-    while (S > 1 && (size_t)S * (N + 1) * sizeof(cplx<T>) > LDS_MAX) --S;
-    return std::min(S, npairs);
-}
-
-template <typename T>
-static int cols_per_tile(int N, int ncols) {
-    int TC = 16;
-    while (TC > 1 && (size_t)TC * (N + 1) * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
-    return std::min(TC, ncols);
-}
-
-template <typename T>
-static void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* const* src,
-                            const T* scales, long src_plane, int src_ld, int src_lev0,
-                            cplx<T>* const* dst, long dst_plane, int dst_lev0, int nrows,
-                            int ncols, int nlev, hipStream_t s) {
-    for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
-        const int nj = std::min(MAX_JOBS, njobs - j0);
-        RowR2CArgs<T> a{};
-        a.plan = pl;
-        for (int i = 0; i < nj; ++i) {
-            a.src[i] = src[j0 + i];
-            a.dst[i] = dst[j0 + i];
-            a.scale[i] = scales ? scales[j0 + i] : (T)1;
-        }
-        a.src_plane = src_plane;
-        a.dst_plane = dst_plane;
-        a.src_ld = src_ld;
-        a.nrows = nrows;
-        a.ncols = ncols;
-        a.lev0_src = src_lev0;
-        a.lev0_dst = dst_lev0;
-        int n1, n2;
-        const bool v2 = split_of(pl.N, n1, n2);
-        size_t lds;
-        if (v2) {
-            const int YS = n1 * (n2 + 1) + 1;
-            int S = std::min(std::min(BLK / n1, 16), nrows / 2);
-            while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_MAX) --S;
-            a.S = S;
-            a.stride = YS;
-            lds = (size_t)a.S * YS * sizeof(cplx<T>);
-        } else {
-            a.stride = pl.N + 1;
-            a.S = rows_per_wg<T>(pl.N, nrows / 2);
-            lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
-        }
-        dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev, nj);
-        const double bytes = (double)nj * nlev * nrows * ((double)pl.N * sizeof(T) +
-                                                           (double)ncols * sizeof(cplx<T>));
-        tm.run(s, KK_ROWS_R2C, bytes, [&] {
-            if (!(v2 && v2_r2c<T>(n1, n2, grid, lds, s, a)))
-                hipLaunchKernelGGL(k_rows_r2c<T>, grid, dim3(BLK), lds, s, a);
-        });
-    }
-}
-
-template <typename T>
-static void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs,
-                            const cplx<T>* const* src, long src_plane, int src_lev0,
-                            T* const* dst, long dst_plane, int dst_ld, int dst_lev0, int nrows,
-                            int ncols, int nlev, hipStream_t s) {
-    for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
-        const int nj = std::min(MAX_JOBS, njobs - j0);
-        RowC2RArgs<T> a{};
-        a.plan = pl;
-        for (int i = 0; i < nj; ++i) {
-            a.src[i] = src[j0 + i];
-            a.dst[i] = dst[j0 + i];
-        }
-        a.src_plane = src_plane;
-        a.dst_plane = dst_plane;
-        a.dst_ld = dst_ld;
-        a.nrows = nrows;
-        a.ncols = ncols;
-        a.lev0_src = src_lev0;
-        a.lev0_dst = dst_lev0;
-        int n1, n2;
-        const bool v2 = split_of(pl.N, n1, n2);
-        size_t lds;
-        if (v2) {
-            const int YS = n1 * (n2 + 1) + 1;
-            int S = std::min(16, nrows / 2);
-            while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_MAX) --S;
-            a.S = S;
-            a.stride = YS;
-            lds = (size_t)a.S * YS * sizeof(cplx<T>);
-        } else {
-            a.stride = pl.N + 1;
-            a.S = rows_per_wg<T>(pl.N, nrows / 2);
-            lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
-        }
-        dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev, nj);
-        const double bytes = (double)nj * nlev * nrows * ((double)ncols * sizeof(cplx<T>) +
-                                                           (double)dst_ld * sizeof(T));
-        tm.run(s, KK_ROWS_C2R, bytes, [&] {
-            if (!(v2 && v2_c2r<T>(n1, n2, grid, lds, s, a)))
-                hipLaunchKernelGGL(k_rows_c2r<T>, grid, dim3(BLK), lds, s, a);
-        });
-    }
+static void col_common(ColArgs<T>& c, const Dims& d, const T* G1, const T* G2) {
+    c.ncols = d.NK;
+    c.ny = d.ny;
+    c.kxs = (T)(2.0 * PI_D / d.prm.lx);
+    c.kys = (T)(2.0 * PI_D / d.prm.ly);
+    c.G1 = G1;
+    c.G2 = G2;
 }
 
 template <typename T>
 void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
-                    cplx<T>* const* dst, long dst_plane, int dst_lev0, int nlev) {
+                    long src_ms, cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms,
+                    int nlev) {
     if (nlev <= 0) return;
-    launch_rows_r2c<T>(timer, px.plan, njobs, src, scales, src_plane, d.ld, src_lev0, dst, dst_plane,
-                       dst_lev0, d.ny, d.NK, nlev, stream);
+    launch_rows_r2c<T>(timer, px.plan, njobs, src, scales, src_plane, d.ld, src_lev0, src_ms, dst,
+                       dst_plane, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr);
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
         ColArgs<T> c{};
@@ -386,29 +256,21 @@ void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_pl
         }
         c.src_plane = c.dst_plane = dst_plane;
         c.lev0_src = c.lev0_dst = dst_lev0;
-        c.ncols = d.NK;
-        c.ny = d.ny;
+        c.src_mstride = c.dst_mstride = dst_ms;
         c.map = MAP_ID;
         c.zero_nyq = 1;
-        c.TC = cols_per_tile<T>(d.ny, d.NK);
-        c.stride = d.ny + 1;
-        c.kxs = (T)(2.0 * PI_D / d.prm.lx);
-        c.kys = (T)(2.0 * PI_D / d.prm.ly);
-        c.G1 = G1;
-        c.G2 = G2;
-        const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
-        dim3 grid(nj, (d.NK + c.TC - 1) / c.TC, nlev);
-        const int nj_ = nj, nlev_ = nlev;
-        const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny + d.ny);
-        timer.run(stream, KK_COLS_FWD, cbytes, [&] { launch_cols<T, -1>(c, nj_, nlev_, stream); });
+        col_common(c, d, G1, G2);
+        launch_cols<T, -1>(timer, c, nj, nlev, Ma, d.ny, stream, nullptr);
     }
 }
 
 template <typename T>
 void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane,
-                    int src_lev0, T* const* dst, long dst_plane, int dst_lev0, int nlev,
-                    cplx<T>* const* tmpspec, long tmp_plane, const cplx<T>* const* src2) {
+                    int src_lev0, long src_ms, T* const* dst, long dst_plane, int dst_lev0,
+                    long dst_ms, int nlev, cplx<T>* const* tmpspec, const cplx<T>* const* src2) {
     if (nlev <= 0) return;
+    // column-pass output goes to tmpspec[i], planes 0..nlev-1, members packed
+    const long tms = (long)nlev * d.splane;
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
         ColArgs<T> c{};
@@ -420,27 +282,18 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
             c.op[i] = ops[j0 + i];
         }
         c.src_plane = src_plane;
-        c.dst_plane = tmp_plane;
+        c.dst_plane = d.splane;
         c.lev0_src = src_lev0;
         c.lev0_dst = 0;
-        c.ncols = d.NK;
-        c.ny = d.ny;
+        c.src_mstride = src_ms;
+        c.dst_mstride = tms;
         c.map = MAP_ID;
         c.zero_nyq = 0;
-        c.TC = cols_per_tile<T>(d.ny, d.NK);
-        c.stride = d.ny + 1;
-        c.kxs = (T)(2.0 * PI_D / d.prm.lx);
-        c.kys = (T)(2.0 * PI_D / d.prm.ly);
-        c.G1 = G1;
-        c.G2 = G2;
-        const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
-        dim3 grid(nj, (d.NK + c.TC - 1) / c.TC, nlev);
-        const int nj_ = nj, nlev_ = nlev;
-        const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny + d.ny);
-        timer.run(stream, KK_COLS_INV, cbytes, [&] { launch_cols<T, +1>(c, nj_, nlev_, stream); });
+        col_common(c, d, G1, G2);
+        launch_cols<T, +1>(timer, c, nj, nlev, Ma, d.ny, stream, nullptr);
         std::vector<const cplx<T>*> ts(tmpspec, tmpspec + nj);
-        launch_rows_c2r<T>(timer, px.plan, nj, ts.data(), tmp_plane, 0, dst + j0, dst_plane, d.ld,
-                           dst_lev0, d.ny, d.NK, nlev, stream);
+        launch_rows_c2r<T>(timer, px.plan, nj, ts.data(), d.splane, 0, tms, dst + j0, dst_plane,
+                           d.ld, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr);
     }
 }
 
@@ -457,29 +310,22 @@ void Case<T>::pad_inv_big(int njobs, const cplx<T>* const* src, int lev0, int nl
     c.dst_plane = d.bsplane;
     c.lev0_src = lev0;
     c.lev0_dst = lev0;
-    c.ncols = d.NK;
-    c.ny = d.ny;
+    c.src_mstride = msp;
+    c.dst_mstride = mbsp;
     c.map = MAP_PAD;
     c.zero_nyq = 0;
-    c.TC = cols_per_tile<T>(d.ny2, d.NK);
-    c.stride = d.ny2 + 1;
-    c.G1 = G1;
-    c.G2 = G2;
-    const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
-    dim3 grid(njobs, (d.NK + c.TC - 1) / c.TC, nlev);
-    const int nj_ = njobs, nlev_ = nlev;
-        const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny + d.ny2);
-        timer.run(stream, KK_COLS_INV, cbytes, [&] { launch_cols<T, +1>(c, nj_, nlev_, stream); });
+    col_common(c, d, G1, G2);
+    launch_cols<T, +1>(timer, c, njobs, nlev, Ma, d.ny, stream, nullptr);
     std::vector<const cplx<T>*> bs(bigspec, bigspec + njobs);
-    launch_rows_c2r<T>(timer, pbx.plan, njobs, bs.data(), d.bsplane, lev0, big, d.bplane, d.ld_big, lev0,
-                       d.ny2, d.NK, nlev, stream);
+    launch_rows_c2r<T>(timer, pbx.plan, njobs, bs.data(), d.bsplane, lev0, mbsp, big, d.bplane,
+                       d.ld_big, lev0, mbp, d.ny2, d.NK, nlev, Ma, stream, nullptr);
 }
 
 template <typename T>
 void Case<T>::fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst, int lev0,
                             int nlev) {
-    launch_rows_r2c<T>(timer, pbx.plan, njobs, src, nullptr, d.bplane, d.ld_big, lev0, bigspec,
-                       d.bsplane, lev0, d.ny2, d.NK, nlev, stream);
+    launch_rows_r2c<T>(timer, pbx.plan, njobs, src, nullptr, d.bplane, d.ld_big, lev0, mbp,
+                       bigspec, d.bsplane, lev0, mbsp, d.ny2, d.NK, nlev, Ma, stream, nullptr);
     ColArgs<T> c{};
     c.plan = pby.plan;
     for (int i = 0; i < njobs; ++i) {
@@ -491,41 +337,49 @@ void Case<T>::fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst,
     c.dst_plane = d.splane;
     c.lev0_src = lev0;
     c.lev0_dst = lev0;
-    c.ncols = d.NK;
-    c.ny = d.ny;
+    c.src_mstride = mbsp;
+    c.dst_mstride = msp;
     c.map = MAP_UNPAD;
     c.zero_nyq = 1;
-    c.TC = cols_per_tile<T>(d.ny2, d.NK);
-    c.stride = d.ny2 + 1;
-    c.G1 = G1;
-    c.G2 = G2;
-    const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
-    dim3 grid(njobs, (d.NK + c.TC - 1) / c.TC, nlev);
-    const int nj_ = njobs, nlev_ = nlev;
-        const double cbytes = (double)nj_ * nlev_ * c.ncols * sizeof(cplx<T>) * (d.ny2 + d.ny);
-        timer.run(stream, KK_COLS_FWD, cbytes, [&] { launch_cols<T, -1>(c, nj_, nlev_, stream); });
+    col_common(c, d, G1, G2);
+    launch_cols<T, -1>(timer, c, njobs, nlev, Ma, d.ny2, stream, nullptr);
 }
 
 // ---------------------------------------------------------------------------
 // Small helpers
 // ---------------------------------------------------------------------------
-static inline Geo geo_of(const Dims& d) { return Geo{d.nx, d.ny, d.nz, d.ld, d.plane}; }
+#define GEO(k0, nlev) (Geo{d.nx, d.ny, d.nz, d.ld, d.plane, (k0), (nlev), d.field})
 
-static inline dim3 grid_ij(int W, int ny, int nlev) {
-    return dim3((unsigned)((W * ny + BLK - 1) / BLK), (unsigned)nlev);
+static inline dim3 grid_ij(int W, int ny, int nlev, int M) {
+    return dim3((unsigned)((W * ny + BLK - 1) / BLK), (unsigned)(nlev * M));
 }
 
 template <typename T>
 void Case<T>::fill_level(int id, int k, double v) {
     T* p = fld[id] + (long)k * d.plane;
-    hipLaunchKernelGGL(k_fill_planes<T>, dim3((unsigned)((d.plane + BLK - 1) / BLK)), dim3(BLK),
-                       0, stream, p, d.plane, d.plane, (T)v);
+    hipLaunchKernelGGL(k_fill_planes<T>, dim3((unsigned)((d.plane + BLK - 1) / BLK), Ma),
+                       dim3(BLK), 0, stream, p, d.field, d.plane, (T)v);
+}
+
+// dst[q] = src[q], q < n (a kernel, so that captured steps hold no copy nodes)
+template <typename T>
+static void copy_dev(const T* src, T* dst, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_plane<T>, dim3((unsigned)((n + BLK - 1) / BLK), 1), dim3(BLK), 0, s,
+                       src, dst, 0L, n);
 }
 
 template <typename T>
-static void fill_ptr(T* p, long n, double v, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_planes<T>, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, s,
-                       p, n, n, (T)v);
+void Case<T>::copy_member(int src, int dst) {
+    if (src == dst) return;
+    for (int id = 0; id < LG_NFIELDS; ++id)
+        LG_HIP(hipMemcpyAsync(mem(id, dst), mem(id, src), d.field * sizeof(T),
+                              hipMemcpyDeviceToDevice, stream));
+    set_member_params(dst, zo[src], mpf[2 * src], mpf[2 * src + 1]);
+    if (turb || mturb[dst].n() || mturb[src].n()) {
+        pull_turbine_state();
+        mturb[dst] = mturb[src];
+        rebuild_turbines();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -537,12 +391,12 @@ void Case<T>::filt_da(int f, int dfdx, int dfdy, cplx<T>* keep) {
     const T* src[1] = {fld[f]};
     cplx<T>* sp = keep ? keep : spec[0];
     cplx<T>* S0[1] = {sp};
-    fwd2d(1, src, &sc, d.plane, 0, S0, d.splane, 0, d.nz + 1);
+    fwd2d(1, src, &sc, d.plane, 0, d.field, S0, d.splane, 0, msp, d.nz + 1);
     const cplx<T>* s3[3] = {sp, sp, sp};
     const int ops[3] = {OP_NONE, OP_IKX, OP_IKY};
     T* dst[3] = {fld[f], fld[dfdx], fld[dfdy]};
     cplx<T>* tsp[3] = {spec[1], spec[2], spec[3]};
-    inv2d(3, s3, ops, d.splane, 0, dst, d.plane, 0, d.nz + 1, tsp, d.splane);
+    inv2d(3, s3, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, d.nz + 1, tsp);
 }
 
 template <typename T>
@@ -550,21 +404,20 @@ void Case<T>::ddxy(int f, int dfdx, int dfdy) {
     const T sc = (T)(1.0 / (d.nx * d.ny));
     const T* src[1] = {fld[f]};
     cplx<T>* S0[1] = {spec[0]};
-    fwd2d(1, src, &sc, d.plane, 0, S0, d.splane, 0, d.nz + 1);
+    fwd2d(1, src, &sc, d.plane, 0, d.field, S0, d.splane, 0, msp, d.nz + 1);
     const cplx<T>* s2[2] = {spec[0], spec[0]};
     int ops[2] = {OP_IKX, OP_IKY};
     T* dst[2] = {dfdx >= 0 ? fld[dfdx] : nullptr, dfdy >= 0 ? fld[dfdy] : nullptr};
     cplx<T>* tsp[2] = {spec[1], spec[2]};
+    int nj = 2;
     if (dfdx < 0) {
-        s2[0] = spec[0];
         ops[0] = OP_IKY;
         dst[0] = fld[dfdy];
-        inv2d(1, s2, ops, d.splane, 0, dst, d.plane, 0, d.nz + 1, tsp, d.splane);
+        nj = 1;
     } else if (dfdy < 0) {
-        inv2d(1, s2, ops, d.splane, 0, dst, d.plane, 0, d.nz + 1, tsp, d.splane);
-    } else {
-        inv2d(2, s2, ops, d.splane, 0, dst, d.plane, 0, d.nz + 1, tsp, d.splane);
+        nj = 1;
     }
+    inv2d(nj, s2, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, d.nz + 1, tsp);
 }
 
 template <typename T>
@@ -575,9 +428,8 @@ void Case<T>::ddy(int f, int dfdy) { ddxy(f, -1, dfdy); }
 
 template <typename T>
 void Case<T>::ddz_uv(int f, int dfdz) {
-    const Geo g = geo_of(d);
-    hipLaunchKernelGGL(k_ddz_uv<T>, grid_ij(d.nx, d.ny, d.nz), dim3(BLK), 0, stream, fld[f],
-                       fld[dfdz], g, (T)(1.0 / d.dz));
+    hipLaunchKernelGGL(k_ddz_uv<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, fld[f],
+                       fld[dfdz], GEO(1, d.nz), (T)(1.0 / d.dz));
     fill_level(dfdz, 0, BOGUS);
     fill_level(dfdz, 1, BOGUS);
     fill_level(dfdz, d.nz, BOGUS);
@@ -585,23 +437,23 @@ void Case<T>::ddz_uv(int f, int dfdz) {
 
 template <typename T>
 void Case<T>::ddz_w(int f, int dfdz) {
-    const Geo g = geo_of(d);
-    hipLaunchKernelGGL(k_ddz_w<T>, grid_ij(d.nx, d.ny, d.nz), dim3(BLK), 0, stream, fld[f],
-                       fld[dfdz], g, (T)(1.0 / d.dz));
+    hipLaunchKernelGGL(k_ddz_w<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, fld[f],
+                       fld[dfdz], GEO(0, d.nz), (T)(1.0 / d.dz));
     fill_level(dfdz, 0, BOGUS);
     fill_level(dfdz, d.nz, BOGUS);
 }
 
+// One plane per member (member stride = field) filtered by op.
 template <typename T>
 void Case<T>::test_filter_plane(const T* in, T* out, int op) {
     const T* src[1] = {in};
     const T one = (T)1;
     cplx<T>* S0[1] = {spec[4]};
-    fwd2d(1, src, &one, d.plane, 0, S0, d.splane, 0, 1);
+    fwd2d(1, src, &one, d.plane, 0, d.field, S0, d.splane, 0, msp, 1);
     const cplx<T>* s1[1] = {spec[4]};
     T* dst[1] = {out};
     cplx<T>* tsp[1] = {spec[5]};
-    inv2d(1, s1, &op, d.splane, 0, dst, d.plane, 0, 1, tsp, d.splane);
+    inv2d(1, s1, &op, d.splane, 0, msp, dst, d.plane, 0, d.field, 1, tsp);
 }
 
 // ---------------------------------------------------------------------------
@@ -613,17 +465,16 @@ void Case<T>::wallstress() {
     const T* src[2] = {lev(LG_U, 1), lev(LG_V, 1)};
     const T sc[2] = {one, one};
     cplx<T>* S0[2] = {spec[4], spec[5]};
-    fwd2d(2, src, sc, d.plane, 0, S0, d.splane, 0, 1);
+    fwd2d(2, src, sc, d.plane, 0, d.field, S0, d.splane, 0, msp, 1);
     const cplx<T>* s2[2] = {spec[4], spec[5]};
     const int ops[2] = {OP_G1, OP_G1};
     T* dst[2] = {tmp[0], tmp[1]};
     cplx<T>* tsp[2] = {spec[6], spec[7]};
-    inv2d(2, s2, ops, d.splane, 0, dst, d.plane, 0, 1, tsp, d.splane);
-    const Geo g = geo_of(d);
-    hipLaunchKernelGGL(k_wall_lbc<T>, grid_ij(d.nx, d.ny, 1), dim3(BLK), 0, stream, tmp[0],
-                       tmp[1], lev(LG_U, 1), lev(LG_V, 1), lev(LG_TXZ, 1), lev(LG_TYZ, 1),
-                       lev(LG_DUDZ, 1), lev(LG_DVDZ, 1), g, std::log(0.5 * d.dz / d.prm.zo),
-                       d.prm.vonk, d.dz);
+    inv2d(2, s2, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, 1, tsp);
+    hipLaunchKernelGGL(k_wall_lbc<T>, grid_ij(d.nx, d.ny, 1, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[0], (const T*)tmp[1], d.field, (const T*)fld[LG_U],
+                       (const T*)fld[LG_V], fld[LG_TXZ], fld[LG_TYZ], fld[LG_DUDZ], fld[LG_DVDZ],
+                       GEO(1, 1), (const double*)d_denom, d.prm.vonk, d.dz);
     fill_level(LG_TXZ, d.nz, 0.0);
     fill_level(LG_TYZ, d.nz, 0.0);
     fill_level(LG_DUDZ, d.nz, 0.0);
@@ -638,8 +489,8 @@ void Case<T>::calc_sij() {
     SijArgs<T> a{fld[LG_DUDX], fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDY],
                  fld[LG_DVDZ], fld[LG_DWDX], fld[LG_DWDY], fld[LG_DWDZ], fld[LG_S11],
                  fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33]};
-    hipLaunchKernelGGL(k_calc_sij<T>, grid_ij(d.nx, d.ny, d.nz), dim3(BLK), 0, stream, a,
-                       geo_of(d));
+    hipLaunchKernelGGL(k_calc_sij<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
+                       GEO(1, d.nz));
 }
 
 template <typename T>
@@ -649,8 +500,8 @@ void Case<T>::sgs_stag() {
     TauArgs<T> a{fld[LG_S11], fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33],
                  fld[LG_CS_OPT2], fld[LG_NU_T], fld[LG_TXX], fld[LG_TXY], fld[LG_TYY],
                  fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ], (T)(d.delta * d.delta), (T)d.prm.nu};
-    hipLaunchKernelGGL(k_sgs_tau<T>, grid_ij(d.nx, d.ny, d.nz), dim3(BLK), 0, stream, a,
-                       geo_of(d));
+    hipLaunchKernelGGL(k_sgs_tau<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
+                       GEO(1, d.nz));
     for (int id : {LG_TXX, LG_TXY, LG_TXZ, LG_TYY, LG_TYZ, LG_TZZ}) fill_level(id, 0, BOGUS);
     for (int id : {LG_TXX, LG_TXY, LG_TYY, LG_TZZ}) fill_level(id, d.nz, BOGUS);
 }
@@ -661,21 +512,19 @@ void Case<T>::sgs_stag() {
 template <typename T>
 void Case<T>::interpolag_sdep() {
     const int ids[4] = {LG_F_LM, LG_F_MM, LG_F_QN, LG_F_NN};
-    for (int q = 0; q < 4; ++q)
-        LG_HIP(hipMemcpyAsync(Ftmp + q * d.field, fld[ids[q]], d.field * sizeof(T),
-                              hipMemcpyDeviceToDevice, stream));
+    const long Fq = (long)M * d.field;
+    for (int q = 0; q < 4; ++q) copy_dev<T>(fld[ids[q]], Ftmp + q * Fq, Ma * d.field, stream);
     InterpGeo g{d.nx, d.ny, d.nz, d.ld, d.plane, d.dx, d.dy, d.dz,
-                d.prm.lx, d.prm.ly, d.prm.lz, d.lagran_dt};
-    hipLaunchKernelGGL(k_interpolag<T>, grid_ij(d.nx, d.ny, d.nz), dim3(BLK), 0, stream,
-                       fld[LG_U], fld[LG_V], fld[LG_W], Ftmp, Ftmp + d.field, Ftmp + 2 * d.field,
-                       Ftmp + 3 * d.field, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN],
-                       fld[LG_F_NN], g);
+                d.prm.lx, d.prm.ly, d.prm.lz, d.lagran_dt, d.field};
+    hipLaunchKernelGGL(k_interpolag<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+                       fld[LG_U], fld[LG_V], fld[LG_W], Ftmp, Ftmp + Fq, Ftmp + 2 * Fq,
+                       Ftmp + 3 * Fq, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN], fld[LG_F_NN], g);
 }
 
 template <typename T>
 void Case<T>::lagrange_sdep() {
     interpolag_sdep();
-    const Geo g = geo_of(d);
+    const Geo g = GEO(1, d.nz);
     const double tf1 = 2.0, tf2 = 4.0;
     LagUpd up{};
     up.opftdelta = 1.5 * d.delta;
@@ -690,20 +539,30 @@ void Case<T>::lagrange_sdep() {
     const int sids[6] = {LG_S11, LG_S12, LG_S13, LG_S22, LG_S23, LG_S33};
     for (int k0 = 1; k0 <= d.nz; k0 += lag_chunk) {
         const int nc = std::min(lag_chunk, d.nz - k0 + 1);
-        hipLaunchKernelGGL(k_lag_prep<T>, grid_ij(d.nx, d.ny, nc), dim3(BLK), 0, stream,
+        hipLaunchKernelGGL(k_lag_prep<T>, grid_ij(d.nx, d.ny, nc, Ma), dim3(BLK), 0, stream,
                            fld[LG_U], fld[LG_V], fld[LG_W], fld[LG_S11], fld[LG_S12],
                            fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33], lag_phys, g, k0,
-                           nc);
-        const long Qp = (long)nc * d.plane;
-        const long Qs = (long)nc * d.splane;
-        // slots 0-8 prep, 9-14 S_ij, 15-20 |S|S_ij
-        const T* src[21];
+                           nc, Ma);
+        // slot q of a chunk buffer: Ma members of nc planes each
+        const long Qp = (long)Ma * nc * d.plane, mp = (long)nc * d.plane;
+        const long Qs = (long)Ma * nc * d.splane, ms = (long)nc * d.splane;
+        // spectral slots 0-8 prep, 9-14 S_ij, 15-20 |S| S_ij
         cplx<T>* dst[21];
-        for (int q = 0; q < 9; ++q) src[q] = lag_phys + q * Qp;
-        for (int q = 0; q < 6; ++q) src[9 + q] = fld[sids[q]] + (long)k0 * d.plane;
-        for (int q = 0; q < 6; ++q) src[15 + q] = lag_phys + (9 + q) * Qp;
         for (int q = 0; q < 21; ++q) dst[q] = lag_spec + q * Qs;
-        fwd2d(21, src, nullptr, d.plane, 0, dst, d.splane, 0, nc);
+        const T* psrc[15];
+        cplx<T>* pdst[15];
+        for (int q = 0; q < 9; ++q) {
+            psrc[q] = lag_phys + q * Qp;
+            pdst[q] = dst[q];
+        }
+        for (int q = 0; q < 6; ++q) {
+            psrc[9 + q] = lag_phys + (9 + q) * Qp;
+            pdst[9 + q] = dst[15 + q];
+        }
+        fwd2d(15, psrc, nullptr, d.plane, 0, mp, pdst, d.splane, 0, ms, nc);
+        const T* ssrc[6];
+        for (int q = 0; q < 6; ++q) ssrc[q] = fld[sids[q]];
+        fwd2d(6, ssrc, nullptr, d.plane, k0, d.field, dst + 9, d.splane, 0, ms, nc);
         const cplx<T>* isrc[42];
         int ops[42];
         T* idst[42];
@@ -715,10 +574,11 @@ void Case<T>::lagrange_sdep() {
             ops[21 + q] = OP_G2;
             idst[21 + q] = lag_filt + (21 + q) * Qp;
         }
-        inv2d(42, isrc, ops, d.splane, 0, idst, d.plane, 0, nc, spec, d.splane);
-        hipLaunchKernelGGL(k_lag_update<T>, grid_ij(d.ld, d.ny, nc), dim3(BLK), 0, stream,
-                           lag_filt, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN], fld[LG_F_NN],
-                           fld[LG_CS_OPT2], fld[LG_BETA], fld[LG_TN_ALL], g, k0, nc, up);
+        inv2d(42, isrc, ops, d.splane, 0, ms, idst, d.plane, 0, mp, nc, spec);
+        hipLaunchKernelGGL(k_lag_update<T>, grid_ij(d.ld, d.ny, nc, Ma), dim3(BLK), 0, stream,
+                           (const T*)lag_filt, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN],
+                           fld[LG_F_NN], fld[LG_CS_OPT2], fld[LG_BETA], fld[LG_TN_ALL], g, k0, nc,
+                           Ma, up);
     }
 }
 
@@ -734,19 +594,18 @@ void Case<T>::divstress_uv() {
     const T* src[3] = {fld[LG_TXX], fld[LG_TXY], fld[LG_TYY]};
     const T scs[3] = {sc, sc, sc};
     cplx<T>* S[3] = {spec[0], spec[1], spec[2]};
-    fwd2d(3, src, scs, d.plane, 1, S, d.splane, 1, nl);
+    fwd2d(3, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, nl);
     const cplx<T>* is[2] = {spec[0], spec[1]};
     const cplx<T>* is2[2] = {spec[1], spec[2]};
     const int ops[2] = {OP_DIVXY, OP_DIVXY};
     T* dst[2] = {tmp[0], tmp[2]};
     cplx<T>* tsp[2] = {spec[3], spec[4]};
-    inv2d(2, is, ops, d.splane, 1, dst, d.plane, 1, nl, tsp, d.splane, is2);
-    const Geo g = geo_of(d);
+    inv2d(2, is, ops, d.splane, 1, msp, dst, d.plane, 1, d.field, nl, tsp, is2);
     const T rdz = (T)(1.0 / d.dz);
-    hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl), dim3(BLK), 0, stream, tmp[0],
-                       fld[LG_TXZ], fld[LG_DIVTX], g, rdz);
-    hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl), dim3(BLK), 0, stream, tmp[2],
-                       fld[LG_TYZ], fld[LG_DIVTY], g, rdz);
+    hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[0], (const T*)fld[LG_TXZ], fld[LG_DIVTX], GEO(1, nl), rdz);
+    hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[2], (const T*)fld[LG_TYZ], fld[LG_DIVTY], GEO(1, nl), rdz);
     for (int id : {LG_DIVTX, LG_DIVTY}) {
         fill_level(id, 0, BOGUS);
         fill_level(id, d.nz, BOGUS);
@@ -759,15 +618,16 @@ void Case<T>::divstress_w() {
     const T* src[2] = {fld[LG_TXZ], fld[LG_TYZ]};
     const T scs[2] = {sc, sc};
     cplx<T>* S[2] = {spec[0], spec[1]};
-    fwd2d(2, src, scs, d.plane, 1, S, d.splane, 1, d.nz);
+    fwd2d(2, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
     const cplx<T>* is[1] = {spec[0]};
     const cplx<T>* is2[1] = {spec[1]};
     const int ops[1] = {OP_DIVXY};
     T* dst[1] = {tmp[4]};
     cplx<T>* tsp[1] = {spec[2]};
-    inv2d(1, is, ops, d.splane, 1, dst, d.plane, 1, d.nz, tsp, d.splane, is2);
-    hipLaunchKernelGGL(k_divt_w<T>, grid_ij(d.ld, d.ny, d.nz), dim3(BLK), 0, stream, tmp[4],
-                       fld[LG_TZZ], fld[LG_DIVTZ], geo_of(d), (T)(1.0 / d.dz));
+    inv2d(1, is, ops, d.splane, 1, msp, dst, d.plane, 1, d.field, d.nz, tsp, is2);
+    hipLaunchKernelGGL(k_divt_w<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[4], (const T*)fld[LG_TZZ], fld[LG_DIVTZ], GEO(1, d.nz),
+                       (T)(1.0 / d.dz));
     fill_level(LG_DIVTZ, 0, BOGUS);
 }
 
@@ -782,16 +642,19 @@ void Case<T>::convec(bool reuse_spectra) {
     if (reuse_spectra) {
         // u, v, w spectra are the ones filt_da just produced (the filtered
         // fields' transforms); the vorticity is formed from them in spectral
-        // space, with only the wall-level dudz/dvdz planes transformed.
+        // space, with only the wall-level dvdz/dudz planes transformed (into
+        // planes 0 and 1 of spec[6]).
         const T* wsrc[2] = {lev(LG_DVDZ, 1), lev(LG_DUDZ, 1)};
         const T wsc[2] = {c, c};
         cplx<T>* wd[2] = {spec[6], spec[6] + d.splane};
-        fwd2d(2, wsrc, wsc, d.plane, 0, wd, d.splane, 0, 1);
+        fwd2d(2, wsrc, wsc, d.plane, 0, d.field, wd, d.splane, 0, msp, 1);
         const int n = d.ny * d.NK;
-        hipLaunchKernelGGL(k_vort_spec<T>, dim3((n + BLK - 1) / BLK, nl), dim3(BLK), 0, stream,
-                           sU, sV, sW, spec[6], spec[6] + d.splane, spec[3], spec[4], spec[5],
-                           d.ny, d.NK, d.nz, d.splane, (T)(2.0 * PI_D / d.prm.lx),
-                           (T)(2.0 * PI_D / d.prm.ly), (T)(1.0 / d.dz));
+        hipLaunchKernelGGL(k_vort_spec<T>, dim3((n + BLK - 1) / BLK, nl * Ma), dim3(BLK), 0,
+                           stream, (const cplx<T>*)sU, (const cplx<T>*)sV, (const cplx<T>*)sW,
+                           (const cplx<T>*)spec[6], (const cplx<T>*)(spec[6] + d.splane), spec[3],
+                           spec[4], spec[5], d.ny, d.NK, d.nz, d.splane,
+                           (T)(2.0 * PI_D / d.prm.lx), (T)(2.0 * PI_D / d.prm.ly),
+                           (T)(1.0 / d.dz), msp);
         ps[0] = sU;
         ps[1] = sV;
         ps[2] = sW;
@@ -799,19 +662,19 @@ void Case<T>::convec(bool reuse_spectra) {
         const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
         const T scs[3] = {c, c, c};
         cplx<T>* S[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
-        fwd2d(3, src, scs, d.plane, 1, S, d.splane, 1, nl);
+        fwd2d(3, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, nl);
         VortArgs<T> va{fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDZ], fld[LG_DWDX],
                        fld[LG_DWDY], tmp[0], tmp[1], tmp[2], c};
-        hipLaunchKernelGGL(k_vort<T>, grid_ij(d.nx, d.ny, nl), dim3(BLK), 0, stream, va,
-                           geo_of(d));
+        hipLaunchKernelGGL(k_vort<T>, grid_ij(d.nx, d.ny, nl, Ma), dim3(BLK), 0, stream, va,
+                           GEO(1, nl));
         const T* vs[3] = {tmp[0], tmp[1], tmp[2]};
-        fwd2d(3, vs, nullptr, d.plane, 1, S + 3, d.splane, 1, nl);
+        fwd2d(3, vs, nullptr, d.plane, 1, d.field, S + 3, d.splane, 1, msp, nl);
     }
     pad_inv_big(6, ps, 1, nl);
     ProdArgs<T> pa{big[0], big[1], big[2], big[3], big[4], big[5], big[6], big[7], big[8],
                    (T)(1.0 / (d.nx2 * d.ny2))};
-    hipLaunchKernelGGL(k_products<T>, grid_ij(d.nx2, d.ny2, nl), dim3(BLK), 0, stream, pa,
-                       d.nx2, d.ny2, d.ld_big, d.nz);
+    hipLaunchKernelGGL(k_products<T>, grid_ij(d.nx2, d.ny2, nl, Ma), dim3(BLK), 0, stream, pa,
+                       d.nx2, d.ny2, d.ld_big, d.nz, mbp);
     const T* cs[3] = {big[6], big[7], big[8]};
     cplx<T>* cd[3] = {spec[0], spec[1], spec[2]};
     fwd_big_unpad(3, cs, cd, 1, nl);
@@ -819,7 +682,7 @@ void Case<T>::convec(bool reuse_spectra) {
     const int ops[3] = {OP_NONE, OP_NONE, OP_NONE};
     T* dst[3] = {fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ]};
     cplx<T>* tsp[3] = {spec[3], spec[4], spec[5]};
-    inv2d(3, is, ops, d.splane, 1, dst, d.plane, 1, nl, tsp, d.splane);
+    inv2d(3, is, ops, d.splane, 1, msp, dst, d.plane, 1, d.field, nl, tsp);
     for (int id : {LG_RHSX, LG_RHSY, LG_RHSZ}) fill_level(id, 0, BOGUS);
     fill_level(LG_RHSX, d.nz, BOGUS);
     fill_level(LG_RHSY, d.nz, BOGUS);
@@ -836,44 +699,49 @@ void Case<T>::press_stag_array() {
         const T* src[2] = {fld[LG_U], fld[LG_V]};
         const T scs[2] = {c2, c2};
         cplx<T>* S[2] = {spec[0], spec[1]};
-        fwd2d(2, src, scs, d.plane, 1, S, d.splane, 1, d.nz - 1);
+        fwd2d(2, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz - 1);
     }
     {
         const T* src[1] = {fld[LG_W]};
         cplx<T>* S[1] = {spec[2]};
-        fwd2d(1, src, &c2, d.plane, 1, S, d.splane, 1, d.nz);
+        fwd2d(1, src, &c2, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
     }
     {
+        // divtz at k = 1 and k = nz into planes 0 and 1 of spec[3]
         const T* src[2] = {lev(LG_DIVTZ, 1), lev(LG_DIVTZ, d.nz)};
         const T scs[2] = {(T)cst, (T)cst};
         cplx<T>* S[2] = {spec[3], spec[3] + d.splane};
-        fwd2d(2, src, scs, d.plane, 0, S, d.splane, 0, 1);
+        fwd2d(2, src, scs, d.plane, 0, d.field, S, d.splane, 0, msp, 1);
     }
-    TriGeo tg{d.ny, d.NK, d.nz, d.splane, 2.0 * PI_D / d.prm.lx, 2.0 * PI_D / d.prm.ly, d.dz,
-              1.0 / (d.dz * d.dz), 1.0 / d.dz};
-    const int nsys = d.ny * d.NK;
-    const double tbytes = (double)d.splane * sizeof(cplx<T>) * (3.0 * d.nz + 1 + 2 + d.nz + 1);
+    TriGeo tg{d.ny, d.NK, d.nz, d.splane, msp, Ma, 2.0 * PI_D / d.prm.lx,
+              2.0 * PI_D / d.prm.ly, d.dz, 1.0 / (d.dz * d.dz), 1.0 / d.dz};
+    const int nsys = Ma * d.ny * d.NK;
+    const double tbytes =
+        (double)Ma * d.splane * sizeof(cplx<T>) * (3.0 * d.nz + 1 + 2 + d.nz + 1);
+    const PassChoice ch = choice_for<T>(PK_TRIDAG, d.ny, d.NK);
+    const int tb = ch.S;
     timer.run(stream, KK_TRIDAG, tbytes, [&] {
-        hipLaunchKernelGGL(k_tridag<T>, dim3((nsys + 127) / 128), dim3(128), 0, stream, spec[0],
-                           spec[1], spec[2], spec[3], spec[3] + d.splane, spec[4], gam, usc, tg,
-                           dev_err);
+        hipLaunchKernelGGL(k_tridag<T>, dim3((nsys + tb - 1) / tb), dim3(tb), 0, stream,
+                           (const cplx<T>*)spec[0], (const cplx<T>*)spec[1],
+                           (const cplx<T>*)spec[2], (const cplx<T>*)spec[3],
+                           (const cplx<T>*)(spec[3] + d.splane), spec[4], gam, usc, tg, dev_err);
     });
     {
         const cplx<T>* is[1] = {spec[4]};
         const int ops[1] = {OP_NONE};
         T* dst[1] = {fld[LG_P]};
         cplx<T>* tsp[1] = {spec[5]};
-        inv2d(1, is, ops, d.splane, 0, dst, d.plane, 0, d.nz + 1, tsp, d.splane);
+        inv2d(1, is, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, d.nz + 1, tsp);
     }
     {
         const cplx<T>* is[2] = {spec[4], spec[4]};
         const int ops[2] = {OP_IKX, OP_IKY};
         T* dst[2] = {fld[LG_DPDX], fld[LG_DPDY]};
         cplx<T>* tsp[2] = {spec[5], spec[6]};
-        inv2d(2, is, ops, d.splane, 1, dst, d.plane, 1, d.nz - 1, tsp, d.splane);
+        inv2d(2, is, ops, d.splane, 1, msp, dst, d.plane, 1, d.field, d.nz - 1, tsp);
     }
-    hipLaunchKernelGGL(k_dpdz<T>, grid_ij(d.nx, d.ny, d.nz), dim3(BLK), 0, stream, fld[LG_P],
-                       fld[LG_DPDZ], geo_of(d), (T)d.dz);
+    hipLaunchKernelGGL(k_dpdz<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+                       (const T*)fld[LG_P], fld[LG_DPDZ], GEO(1, d.nz), (T)d.dz);
     fill_level(LG_DPDX, d.nz, BOGUS);
     fill_level(LG_DPDY, d.nz, BOGUS);
 }
@@ -883,14 +751,15 @@ void Case<T>::press_stag_array() {
 // ---------------------------------------------------------------------------
 template <typename T>
 void Case<T>::project() {
-    hipLaunchKernelGGL(k_project<T>, grid_ij(d.nx, d.ny, d.nz - 1), dim3(BLK), 0, stream,
-                       fld[LG_U], fld[LG_V], fld[LG_W], fld[LG_DPDX], fld[LG_DPDY],
-                       fld[LG_DPDZ], geo_of(d), (T)d.prm.dt, (T)d.tadv1);
-    const unsigned nb = (unsigned)((d.plane + BLK - 1) / BLK);
-    hipLaunchKernelGGL(k_copy_plane<T>, dim3(nb), dim3(BLK), 0, stream, lev(LG_U, d.nz - 1),
-                       lev(LG_U, d.nz), d.plane);
-    hipLaunchKernelGGL(k_copy_plane<T>, dim3(nb), dim3(BLK), 0, stream, lev(LG_V, d.nz - 1),
-                       lev(LG_V, d.nz), d.plane);
+    hipLaunchKernelGGL(k_project<T>, grid_ij(d.nx, d.ny, d.nz - 1, Ma), dim3(BLK), 0, stream,
+                       fld[LG_U], fld[LG_V], fld[LG_W], (const T*)fld[LG_DPDX],
+                       (const T*)fld[LG_DPDY], (const T*)fld[LG_DPDZ], GEO(1, d.nz - 1),
+                       (T)d.prm.dt, (T)d.tadv1);
+    const dim3 gp((unsigned)((d.plane + BLK - 1) / BLK), Ma);
+    hipLaunchKernelGGL(k_copy_plane<T>, gp, dim3(BLK), 0, stream, (const T*)lev(LG_U, d.nz - 1),
+                       lev(LG_U, d.nz), d.field, d.plane);
+    hipLaunchKernelGGL(k_copy_plane<T>, gp, dim3(BLK), 0, stream, (const T*)lev(LG_V, d.nz - 1),
+                       lev(LG_V, d.nz), d.field, d.plane);
     fill_level(LG_W, d.nz, 0.0);
     fill_level(LG_W, 1, 0.0);
 }
@@ -904,22 +773,25 @@ void Case<T>::rhs_ab2() {
     Ab2Args<T> a{fld[LG_U], fld[LG_V], fld[LG_W], fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ],
                  fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F], fld[LG_DIVTX], fld[LG_DIVTY],
                  fld[LG_DIVTZ], tb ? fld[LG_FXA] : nullptr, tb ? fld[LG_FYA] : nullptr,
-                 tb ? fld[LG_FZA] : nullptr, (T)d.prm.dt, (T)d.tadv1, (T)d.tadv2,
-                 (T)d.prm.mean_p_force_x, (T)d.prm.mean_p_force_y, d.prm.use_mean_p_force};
-    hipLaunchKernelGGL(k_rhs_ab2<T>, grid_ij(d.ld, d.ny, d.nz), dim3(BLK), 0, stream, a,
-                       geo_of(d));
+                 tb ? fld[LG_FZA] : nullptr, (T)d.prm.dt, (T)d.tadv1, (T)d.tadv2, d_mpf,
+                 d.prm.use_mean_p_force};
+    hipLaunchKernelGGL(k_rhs_ab2<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
+                       GEO(1, d.nz));
     for (int id : {LG_U, LG_V, LG_W}) fill_level(id, 0, BOGUS);
     fill_level(LG_U, d.nz, BOGUS);
     fill_level(LG_V, d.nz, BOGUS);
 }
 
 template <typename T>
-void Case<T>::step() {
-    ++jt;
-    ++jt_total;
-    std::swap(fld[LG_RHSX], fld[LG_RHSX_F]);
-    std::swap(fld[LG_RHSY], fld[LG_RHSY_F]);
-    std::swap(fld[LG_RHSZ], fld[LG_RHSZ_F]);
+void Case<T>::rhs_gradp() {
+    hipLaunchKernelGGL(k_rhs_gradp<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+                       fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ], (const T*)fld[LG_DPDX],
+                       (const T*)fld[LG_DPDY], (const T*)fld[LG_DPDZ], GEO(1, d.nz));
+}
+
+// The loop body after the counters advance and RHS_f = RHS (main.f90:155-326).
+template <typename T>
+void Case<T>::step_body() {
     filt_da(LG_U, LG_DUDX, LG_DUDY, sU);
     filt_da(LG_V, LG_DVDX, LG_DVDY, sV);
     filt_da(LG_W, LG_DWDX, LG_DWDY, sW);
@@ -938,21 +810,78 @@ void Case<T>::step() {
     project();
 }
 
+// RHS_f = RHS by exchanging the storage: RHS is fully rewritten each step.
+#define LG_SWAP_RHS()                                 \
+    do {                                              \
+        std::swap(fld[LG_RHSX], fld[LG_RHSX_F]);      \
+        std::swap(fld[LG_RHSY], fld[LG_RHSY_F]);      \
+        std::swap(fld[LG_RHSZ], fld[LG_RHSZ_F]);      \
+        swapped = !swapped;                           \
+    } while (0)
+
 template <typename T>
-void Case<T>::rhs_gradp() {
-    hipLaunchKernelGGL(k_rhs_gradp<T>, grid_ij(d.ld, d.ny, d.nz), dim3(BLK), 0, stream,
-                       fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ], fld[LG_DPDX], fld[LG_DPDY],
-                       fld[LG_DPDZ], geo_of(d));
+void Case<T>::step() {
+    ++jt;
+    ++jt_total;
+    LG_SWAP_RHS();
+    step_body();
+}
+
+template <typename T>
+void Case<T>::drop_graphs() {
+    for (int q = 0; q < 4; ++q) {
+        if (graphs[q]) (void)hipGraphExecDestroy(graphs[q]);
+        graphs[q] = nullptr;
+        graph_ma[q] = 0;
+    }
+}
+
+// One step replayed from a captured graph.  A step's launches depend on the
+// RHS storage parity and on whether the dynamic model runs, so up to four
+// graphs are captured, each on first use (and again after the member count
+// or the disk set changes).
+template <typename T>
+void Case<T>::step_graph() {
+    ++jt;
+    ++jt_total;
+    LG_SWAP_RHS();
+    const int key = (swapped ? 1 : 0) + ((jt_total % d.prm.cs_count == 0) ? 2 : 0);
+    if (!graphs[key] || graph_ma[key] != Ma) {
+        if (graphs[key]) (void)hipGraphExecDestroy(graphs[key]);
+        graphs[key] = nullptr;
+        const bool ton = timer.on;
+        timer.on = false;
+        LG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            step_body();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            timer.on = ton;
+            throw;
+        }
+        timer.on = ton;
+        LG_HIP(hipStreamEndCapture(stream, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        LG_HIP(e);
+        graphs[key] = ex;
+        graph_ma[key] = Ma;
+    }
+    LG_HIP(hipGraphLaunch(graphs[key], stream));
 }
 
 // ---------------------------------------------------------------------------
-// Diagnostics
+// Diagnostics (member m)
 // ---------------------------------------------------------------------------
 template <typename T>
-double Case<T>::rmsdiv() {
+double Case<T>::rmsdiv(int m) {
     const int nb = 256;
-    hipLaunchKernelGGL(k_rmsdiv_part<T>, dim3(nb), dim3(BLK), 0, stream, fld[LG_DUDX],
-                       fld[LG_DVDY], fld[LG_DWDZ], geo_of(d), red);
+    hipLaunchKernelGGL(k_rmsdiv_part<T>, dim3(nb), dim3(BLK), 0, stream,
+                       (const T*)mem(LG_DUDX, m), (const T*)mem(LG_DVDY, m),
+                       (const T*)mem(LG_DWDZ, m), GEO(1, d.nz), red);
     std::vector<double> h(nb);
     LG_HIP(hipMemcpyAsync(h.data(), red, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
     LG_HIP(hipStreamSynchronize(stream));
@@ -962,10 +891,10 @@ double Case<T>::rmsdiv() {
 }
 
 template <typename T>
-double Case<T>::energy() {
+double Case<T>::energy(int m) {
     const int nb = 256;
-    hipLaunchKernelGGL(k_energy_part<T>, dim3(nb), dim3(BLK), 0, stream, fld[LG_U], fld[LG_V],
-                       fld[LG_W], geo_of(d), red);
+    hipLaunchKernelGGL(k_energy_part<T>, dim3(nb), dim3(BLK), 0, stream, (const T*)mem(LG_U, m),
+                       (const T*)mem(LG_V, m), (const T*)mem(LG_W, m), GEO(1, d.nz), red);
     std::vector<double> h(nb);
     LG_HIP(hipMemcpyAsync(h.data(), red, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
     LG_HIP(hipStreamSynchronize(stream));
@@ -975,32 +904,32 @@ double Case<T>::energy() {
 }
 
 template <typename T>
-double Case<T>::max_cfl() {
+double Case<T>::max_cfl(int m) {
     const int nb = 256;
-    double m[3];
+    double mx3[3];
     const double h3[3] = {d.dx, d.dy, d.dz};
     const int ids[3] = {LG_U, LG_V, LG_W};
     for (int q = 0; q < 3; ++q) {
-        hipLaunchKernelGGL(k_maxabs_part<T>, dim3(nb), dim3(BLK), 0, stream, fld[ids[q]],
-                           geo_of(d), red);
+        hipLaunchKernelGGL(k_maxabs_part<T>, dim3(nb), dim3(BLK), 0, stream,
+                           (const T*)mem(ids[q], m), GEO(1, d.nz), red);
         std::vector<double> h(nb);
         LG_HIP(hipMemcpyAsync(h.data(), red, nb * sizeof(double), hipMemcpyDeviceToHost,
                               stream));
         LG_HIP(hipStreamSynchronize(stream));
         double mx = 0.0;
         for (double x : h) mx = std::max(mx, x);
-        m[q] = mx / h3[q];
+        mx3[q] = mx / h3[q];
     }
-    return d.prm.dt * std::max(m[0], std::max(m[1], m[2]));
+    return d.prm.dt * std::max(mx3[0], std::max(mx3[1], mx3[2]));
 }
 
 template <typename T>
-double Case<T>::tau_wall_bot() {
+double Case<T>::tau_wall_bot(int m) {
     std::vector<T> hx(d.plane), hy(d.plane);
-    LG_HIP(hipMemcpyAsync(hx.data(), lev(LG_TXZ, 1), d.plane * sizeof(T), hipMemcpyDeviceToHost,
-                          stream));
-    LG_HIP(hipMemcpyAsync(hy.data(), lev(LG_TYZ, 1), d.plane * sizeof(T), hipMemcpyDeviceToHost,
-                          stream));
+    LG_HIP(hipMemcpyAsync(hx.data(), mem(LG_TXZ, m) + d.plane, d.plane * sizeof(T),
+                          hipMemcpyDeviceToHost, stream));
+    LG_HIP(hipMemcpyAsync(hy.data(), mem(LG_TYZ, m) + d.plane, d.plane * sizeof(T),
+                          hipMemcpyDeviceToHost, stream));
     LG_HIP(hipStreamSynchronize(stream));
     double sx = 0.0, sy = 0.0;
     for (int i = 0; i < d.nx; ++i)

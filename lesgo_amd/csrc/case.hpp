@@ -7,6 +7,7 @@
 #include "../../include/lesgo_amd.h"
 #include "common.hpp"
 #include "fft.hpp"
+#include "passes.hpp"
 #include "timer.hpp"
 
 namespace lg {
@@ -38,15 +39,24 @@ struct PlanTables {
     void build(int N);
 };
 
-// Actuator-disk node lists on the device (turbines.f90:275-462 output).
-struct TurbineSet {
-    int n = 0;
-    int total_nodes = 0;
-    bool adm_correction = true;
+// Actuator disks of one member (turbines.f90:275-462 output, host side).
+struct MemberTurbines {
     std::vector<double> spec;        // 8 per turbine
     std::vector<double> M;           // indicator M per turbine
-    std::vector<int> h_off;
-    int* node_lin = nullptr;         // linear offset (k*ny + j)*ld + i
+    std::vector<long> lin;           // node offset (k*ny + j)*ld + i within the member
+    std::vector<double> ind;         // normalised indicator weight
+    std::vector<int> off;            // n+1 offsets into lin / ind
+    std::vector<double> coef;        // per turbine: n1,n2,n3,ct',dia,corrfac
+    std::vector<double> state;       // per turbine: u_d, u_d_T, f_n (last download)
+    int n() const { return (int)spec.size() / 8; }
+};
+
+// Device node lists of every member's disks, concatenated member by member.
+struct TurbineSet {
+    int n = 0;                       // turbines over all members
+    int total_nodes = 0;
+    std::vector<int> first;          // first turbine of each member (M+1)
+    long* node_lin = nullptr;        // m*field + (k*ny + j)*ld + i
     double* node_ind = nullptr;      // normalised indicator weight
     int* off = nullptr;              // n+1 offsets
     int* node_tid = nullptr;         // turbine of each node
@@ -56,11 +66,25 @@ struct TurbineSet {
     ~TurbineSet();
 };
 
+// Batch of M same-grid members.  Member m of every array starts at
+// m * (its member stride); Ma <= M members are stepped.
 template <typename T>
 struct Case {
     Dims d;
+    int M = 1;                       // member capacity
+    int Ma = 1;                      // members stepped
+    long msp = 0;                    // (nz+1) * splane
+    long mbp = 0;                    // (nz+1) * bplane
+    long mbsp = 0;                   // (nz+1) * bsplane
     hipStream_t stream = nullptr;
     int jt = 0, jt_total = 0;
+    bool swapped = false;            // RHS / RHS_f storage exchanged (odd steps)
+    std::vector<double> zo, mpf;     // per member z0 and mean_p_force (x, y)
+    double* d_denom = nullptr;       // per member ln(0.5 dz / z0)
+    double* d_mpf = nullptr;         // per member mean_p_force_x, _y
+    std::vector<MemberTurbines> mturb;
+    hipGraphExec_t graphs[4] = {};   // captured steps keyed by (swapped, dynamic model)
+    int graph_ma[4] = {};
 
     T* fld[LG_NFIELDS] = {};
     // spectral scratch, each (nz+1) small spectral planes
@@ -93,20 +117,22 @@ struct Case {
     std::unique_ptr<TurbineSet> turb;
     KernelTimer timer;
 
-    explicit Case(const lg_params& p);
+    Case(const lg_params& p, int members);
     ~Case();
     Case(const Case&) = delete;
     Case& operator=(const Case&) = delete;
 
     T* F(int id) { return fld[id]; }
     T* lev(int id, int k) { return fld[id] + (long)k * d.plane; }
+    T* mem(int id, int m) { return fld[id] + (long)m * d.field; }
 
-    // 2-D transform helpers over planes [lev0, lev0+nlev)
+    // 2-D transform helpers over planes [lev0, lev0+nlev) of every member
     void fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
-               cplx<T>* const* dst, long dst_plane, int dst_lev0, int nlev);
+               long src_ms, cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms,
+               int nlev);
     void inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane, int src_lev0,
-               T* const* dst, long dst_plane, int dst_lev0, int nlev, cplx<T>* const* tmpspec,
-               long tmp_plane, const cplx<T>* const* src2 = nullptr);
+               long src_ms, T* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nlev,
+               cplx<T>* const* tmpspec, const cplx<T>* const* src2 = nullptr);
     void pad_inv_big(int njobs, const cplx<T>* const* src, int lev0, int nlev);
     void fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst, int lev0, int nlev);
 
@@ -130,18 +156,31 @@ struct Case {
     void project();
     void rhs_ab2();
     void rhs_gradp();
+    void step_body();
     void step();
+    void step_graph();
+    void drop_graphs();
     void turbines_forcing();
-    double rmsdiv();
-    double energy();
-    double max_cfl();
-    double tau_wall_bot();
+    void pull_turbine_state();
+    void rebuild_turbines();
+    double rmsdiv(int m = 0);
+    double energy(int m = 0);
+    double max_cfl(int m = 0);
+    double tau_wall_bot(int m = 0);
     void fill_level(int id, int k, double v);
+    void copy_member(int src, int dst);
+    void set_member_params(int m, double zo_m, double mpfx, double mpfy);
     void check_device_error();
 };
 
+// Time every launch choice of c's passes and record the fastest in the
+// tuning table; returns the number of passes tuned.
 template <typename T>
-void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, double alpha2,
-                   double filter_cutoff, int adm_correction);
+int autotune(Case<T>& c, int reps);
+
+// Replace member m's actuator disks (nturb = 0 removes them).
+template <typename T>
+void turbines_init(Case<T>& c, int m, int nturb, const double* spec8, double alpha1,
+                   double alpha2, double filter_cutoff, int adm_correction);
 
 }  // namespace lg

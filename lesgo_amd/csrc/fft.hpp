@@ -143,6 +143,9 @@ struct RowR2CArgs {
     int lev0_dst;          // first destination plane index
     int S;                 // row pairs per workgroup
     int stride;            // LDS stride per sequence (complex elements)
+    int nlev;              // planes per member (grid.y = nlev * members)
+    long src_mstride;      // elements between members of src
+    long dst_mstride;      // complex elements between members of dst
 };
 
 template <typename T>
@@ -151,12 +154,13 @@ __global__ void __launch_bounds__(256) k_rows_r2c(RowR2CArgs<T> a) {
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2c);
     const int N = a.plan.N;
     const int job = blockIdx.z;
-    const int plane = blockIdx.y;
+    const int mem = blockIdx.y / a.nlev;
+    const int plane = blockIdx.y - mem * a.nlev;
     const int npairs = a.nrows / 2;
     const int pair0 = blockIdx.x * a.S;
     const int S = min(a.S, npairs - pair0);
-    const T* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
-    cplx<T>* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const T* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const T sc = a.scale[job];
     const int* perm = a.plan.perm;
     for (int idx = threadIdx.x; idx < S * N; idx += blockDim.x) {
@@ -205,6 +209,9 @@ struct RowC2RArgs {
     int lev0_dst;
     int S;
     int stride;
+    int nlev;
+    long src_mstride;      // complex elements between members of src
+    long dst_mstride;      // elements between members of dst
 };
 
 template <typename T>
@@ -213,12 +220,13 @@ __global__ void __launch_bounds__(256) k_rows_c2r(RowC2RArgs<T> a) {
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2r);
     const int N = a.plan.N;
     const int job = blockIdx.z;
-    const int plane = blockIdx.y;
+    const int mem = blockIdx.y / a.nlev;
+    const int plane = blockIdx.y - mem * a.nlev;
     const int npairs = a.nrows / 2;
     const int pair0 = blockIdx.x * a.S;
     const int S = min(a.S, npairs - pair0);
-    const cplx<T>* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
-    T* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    T* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int* perm = a.plan.perm;
     const int nc = a.ncols;
     const int half = N / 2;
@@ -306,6 +314,9 @@ struct ColArgs {
     int zero_nyq;      // forward: zero the ky = ny/2 row of the (small) output
     int TC;            // columns per tile
     int stride;        // LDS stride per column sequence
+    int nlev;          // planes per member (grid.z = nlev * members)
+    long src_mstride;  // complex elements between members (src, src2)
+    long dst_mstride;
     T kxs;             // 2 pi / Lx
     T kys;             // 2 pi / Ly
     const T* G1;       // [ny][ncols] filter tables
@@ -349,9 +360,10 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
     const int job = blockIdx.x;
     const int c0 = blockIdx.y * a.TC;
     const int tc = min(a.TC, a.ncols - c0);
-    const int plane = blockIdx.z;
-    const cplx<T>* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
-    cplx<T>* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const int mem = blockIdx.z / a.nlev;
+    const int plane = blockIdx.z - mem * a.nlev;
+    const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int op = a.op[job];
     const int* perm = a.plan.perm;
     const int ny = a.ny;
@@ -370,7 +382,7 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
             v = src[(long)jb * a.ncols + c0 + c];
             if (DIR > 0 && a.map == MAP_ID) {
                 if (op == OP_DIVXY) {
-                    const cplx<T>* s2 = a.src2[job] + (long)(a.lev0_src + plane) * a.src_plane;
+                    const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
                     v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);
                 } else {
                     v = apply_op<T>(op, v, c0 + c, jb, a);

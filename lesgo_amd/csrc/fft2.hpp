@@ -122,12 +122,13 @@ __global__ void __launch_bounds__(256) k_rows_r2c_v2(RowR2CArgs<T> a) {
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2);
     const auto* tw = plan_tw<T>(a.plan);
     const int job = blockIdx.z;
-    const int plane = blockIdx.y;
+    const int mem = blockIdx.y / a.nlev;
+    const int plane = blockIdx.y - mem * a.nlev;
     const int npairs = a.nrows / 2;
     const int pair0 = blockIdx.x * a.S;
     const int S = min(a.S, npairs - pair0);
-    const T* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
-    cplx<T>* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const T* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const T sc = a.scale[job];
     for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
         const int s = t / N2, n2 = t - s * N2;
@@ -191,12 +192,13 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2);
     const auto* tw = plan_tw<T>(a.plan);
     const int job = blockIdx.z;
-    const int plane = blockIdx.y;
+    const int mem = blockIdx.y / a.nlev;
+    const int plane = blockIdx.y - mem * a.nlev;
     const int npairs = a.nrows / 2;
     const int pair0 = blockIdx.x * a.S;
     const int S = min(a.S, npairs - pair0);
-    const cplx<T>* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
-    T* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    T* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int nc = a.ncols;
     for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
         const int s = t / N2, n2 = t - s * N2;
@@ -272,9 +274,10 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
     const int TC = a.TC;
     const int c0 = blockIdx.y * TC;
     const int tc = min(TC, a.ncols - c0);
-    const int plane = blockIdx.z;
-    const cplx<T>* src = a.src[job] + (long)(a.lev0_src + plane) * a.src_plane;
-    cplx<T>* dst = a.dst[job] + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const int mem = blockIdx.z / a.nlev;
+    const int plane = blockIdx.z - mem * a.nlev;
+    const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int op = a.op[job];
     const int ny = a.ny;
     const int nyh = ny / 2;
@@ -296,7 +299,7 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 v = src[(long)jb * a.ncols + c0 + c];
                 if (DIR > 0 && a.map == MAP_ID) {
                     if (op == OP_DIVXY) {
-                        const cplx<T>* s2 = a.src2[job] + (long)(a.lev0_src + plane) * a.src_plane;
+                        const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
                         v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);
                     } else {
                         v = apply_op<T>(op, v, c0 + c, jb, a);

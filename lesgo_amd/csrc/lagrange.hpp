@@ -11,6 +11,7 @@ struct InterpGeo {
     int nx, ny, nz, ld;
     long plane;
     double dx, dy, dz, lx, ly, lz, ldt;
+    long mstride;          // elements between members (all arrays)
 };
 
 // cell_indx_w for 'i'/'j' (functions.f90:188-237): px is wrapped with
@@ -82,7 +83,20 @@ __global__ void k_interpolag(const T* u, const T* v, const T* w, const T* tLM, c
                              InterpGeo g) {
     int j, i;
     if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
+    const int m = blockIdx.y / g.nz;
+    const int k = blockIdx.y - m * g.nz + 1;
+    const long mb = (long)m * g.mstride;
+    u += mb;
+    v += mb;
+    w += mb;
+    tLM += mb;
+    tMM += mb;
+    tQN += mb;
+    tNN += mb;
+    LM += mb;
+    MM += mb;
+    QN += mb;
+    NN += mb;
     const long P = g.plane;
     const long o = (long)k * P + (long)j * g.ld + i;
     const double x = (double)i * g.dx;
@@ -113,13 +127,14 @@ __global__ void k_interpolag(const T* u, const T* v, const T* w, const T* tLM, c
 template <typename T>
 __global__ void k_lag_prep(const T* u, const T* v, const T* w, const T* S11, const T* S12,
                            const T* S13, const T* S22, const T* S23, const T* S33, T* out,
-                           Geo g, int k0, int nc) {
+                           Geo g, int k0, int nc, int M) {
     int j, i;
     if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int c = blockIdx.y;
+    const int m = blockIdx.y / nc;
+    const int c = blockIdx.y - m * nc;
     const int k = k0 + c;
     const long P = g.plane;
-    const long o = (long)k * P + (long)j * g.ld + i;
+    const long o = (long)m * g.mstride + (long)k * P + (long)j * g.ld + i;
     T ub, vb, wb;
     if (k == 1) {
         ub = u[o];
@@ -134,8 +149,8 @@ __global__ void k_lag_prep(const T* u, const T* v, const T* w, const T* S11, con
         vb = (T)0.5 * (v[o] + v[o - P]);
         wb = w[o];
     }
-    const long Q = (long)nc * P;
-    const long oc = (long)c * P + (long)j * g.ld + i;
+    const long Q = (long)M * nc * P;
+    const long oc = ((long)m * nc + c) * P + (long)j * g.ld + i;
     out[oc + 0 * Q] = ub;
     out[oc + 1 * Q] = vb;
     out[oc + 2 * Q] = wb;
@@ -163,20 +178,21 @@ struct LagUpd {
 
 template <typename T>
 __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, T* Cs,
-                             T* Beta, T* Tn_all, Geo g, int k0, int nc, LagUpd p) {
+                             T* Beta, T* Tn_all, Geo g, int k0, int nc, int M, LagUpd p) {
     int j, i;
     if (!ij_of(g.ld, g.ny, j, i)) return;
-    const int c = blockIdx.y;
+    const int m = blockIdx.y / nc;
+    const int c = blockIdx.y - m * nc;
     const int k = k0 + c;
     const long P = g.plane;
-    const long o = (long)k * P + (long)j * g.ld + i;
+    const long o = (long)m * g.mstride + (long)k * P + (long)j * g.ld + i;
     if (i >= g.nx) {
         // Cs_opt2(ld-1:ld) = zero (lagrange_Sdep.f90:397-399)
         Cs[o] = (T)p.zero;
         return;
     }
-    const long Q = (long)nc * P;
-    const long oc = (long)c * P + (long)j * g.ld + i;
+    const long Q = (long)M * nc * P;
+    const long oc = ((long)m * nc + c) * P + (long)j * g.ld + i;
     auto B = [&](int q) -> double { return (double)filt[oc + (long)q * Q]; };
     auto H = [&](int q) -> double { return (double)filt[oc + (long)(21 + q) * Q]; };
     // L_ij and Q_ij (lagrange_Sdep.f90:137-167)

@@ -1,0 +1,81 @@
+// Transform-pass launchers with tunable launch choices.  A pass moves
+// njobs arrays of nlev planes for each of M members: member m's planes start
+// at m * mstride.
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "fft.hpp"
+#include "timer.hpp"
+
+namespace lg {
+
+constexpr size_t LDS_MAX = 64 * 1024;
+
+// Launch choice of one transform pass: the four-step split (n1 = 0: the
+// generic LDS kernel) and the rows (pairs) or columns per workgroup.
+struct PassChoice {
+    int n1 = 0, n2 = 0, S = 0;
+};
+
+// Pass kinds of the tuning table (column passes by direction and map).
+enum PassKind : int {
+    PK_R2C = 0,
+    PK_C2R = 1,
+    PK_COLS_FWD = 2,       // forward, in place (MAP_ID)
+    PK_COLS_INV = 3,       // inverse (MAP_ID)
+    PK_TRIDAG = 4,
+    PK_COLS_UNPAD = 5,     // forward, big grid -> small (MAP_UNPAD)
+    PK_COLS_PAD = 6,       // inverse, small -> big grid (MAP_PAD)
+    PK_NKINDS = 7
+};
+
+inline int cols_kind(int dir, int map) {
+    if (map == MAP_PAD) return PK_COLS_PAD;
+    if (map == MAP_UNPAD) return PK_COLS_UNPAD;
+    return dir < 0 ? PK_COLS_FWD : PK_COLS_INV;
+}
+
+// LDS complex elements per sequence (row or column) of a pass choice.
+inline int seq_stride(const PassChoice& c, int N) { return c.n1 ? c.n1 * (c.n2 + 1) + 1 : N + 1; }
+
+// Process-wide table of tuned launch choices keyed by (kind, element size,
+// transform length, rows or columns).  Choices depend on the grid only,
+// never on the member count, so a member's arithmetic is the same alone
+// and in any batch.
+bool tuned_choice(int kind, int tsize, int N, int other, PassChoice& out);
+void set_tuned_choice(int kind, int tsize, int N, int other, const PassChoice& c);
+
+void clear_tuned_choices();
+
+// Launchable (LDS, split instantiated, r2c one-round rule) / heuristic choice.
+template <typename T>
+bool choice_ok(int kind, const PassChoice& c, int N, int other);
+template <typename T>
+PassChoice default_choice(int kind, int N, int other);
+// Tuned choice if one is recorded and launchable, the heuristic otherwise.
+template <typename T>
+PassChoice choice_for(int kind, int N, int other);
+// Four-step splits instantiated for length N.
+std::vector<std::pair<int, int>> pass_splits(int N);
+
+// Row pass, real to complex (paired rows) / complex to real.  force, when
+// given, overrides the table (the tuner's trial launches).
+template <typename T>
+void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* const* src,
+                     const T* scales, long src_plane, int src_ld, int src_lev0, long src_ms,
+                     cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nrows,
+                     int ncols, int nlev, int M, hipStream_t s, const PassChoice* force);
+template <typename T>
+void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T>* const* src,
+                     long src_plane, int src_lev0, long src_ms, T* const* dst, long dst_plane,
+                     int dst_ld, int dst_lev0, long dst_ms, int nrows, int ncols, int nlev, int M,
+                     hipStream_t s, const PassChoice* force);
+// Column pass of nj jobs over nlev * M planes (c.src/dst/op/plan filled;
+// sets c.TC, c.stride, c.nlev).
+template <typename T, int DIR>
+void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny_in,
+                 hipStream_t s, const PassChoice* force);
+
+}  // namespace lg

@@ -1,0 +1,23 @@
+// Transform-pass launchers instantiated for double.
+#include "passes_impl.hpp"
+
+namespace lg {
+
+template bool choice_ok<double>(int, const PassChoice&, int, int);
+template PassChoice default_choice<double>(int, int, int);
+template PassChoice choice_for<double>(int, int, int);
+template void launch_rows_r2c<double>(KernelTimer&, const FftPlan&, int, const double* const*,
+                                   const double*, long, int, int, long, cplx<double>* const*, long,
+                                   int, long, int, int, int, int, hipStream_t,
+                                   const PassChoice*);
+template void launch_rows_c2r<double>(KernelTimer&, const FftPlan&, int, const cplx<double>* const*,
+                                   long, int, long, double* const*, long, int, int, long, int, int,
+                                   int, int, hipStream_t, const PassChoice*);
+template void launch_cols<double, -1>(KernelTimer&, ColArgs<double>&, int, int, int, int, hipStream_t,
+                                   const PassChoice*);
+template void launch_cols<double, +1>(KernelTimer&, ColArgs<double>&, int, int, int, int, hipStream_t,
+                                   const PassChoice*);
+
+std::vector<std::pair<int, int>> pass_splits(int N) { return splits_for(N); }
+
+}  // namespace lg

@@ -1,0 +1,227 @@
+// Definitions of the transform-pass launchers (included by one translation
+// unit per precision, which instantiates them).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "fft2.hpp"
+#include "passes.hpp"
+
+namespace lg {
+
+static constexpr int PBLK = 256;
+
+// ---------------------------------------------------------------------------
+// Four-step kernel instances: every (N1, N2) split a transform length may
+// use.  split_of() gives the default; the tuner may pick another.
+// ---------------------------------------------------------------------------
+#define LG_SPLITS(X)                                                                      \
+    X(4, 4) X(4, 6) X(6, 4) X(4, 8) X(8, 4) X(6, 8) X(8, 6) X(4, 12) X(8, 8) X(4, 16)    \
+    X(16, 4) X(8, 9) X(9, 8) X(8, 12) X(12, 8) X(6, 16) X(8, 16) X(16, 8) X(12, 12)      \
+    X(9, 16) X(16, 9) X(12, 16) X(16, 12) X(8, 24) X(16, 16) X(16, 24) X(24, 16)
+
+static std::vector<std::pair<int, int>> splits_for(int N) {
+    std::vector<std::pair<int, int>> v;
+#define LG_CASE(A, B) \
+    if (A * B == N) v.push_back({A, B});
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return v;
+}
+
+template <typename T>
+static bool v2_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        hipLaunchKernelGGL((k_rows_r2c_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+template <typename T>
+static bool v2_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        hipLaunchKernelGGL((k_rows_c2r_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+template <typename T, int DIR>
+static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        hipLaunchKernelGGL((k_cols_v2<T, A, B, DIR>), g, dim3(PBLK), lds, s, a);          \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+
+// Is the choice launchable: LDS within bounds, and for the four-step r2c
+// pass every row pair's first-step tasks in one round of the block
+// (S * N1 <= block size).
+template <typename T>
+bool choice_ok(int kind, const PassChoice& c, int N, int other) {
+    if (kind == PK_TRIDAG) return c.S == 64 || c.S == 128 || c.S == 256;
+    if (c.S < 1) return false;
+    if (c.n1) {
+        if (c.n1 * c.n2 != N) return false;
+        bool known = false;
+        for (auto& p : splits_for(N)) known |= (p.first == c.n1 && p.second == c.n2);
+        if (!known) return false;
+    }
+    const size_t lds = (size_t)c.S * seq_stride(c, N) * sizeof(cplx<T>);
+    if (lds > LDS_MAX) return false;
+    if (kind == PK_R2C || kind == PK_C2R) {
+        if (c.S > std::max(1, other / 2)) return false;
+        if (kind == PK_R2C && c.n1 && c.S * c.n1 > PBLK) return false;
+    } else {
+        if (c.S > other) return false;
+    }
+    return true;
+}
+
+template <typename T>
+PassChoice default_choice(int kind, int N, int other) {
+    PassChoice c;
+    if (kind == PK_TRIDAG) {
+        c.S = 128;
+        return c;
+    }
+    int n1, n2;
+    if (split_of(N, n1, n2)) {
+        c.n1 = n1;
+        c.n2 = n2;
+    }
+    const int YS = seq_stride(c, N);
+    if (kind == PK_R2C || kind == PK_C2R) {
+        int S;
+        if (c.n1) S = std::min(kind == PK_R2C ? std::min(PBLK / c.n1, 16) : 16, other / 2);
+        else S = std::min(std::max(1, 2048 / N), other / 2);
+        S = std::max(S, 1);
+        while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_MAX) --S;
+        c.S = S;
+    } else {
+        int TC = 16;
+        while (TC > 1 && (size_t)TC * YS * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
+        c.S = std::min(TC, other);
+    }
+    return c;
+}
+
+template <typename T>
+PassChoice choice_for(int kind, int N, int other) {
+    PassChoice c;
+    if (tuned_choice(kind, (int)sizeof(T), N, other, c) && choice_ok<T>(kind, c, N, other)) return c;
+    return default_choice<T>(kind, N, other);
+}
+
+// ---------------------------------------------------------------------------
+// Pass launchers.  grid.y (rows) / grid.z (columns) = nlev * M: member m
+// covers planes lev0..lev0+nlev-1 at member offset m * mstride.
+// ---------------------------------------------------------------------------
+template <typename T>
+void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* const* src,
+                     const T* scales, long src_plane, int src_ld, int src_lev0, long src_ms,
+                     cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nrows,
+                     int ncols, int nlev, int M, hipStream_t s, const PassChoice* force) {
+    const PassChoice ch = force ? *force : choice_for<T>(PK_R2C, pl.N, nrows);
+    for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
+        const int nj = std::min(MAX_JOBS, njobs - j0);
+        RowR2CArgs<T> a{};
+        a.plan = pl;
+        for (int i = 0; i < nj; ++i) {
+            a.src[i] = src[j0 + i];
+            a.dst[i] = dst[j0 + i];
+            a.scale[i] = scales ? scales[j0 + i] : (T)1;
+        }
+        a.src_plane = src_plane;
+        a.dst_plane = dst_plane;
+        a.src_ld = src_ld;
+        a.nrows = nrows;
+        a.ncols = ncols;
+        a.lev0_src = src_lev0;
+        a.lev0_dst = dst_lev0;
+        a.nlev = nlev;
+        a.src_mstride = src_ms;
+        a.dst_mstride = dst_ms;
+        a.S = ch.S;
+        a.stride = seq_stride(ch, pl.N);
+        const size_t lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
+        const double bytes = (double)nj * nlev * M * nrows *
+                             ((double)pl.N * sizeof(T) + (double)ncols * sizeof(cplx<T>));
+        tm.run(s, KK_ROWS_R2C, bytes, [&] {
+            if (!(ch.n1 && v2_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)))
+                hipLaunchKernelGGL(k_rows_r2c<T>, grid, dim3(PBLK), lds, s, a);
+        });
+    }
+}
+
+template <typename T>
+void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T>* const* src,
+                     long src_plane, int src_lev0, long src_ms, T* const* dst, long dst_plane,
+                     int dst_ld, int dst_lev0, long dst_ms, int nrows, int ncols, int nlev, int M,
+                     hipStream_t s, const PassChoice* force) {
+    const PassChoice ch = force ? *force : choice_for<T>(PK_C2R, pl.N, nrows);
+    for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
+        const int nj = std::min(MAX_JOBS, njobs - j0);
+        RowC2RArgs<T> a{};
+        a.plan = pl;
+        for (int i = 0; i < nj; ++i) {
+            a.src[i] = src[j0 + i];
+            a.dst[i] = dst[j0 + i];
+        }
+        a.src_plane = src_plane;
+        a.dst_plane = dst_plane;
+        a.dst_ld = dst_ld;
+        a.nrows = nrows;
+        a.ncols = ncols;
+        a.lev0_src = src_lev0;
+        a.lev0_dst = dst_lev0;
+        a.nlev = nlev;
+        a.src_mstride = src_ms;
+        a.dst_mstride = dst_ms;
+        a.S = ch.S;
+        a.stride = seq_stride(ch, pl.N);
+        const size_t lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
+        const double bytes = (double)nj * nlev * M * nrows *
+                             ((double)ncols * sizeof(cplx<T>) + (double)dst_ld * sizeof(T));
+        tm.run(s, KK_ROWS_C2R, bytes, [&] {
+            if (!(ch.n1 && v2_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)))
+                hipLaunchKernelGGL(k_rows_c2r<T>, grid, dim3(PBLK), lds, s, a);
+        });
+    }
+}
+
+// Column pass of nj jobs (c.src/dst/op and c.plan filled): picks the tuned
+// kernel, sets TC / stride / nlev and launches over nlev * M planes.
+template <typename T, int DIR>
+void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny_in,
+                 hipStream_t s, const PassChoice* force) {
+    const int N = c.plan.N;
+    const PassChoice ch = force ? *force : choice_for<T>(cols_kind(DIR, c.map), N, c.ncols);
+    c.TC = ch.S;
+    c.stride = seq_stride(ch, N);
+    c.nlev = nlev;
+    const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
+    dim3 grid(nj, (c.ncols + c.TC - 1) / c.TC, nlev * M);
+    const double bytes = (double)nj * nlev * M * c.ncols * sizeof(cplx<T>) * (ny_in + N);
+    tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
+        if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
+            hipLaunchKernelGGL((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
+    });
+}
+
+}  // namespace lg

@@ -7,9 +7,13 @@
 
 namespace lg {
 
+// A launch covers levels k0..k0+nlev-1 of every member: grid.y = nlev * M,
+// member m = blockIdx.y / nlev; member m's arrays start at m * mstride.
 struct Geo {
     int nx, ny, nz, ld;
     long plane;
+    int k0, nlev;
+    long mstride;
 };
 
 __device__ __forceinline__ bool ij_of(int W, int ny, int& j, int& i) {
@@ -20,50 +24,70 @@ __device__ __forceinline__ bool ij_of(int W, int ny, int& j, int& i) {
     return true;
 }
 
+__device__ __forceinline__ bool pt_of(const Geo& g, int W, int& m, int& k, int& j, int& i,
+                                      long& mb) {
+    if (!ij_of(W, g.ny, j, i)) return false;
+    m = blockIdx.y / g.nlev;
+    k = g.k0 + (int)(blockIdx.y - m * g.nlev);
+    mb = (long)m * g.mstride;
+    return true;
+}
+
+// f[m * mstride + q] = v for q < n, every member (grid.y = members)
 template <typename T>
-__global__ void k_fill_planes(T* f, long plane, long n, T v) {
+__global__ void k_fill_planes(T* f, long mstride, long n, T v) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n) f[idx] = v;
+    if (idx < n) f[(long)blockIdx.y * mstride + idx] = v;
+}
+
+template <typename T>
+__global__ void k_copy_plane(const T* src, T* dst, long mstride, long n) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) dst[(long)blockIdx.y * mstride + idx] = src[(long)blockIdx.y * mstride + idx];
 }
 
 // ddz_uv (derivatives.f90:214-264): dfdz(k) = (1/dz)(f(k) - f(k-1)), k=1..nz.
 template <typename T>
 __global__ void k_ddz_uv(const T* f, T* out, Geo g, T rdz) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     out[o] = rdz * (f[o] - f[o - g.plane]);
 }
 
 // ddz_w (derivatives.f90:267-311): dfdz(k) = (1/dz)(f(k+1) - f(k)), k=0..nz-1.
 template <typename T>
 __global__ void k_ddz_w(const T* f, T* out, Geo g, T rdz) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     out[o] = rdz * (f[o + g.plane] - f[o]);
 }
 
-// ws_equilibrium_lbc (wallstress.f90:171-219) on the k=1 plane.
+// ws_equilibrium_lbc (wallstress.f90:171-219) on the k=1 plane; u1f, v1f
+// are the test-filtered k=1 planes (member stride fmstride), the other
+// pointers field bases; denom[m] = ln(0.5 dz / z0_m).
 template <typename T>
-__global__ void k_wall_lbc(const T* u1f, const T* v1f, const T* u1, const T* v1, T* txz1,
-                           T* tyz1, T* dudz1, T* dvdz1, Geo g, double denom, double vonk,
-                           double dz) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const long o = (long)j * g.ld + i;
-    const double uf = u1f[o], vf = v1f[o];
+__global__ void k_wall_lbc(const T* u1f, const T* v1f, long fmstride, const T* u, const T* v,
+                           T* txz, T* tyz, T* dudz, T* dvdz, Geo g, const double* denom,
+                           double vonk, double dz) {
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
+    const long of = (long)m * fmstride + (long)j * g.ld + i;
+    const double uf = u1f[of], vf = v1f[of];
     const double ua = sqrt(uf * uf + vf * vf);
-    const double us = ua * vonk / denom;
+    const double us = ua * vonk / denom[m];
     const double c = -(us * us) / ua;
-    txz1[o] = (T)(c * uf);
-    tyz1[o] = (T)(c * vf);
-    const double uu = u1[o], vv = v1[o];
+    txz[o] = (T)(c * uf);
+    tyz[o] = (T)(c * vf);
+    const double uu = u[o], vv = v[o];
     const double a = us / (0.5 * dz * vonk);
-    dudz1[o] = (uu == 0.0) ? (T)0 : (T)(a * uu / ua);
-    dvdz1[o] = (vv == 0.0) ? (T)0 : (T)(a * vv / ua);
+    dudz[o] = (uu == 0.0) ? (T)0 : (T)(a * uu / ua);
+    dvdz[o] = (vv == 0.0) ? (T)0 : (T)(a * vv / ua);
 }
 
 // calc_Sij (sgs_stag_util.f90:467-634), one rank, lbc_mom >= 1, ubc_mom = 0.
@@ -75,11 +99,11 @@ struct SijArgs {
 
 template <typename T>
 __global__ void k_calc_sij(SijArgs<T> a, Geo g) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
     const long P = g.plane;
-    const long o = (long)k * P + (long)j * g.ld + i;
+    const long o = mb + (long)k * P + (long)j * g.ld + i;
     const T h = (T)0.5;
     if (k == 1) {
         a.S11[o] = a.dudx[o];
@@ -132,11 +156,11 @@ __device__ __forceinline__ T nut_at(const TauArgs<T>& a, long o) {
 
 template <typename T>
 __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
     const long P = g.plane;
-    const long o = (long)k * P + (long)j * g.ld + i;
+    const long o = mb + (long)k * P + (long)j * g.ld + i;
     const T nk = nut_at(a, o);
     a.Nu_t[o] = nk;
     const T nu = a.nu;
@@ -177,10 +201,10 @@ __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
 // dxy holds ddx(t_x1) + ddy(t_x2) formed in spectral space.
 template <typename T>
 __global__ void k_divt_uv(const T* dxy, const T* tz, T* out, Geo g, T rdz) {
-    int j, i;
-    if (!ij_of(g.ld, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     if (i >= g.nx) {
         out[o] = (T)0;
         return;
@@ -192,10 +216,10 @@ __global__ void k_divt_uv(const T* dxy, const T* tz, T* out, Geo g, T rdz) {
 // divstress_w sum (divstress_w.f90:63-114): k = 1..nz.
 template <typename T>
 __global__ void k_divt_w(const T* dxy, const T* tzz, T* out, Geo g, T rdz) {
-    int j, i;
-    if (!ij_of(g.ld, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     if (i >= g.nx) {
         if (k < g.nz) out[o] = (T)0;
         return;
@@ -218,11 +242,11 @@ struct VortArgs {
 
 template <typename T>
 __global__ void k_vort(VortArgs<T> a, Geo g) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
     const long P = g.plane;
-    const long o = (long)k * P + (long)j * g.ld + i;
+    const long o = mb + (long)k * P + (long)j * g.ld + i;
     if (k == 1) {
         a.o1[o] = a.c * ((T)0.5 * (a.dwdy[o] + a.dwdy[o + P]) - a.dvdz[o]);
         a.o2[o] = a.c * (a.dudz[o] - (T)0.5 * (a.dwdx[o] + a.dwdx[o + P]));
@@ -243,10 +267,20 @@ template <typename T>
 __global__ void k_vort_spec(const cplx<T>* U, const cplx<T>* V, const cplx<T>* W,
                             const cplx<T>* Dv1, const cplx<T>* Du1, cplx<T>* O1, cplx<T>* O2,
                             cplx<T>* O3, int ny, int NK, int nz, long splane, T kxs, T kys,
-                            T rdz) {
+                            T rdz, long smstride) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= ny * NK) return;
-    const int k = blockIdx.y + 1;
+    const int m = blockIdx.y / nz;
+    const int k = blockIdx.y - m * nz + 1;
+    const long mb = (long)m * smstride;
+    U += mb;
+    V += mb;
+    W += mb;
+    Dv1 += mb;
+    Du1 += mb;
+    O1 += mb;
+    O2 += mb;
+    O3 += mb;
     const int j = idx / NK, c = idx - j * NK;
     T kx = kxs * (T)c, ky = kys * (T)((j < ny / 2) ? j : j - ny);
     if (j == ny / 2) kx = ky = (T)0;
@@ -286,12 +320,13 @@ struct ProdArgs {
 };
 
 template <typename T>
-__global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz) {
+__global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz, long bmstride) {
     int j, i;
     if (!ij_of(nx2, ny2, j, i)) return;
-    const int k = blockIdx.y + 1;
+    const int m = blockIdx.y / nz;
+    const int k = blockIdx.y - m * nz + 1;
     const long P = (long)ny2 * ldb;
-    const long o = (long)k * P + (long)j * ldb + i;
+    const long o = (long)m * bmstride + (long)k * P + (long)j * ldb + i;
     const T h = (T)0.5;
     const T c = a.c;
     if (k <= nz - 1) {
@@ -322,23 +357,24 @@ template <typename T>
 struct Ab2Args {
     T *u, *v, *w, *rx, *ry, *rz;
     const T *rxf, *ryf, *rzf, *dx, *dy, *dz, *fx, *fy, *fz;
-    T dt, t1, t2, mpx, mpy;
+    T dt, t1, t2;
+    const double* mpf;     // 2 per member: mean_p_force_x, mean_p_force_y
     int use_mpf;
 };
 
 template <typename T>
 __global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
-    int j, i;
-    if (!ij_of(g.ld, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     if (k <= g.nz - 1) {
         T rx = -a.rx[o] - a.dx[o];
         T ry = -a.ry[o] - a.dy[o];
         T rz = -a.rz[o] - a.dz[o];
         if (a.use_mpf) {
-            rx = rx + a.mpx;
-            ry = ry + a.mpy;
+            rx = rx + (T)a.mpf[2 * m];
+            ry = ry + (T)a.mpf[2 * m + 1];
         }
         if (a.fx) {
             rx = rx + a.fx[o];
@@ -361,10 +397,10 @@ __global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
 // RHS -= grad p (main.f90:321-326), k = 1..nz (z only at nz).
 template <typename T>
 __global__ void k_rhs_gradp(T* rx, T* ry, T* rz, const T* px, const T* py, const T* pz, Geo g) {
-    int j, i;
-    if (!ij_of(g.ld, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     if (k <= g.nz - 1) {
         rx[o] = rx[o] - px[o];
         ry[o] = ry[o] - py[o];
@@ -375,10 +411,10 @@ __global__ void k_rhs_gradp(T* rx, T* ry, T* rz, const T* px, const T* py, const
 // dpdz by differences (press_stag_array.f90:284-288), k = 1..nz.
 template <typename T>
 __global__ void k_dpdz(const T* p, T* dpdz, Geo g, T dzv) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     dpdz[o] = (p[o] - p[o - g.plane]) / dzv;
 }
 
@@ -386,19 +422,13 @@ __global__ void k_dpdz(const T* p, T* dpdz, Geo g, T dzv) {
 template <typename T>
 __global__ void k_project(T* u, T* v, T* w, const T* px, const T* py, const T* pz, Geo g, T dt,
                           T t1) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     u[o] = u[o] + dt * (-(t1 * px[o]));
     v[o] = v[o] + dt * (-(t1 * py[o]));
     if (k >= 2) w[o] = w[o] + dt * (-(t1 * pz[o]));
-}
-
-template <typename T>
-__global__ void k_copy_plane(const T* src, T* dst, long n) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n) dst[idx] = src[idx];
 }
 
 // Block-partial sums for the diagnostics (rmsdiv.f90:39-50, io.f90:115-124).

@@ -10,11 +10,13 @@ namespace lg {
 struct TriGeo {
     int ny, NK, nz;
     long splane;         // ny * NK complex per level
+    long smstride;       // complex elements between members
+    int M;
     double kxs, kys;     // 2 pi / Lx, 2 pi / Ly
     double dz, const3, const4;
 };
 
-// One thread per (ky, kx < nx/2).  Hx, Hy hold levels 1..nz-1, Hz levels
+// One thread per (member, ky, kx < nx/2).  Hx, Hy hold levels 1..nz-1, Hz levels
 // 1..nz (all already scaled by 1/(nx ny tadv1 dt)); rb, rt the bottom/top
 // Neumann planes scaled by 1/(nx ny).  Writes p_hat for rows 0..nz and
 // uses gam and usc (FP64, (nz+1) * ny * NK each) as the Thomas scratch.
@@ -22,8 +24,19 @@ template <typename T>
 __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz,
                          const cplx<T>* rb, const cplx<T>* rt, cplx<T>* ph, double* gam,
                          double2* usc, TriGeo g, int* err) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.ny * g.NK) return;
+    const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gidx >= g.M * g.ny * g.NK) return;
+    const int mi = gidx / (g.ny * g.NK);
+    const int idx = gidx - mi * (g.ny * g.NK);
+    const long mb = (long)mi * g.smstride;
+    Hx += mb;
+    Hy += mb;
+    Hz += mb;
+    rb += mb;
+    rt += mb;
+    ph += mb;
+    gam += mb;
+    usc += mb;
     const int jy = idx / g.NK;
     const int jx = idx - jy * g.NK;
     const long S = g.splane;

@@ -98,7 +98,7 @@ static int fnint(double x) {
 // Disk-averaged velocity, thrust (turbines.f90:517-580): one block per
 // turbine; T_avg_dim <= 0 so eps = 1.
 template <typename T>
-__global__ void k_turb_reduce(const T* u, const T* v, const T* w, const int* node_lin,
+__global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* node_lin,
                               const double* node_ind, const int* off, const double* coef,
                               double* state, long plane, double dV) {
     __shared__ double sh[256];
@@ -131,7 +131,7 @@ __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const int* nod
 
 // Scatter f_n n ind to the nodes (turbines.f90:597-617).
 template <typename T>
-__global__ void k_turb_scatter(T* fx, T* fy, T* fzu, const int* node_lin, const double* node_ind,
+__global__ void k_turb_scatter(T* fx, T* fy, T* fzu, const long* node_lin, const double* node_ind,
                                const int* node_tid, const double* coef, const double* state,
                                int total) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,25 +147,22 @@ __global__ void k_turb_scatter(T* fx, T* fy, T* fzu, const int* node_lin, const 
 // fza = interp_to_w_grid(fza) (functions.f90:131-185), k = 1..nz.
 template <typename T>
 __global__ void k_interp_w(const T* src, T* dst, Geo g) {
-    int j, i;
-    if (!ij_of(g.ld, g.ny, j, i)) return;
-    const int k = blockIdx.y + 1;
-    const long o = (long)k * g.plane + (long)j * g.ld + i;
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
     dst[o] = (T)0.5 * (src[o - g.plane] + src[o]);
 }
 
 template <typename T>
-void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, double alpha2,
-                   double filter_cutoff, int adm_correction) {
+void turbines_init(Case<T>& c, int m, int nturb, const double* spec8, double alpha1,
+                   double alpha2, double filter_cutoff, int adm_correction) {
     const Dims& d = c.d;
-    auto ts = std::make_unique<TurbineSet>();
-    ts->n = nturb;
-    ts->adm_correction = adm_correction != 0;
-    ts->spec.assign(spec8, spec8 + 8 * nturb);
+    MemberTurbines mt;
+    mt.spec.assign(spec8, spec8 + 8 * nturb);
+    mt.off.push_back(0);
     const double h = std::sqrt(d.dx * d.dx + d.dy * d.dy + d.dz * d.dz);
     const double delta1 = alpha1 * h, delta2 = alpha2 * h;
-    std::vector<int> lin, tid, off(1, 0);
-    std::vector<double> ind, coef;
     const int nx = d.nx, ny = d.ny;
     for (int s = 0; s < nturb; ++s) {
         const double* t = spec8 + 8 * s;
@@ -173,7 +170,7 @@ void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, do
         const double thk = std::max(t[4], d.dx * 1.01);
         const double th1 = t[5], th2 = t[6], ctp = t[7];
         Indicator itf(delta1, delta2, thk, dia);
-        ts->M.push_back(itf.M);
+        mt.M.push_back(itf.M);
         const double search_rad = 0.5 * dia + 3 * std::max(alpha1, alpha2) * h;
         const int imax = std::min((int)(search_rad / d.dx + 2), nx / 2);
         const int jmax = std::min((int)(search_rad / d.dy + 2), ny / 2);
@@ -184,7 +181,7 @@ void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, do
         const int jcp = fnint(yloc / d.dy);
         const double filt_max = itf.val(0.0, 0.0);
         std::vector<double> filt_l;
-        std::vector<int> lin_l;
+        std::vector<long> lin_l;
         double sumA = 0.0;
         for (int k = 1; k <= d.nz - 1; ++k) {
             for (int j = jcp - jmax + 1; j <= jcp + jmax; ++j) {
@@ -218,23 +215,77 @@ void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, do
                     const double f = itf.val(r_disk, r_norm);
                     if (f > filter_cutoff * filt_max) {
                         filt_l.push_back(f);
-                        lin_l.push_back((int)(((long)k * ny + (j2 - 1)) * d.ld + (i2 - 1)));
+                        lin_l.push_back(((long)k * ny + (j2 - 1)) * d.ld + (i2 - 1));
                         sumA = sumA + f * d.dx * d.dy * d.dz;
                     }
                 }
             }
         }
         for (size_t l = 0; l < filt_l.size(); ++l) {
-            ind.push_back(filt_l[l] / sumA);
-            lin.push_back(lin_l[l]);
-            tid.push_back(s);
+            mt.ind.push_back(filt_l[l] / sumA);
+            mt.lin.push_back(lin_l[l]);
         }
-        off.push_back((int)lin.size());
-        const double corr = ts->adm_correction ? (1 + 0.25 * (1 - itf.M) * ctp) : 1.0;
-        coef.insert(coef.end(), {n1, n2, n3, ctp, dia, corr});
+        mt.off.push_back((int)mt.lin.size());
+        const double corr = adm_correction ? (1 + 0.25 * (1 - itf.M) * ctp) : 1.0;
+        mt.coef.insert(mt.coef.end(), {n1, n2, n3, ctp, dia, corr});
+        // u_d_T without a u_d_T.dat (turbines.f90:245-250)
+        mt.state.insert(mt.state.end(), {0.0, -1.0, 0.0});
     }
+    c.pull_turbine_state();     // the other members keep their disk state
+    c.mturb[m] = std::move(mt);
+    c.rebuild_turbines();
+}
+
+// Copy the device disk state (u_d, u_d_T, f_n) into the members' host lists.
+template <typename T>
+void Case<T>::pull_turbine_state() {
+    LG_HIP(hipStreamSynchronize(stream));
+    if (!turb || turb->n == 0) return;
+    std::vector<double> st(3 * (size_t)turb->n);
+    LG_HIP(hipMemcpy(st.data(), turb->state, st.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int m = 0; m < M; ++m) {
+        const int f0 = turb->first[m];
+        std::copy(st.begin() + 3 * f0, st.begin() + 3 * turb->first[m + 1],
+                  mturb[m].state.begin());
+    }
+}
+
+// Concatenate every member's disks into the device node lists.  The forcing
+// fields are cleared: they are zero off the nodes and rewritten on the nodes
+// every step.
+template <typename T>
+void Case<T>::rebuild_turbines() {
+    LG_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
+    turb.reset();
+    auto ts = std::make_unique<TurbineSet>();
+    std::vector<long> lin;
+    std::vector<int> tid, off(1, 0);
+    std::vector<double> ind, coef, st;
+    ts->first.push_back(0);
+    int s = 0;
+    for (int m = 0; m < M; ++m) {
+        const MemberTurbines& mt = mturb[m];
+        for (int q = 0; q < mt.n(); ++q, ++s) {
+            for (int l = mt.off[q]; l < mt.off[q + 1]; ++l) {
+                lin.push_back((long)m * d.field + mt.lin[l]);
+                ind.push_back(mt.ind[l]);
+                tid.push_back(s);
+            }
+            off.push_back((int)lin.size());
+            coef.insert(coef.end(), mt.coef.begin() + 6 * q, mt.coef.begin() + 6 * q + 6);
+            st.insert(st.end(), mt.state.begin() + 3 * q, mt.state.begin() + 3 * q + 3);
+            if (mt.coef[6 * q + 2] != 0.0) ts->has_fz = true;
+        }
+        ts->first.push_back(s);
+    }
+    for (int id : {LG_FXA, LG_FYA, LG_FZA})
+        LG_HIP(hipMemsetAsync(fld[id], 0, M * d.field * sizeof(T), stream));
+    LG_HIP(hipMemsetAsync(tmp[8], 0, M * d.field * sizeof(T), stream));
+    LG_HIP(hipStreamSynchronize(stream));
+    if (s == 0) return;
+    ts->n = s;
     ts->total_nodes = (int)lin.size();
-    ts->h_off = off;
     auto up = [&](auto*& dptr, const auto& hv) {
         using E = typename std::decay<decltype(hv[0])>::type;
         LG_HIP(hipMalloc(&dptr, std::max<size_t>(1, hv.size()) * sizeof(E)));
@@ -246,20 +297,8 @@ void turbines_init(Case<T>& c, int nturb, const double* spec8, double alpha1, do
     up(ts->node_tid, tid);
     up(ts->off, off);
     up(ts->coef, coef);
-    std::vector<double> st(3 * nturb);
-    for (int s = 0; s < nturb; ++s) {
-        st[3 * s + 0] = 0.0;
-        st[3 * s + 1] = -1.0;      // u_d_T without a u_d_T.dat (turbines.f90:245-250)
-        st[3 * s + 2] = 0.0;
-    }
     up(ts->state, st);
-    ts->has_fz = false;
-    for (int s = 0; s < nturb; ++s)
-        if (coef[6 * s + 2] != 0.0) ts->has_fz = true;
-    LG_HIP(hipStreamSynchronize(c.stream));
-    for (int id : {LG_FXA, LG_FYA, LG_FZA})
-        LG_HIP(hipMemsetAsync(c.fld[id], 0, d.field * sizeof(T), c.stream));
-    c.turb = std::move(ts);
+    turb = std::move(ts);
 }
 
 template <typename T>
@@ -267,24 +306,32 @@ void Case<T>::turbines_forcing() {
     if (!turb) return;
     TurbineSet& t = *turb;
     if (t.total_nodes == 0) return;
-    hipLaunchKernelGGL(k_turb_reduce<T>, dim3(t.n), dim3(256), 0, stream, fld[LG_U], fld[LG_V],
-                       fld[LG_W], t.node_lin, t.node_ind, t.off, t.coef, t.state, d.plane,
-                       d.dx * d.dy * d.dz);
-    // the node set is static, so fxa/fya/fza are zero off the nodes after init
+    hipLaunchKernelGGL(k_turb_reduce<T>, dim3(t.n), dim3(256), 0, stream, (const T*)fld[LG_U],
+                       (const T*)fld[LG_V], (const T*)fld[LG_W], (const long*)t.node_lin,
+                       (const double*)t.node_ind, (const int*)t.off, (const double*)t.coef,
+                       t.state, d.plane, d.dx * d.dy * d.dz);
+    // the node set is static, so fxa/fya/fza are zero off the nodes
     T* fz_uv = t.has_fz ? tmp[8] : fld[LG_FZA];
     hipLaunchKernelGGL(k_turb_scatter<T>, dim3((t.total_nodes + 255) / 256), dim3(256), 0, stream,
-                       fld[LG_FXA], fld[LG_FYA], fz_uv, t.node_lin, t.node_ind, t.node_tid,
-                       t.coef, t.state, t.total_nodes);
+                       fld[LG_FXA], fld[LG_FYA], fz_uv, (const long*)t.node_lin,
+                       (const double*)t.node_ind, (const int*)t.node_tid, (const double*)t.coef,
+                       (const double*)t.state, t.total_nodes);
     if (t.has_fz) {
-        Geo g{d.nx, d.ny, d.nz, d.ld, d.plane};
-        hipLaunchKernelGGL(k_interp_w<T>, dim3((unsigned)((d.ld * d.ny + 255) / 256), d.nz),
-                           dim3(256), 0, stream, tmp[8], fld[LG_FZA], g);
+        Geo g{d.nx, d.ny, d.nz, d.ld, d.plane, 1, d.nz, d.field};
+        hipLaunchKernelGGL(k_interp_w<T>, dim3((unsigned)((d.ld * d.ny + 255) / 256), d.nz * Ma),
+                           dim3(256), 0, stream, (const T*)tmp[8], fld[LG_FZA], g);
     }
 }
 
-template void turbines_init<double>(Case<double>&, int, const double*, double, double, double, int);
-template void turbines_init<float>(Case<float>&, int, const double*, double, double, double, int);
+template void turbines_init<double>(Case<double>&, int, int, const double*, double, double,
+                                    double, int);
+template void turbines_init<float>(Case<float>&, int, int, const double*, double, double, double,
+                                   int);
 template void Case<double>::turbines_forcing();
 template void Case<float>::turbines_forcing();
+template void Case<double>::rebuild_turbines();
+template void Case<float>::rebuild_turbines();
+template void Case<double>::pull_turbine_state();
+template void Case<float>::pull_turbine_state();
 
 }  // namespace lg
