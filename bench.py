@@ -90,6 +90,94 @@ def time_precision(name, precision, steps, warmup):
     return res
 
 
+ENSEMBLE = dict(members=48, nx=96, ny=96, Nz=64, dt=2e-4, steps=200, dia=0.15, hub=0.1,
+                ct_prime=1.33)
+
+
+def ensemble_members(n, rng_seed=0):
+    """Per-member (seed, z0, mean_p_force_x, disks): seeds 0..n-1, z0
+    log-uniform in [1e-5, 1e-3], mean pressure gradient uniform in [0.5, 1.5],
+    every other member with a 2x2 aligned disk array."""
+    import numpy as np
+    rng = np.random.default_rng(rng_seed)
+    zo = 10.0 ** rng.uniform(-5.0, -3.0, n)
+    mpf = rng.uniform(0.5, 1.5, n)
+    return [(m, float(zo[m]), float(mpf[m]), m % 2 == 1) for m in range(n)]
+
+
+def ensemble_case(precision, members):
+    import lesgo_amd
+    E = ENSEMBLE
+    cfg = lesgo_amd.Config(E["nx"], E["ny"], E["Nz"], dt=E["dt"], precision=precision)
+    c = lesgo_amd.Case(cfg, members=len(members))
+    farm = lesgo_amd.aligned_farm(cfg, 2, 2, dia=E["dia"], hub=E["hub"], sx_d=7.0, sy_d=5.0,
+                                  ct_prime=E["ct_prime"])
+    for m, (seed, zo, mpf, turb) in enumerate(members):
+        c.set_member(m, zo, mpf, 0.0)
+        c.seed_state(seed, 0.1, member=m)
+        if turb:
+            c.turbines_init(farm, member=m)
+    c.set_counters(0, 0)
+    return cfg, c
+
+
+def ensemble_leg(steps):
+    """48 independent 96x96x64 half channels in one batch: mixed precision,
+    autotuned launch choices, graph-replayed steps; per-step latency from
+    host clocks around each synchronised step; per-member error against the
+    same batch in plain FP64 (a batch member computes exactly what that
+    case computes alone)."""
+    import numpy as np
+    E = ENSEMBLE
+    members = ensemble_members(E["members"])
+    npts = E["nx"] * E["ny"] * E["Nz"] * len(members)
+    cfg, c = ensemble_case("mixed", members)
+    t0 = time.perf_counter()
+    npass = c.autotune(5)
+    t_tune = time.perf_counter() - t0
+    c.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        c.step_graph(1)
+        c.synchronize()
+        lat.append(time.perf_counter() - a)
+    dt = time.perf_counter() - t0
+    mixed = {m: {n: c.download(n, member=m) for n in ("u", "v", "w")} for m in range(len(members))}
+    c.close()
+    log(f"[bench] ensemble mixed: {npts * steps / dt / 1e6:.1f} Mpts*steps/s")
+    cfg64, c64 = ensemble_case("fp64", members)
+    t0 = time.perf_counter()
+    c64.step_graph(steps)
+    c64.synchronize()
+    dt64 = time.perf_counter() - t0
+    X = np.s_[1:cfg.Nz + 1, :, :cfg.nx]
+    errs = []
+    for m in range(len(members)):
+        e = []
+        for n in ("u", "v", "w"):
+            r = c64.download(n, member=m)[X]
+            e.append(float(np.linalg.norm(mixed[m][n][X] - r) / np.linalg.norm(r)))
+        errs.append(max(e))
+    c64.close()
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "workload": (f"{len(members)} x {E['nx']}x{E['ny']}x{E['Nz']} half channels, seeds 0-47, "
+                     "z0 log-uniform [1e-5,1e-3], mean_p_force U[0.5,1.5], odd members 2x2 ADM "
+                     f"D={E['dia']} Ct'={E['ct_prime']}, dt={E['dt']}"),
+        "steps": steps,
+        "value": npts * steps / dt / 1e6,
+        "unit": "Mpts*steps/s",
+        "step_ms_p50": float(np.percentile(lat_ms, 50)),
+        "step_ms_p99": float(np.percentile(lat_ms, 99)),
+        "fp64_value": npts * steps / dt64 / 1e6,
+        "autotune_s": t_tune,
+        "autotuned_passes": npass,
+        "member_vel_rel_l2_vs_fp64": {"max": max(errs), "median": float(np.median(errs))},
+    }
+
+
 def cpu_baseline(name):
     """The numpy oracle on this host: one plain step of the full workload,
     scaled by the dynamic/plain step-time ratio measured at 1/8 the points
@@ -139,6 +227,8 @@ def main():
     ap.add_argument("--workload", default="windfarm256", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--ensemble-steps", type=int, default=ENSEMBLE["steps"],
+                    help="steps of the 48-member ensemble leg (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -193,6 +283,11 @@ def main():
     if "fp64" in out and "mixed" in out:
         line["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     line["kernels"] = head["kernels"]
+    if args.ensemble_steps > 0:
+        log(f"[bench] ensemble: {ENSEMBLE['members']} members, {args.ensemble_steps} steps")
+        line["ensemble"] = ensemble_leg(args.ensemble_steps)
+        log(f"[bench] ensemble done: {line['ensemble']['value']:.1f} Mpts*steps/s, step p50 "
+            f"{line['ensemble']['step_ms_p50']:.2f} ms p99 {line['ensemble']['step_ms_p99']:.2f} ms")
     if not args.no_cpu:
         log("[bench] cpu baseline (numpy oracle)")
         line["cpu_baseline"] = cpu_baseline(args.workload)

@@ -225,6 +225,38 @@ int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int
 int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg);
 int lg_tune_clear(void);
 
+/* Device bytes lg_create_batch allocates for `members` members. */
+int lg_case_bytes(const lg_params* prm, int members, double* bytes);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int lg_mem_info(double* free_bytes, double* total_bytes);
+
+/* --- serving loop (no counterpart in the reference).  Jobs of different
+ * grids are admitted under an HBM budget into pools (batches of
+ * pool_members slots, one stream each) of their grid class (all lg_params
+ * fields but zo and mean_p_force_x/y), kept device-resident and stepped
+ * together: each lg_server_tick admits queued jobs into free slots (a job
+ * joins a pool only at a multiple of cs_count steps, so it computes exactly
+ * what it computes alone), replays one captured step of every non-empty pool
+ * (pools overlap on the device), waits, and retires finished jobs.  Times
+ * are seconds on the server's clock (lg_server_now). */
+typedef struct lg_server lg_server;
+int lg_server_create(double budget_bytes, int pool_members, lg_server** out);
+int lg_server_destroy(lg_server* s);
+int lg_server_now(lg_server* s, double* t);
+/* Queue a job: seeded start (lg_seed_state with prm->zo), nsteps steps,
+ * nturb disks (spec8 as lg_turbines_init; alpha 1.5, cutoff 1e-2, ADM
+ * correction on).  keep_fields keeps u, v, w, p at completion. */
+int lg_server_submit(lg_server* s, const lg_params* prm, long seed, double noise, int nsteps,
+                     int nturb, const double* spec8, double t_arrive, int keep_fields,
+                     int* job);
+int lg_server_tick(lg_server* s, double* tick_seconds, int* running, int* queued);
+/* state: 0 queued, 1 running, 2 done. */
+int lg_server_job(lg_server* s, int job, int* state, int* steps_done, double* t_arrive,
+                  double* t_admit, double* t_done);
+int lg_server_result(lg_server* s, int job, int field, double* host);
+int lg_server_stats(lg_server* s, double* pts_steps, long* ticks, double* bytes_used,
+                    int* npools);
+
 #ifdef __cplusplus
 }
 #endif

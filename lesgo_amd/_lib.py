@@ -95,6 +95,17 @@ SIGNATURES = {
     "lg_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _I, _I, _I]),
     "lg_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "lg_tune_clear": (C.c_int, []),
+    "lg_case_bytes": (C.c_int, [C.POINTER(LgParams), C.c_int, _D]),
+    "lg_mem_info": (C.c_int, [_D, _D]),
+    "lg_server_create": (C.c_int, [C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
+    "lg_server_destroy": (C.c_int, [_P]),
+    "lg_server_now": (C.c_int, [_P, _D]),
+    "lg_server_submit": (C.c_int, [_P, C.POINTER(LgParams), C.c_long, C.c_double, C.c_int,
+                                   C.c_int, _D, C.c_double, C.c_int, _I]),
+    "lg_server_tick": (C.c_int, [_P, _D, _I, _I]),
+    "lg_server_job": (C.c_int, [_P, C.c_int, _I, _I, _D, _D, _D]),
+    "lg_server_result": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_server_stats": (C.c_int, [_P, _D, C.POINTER(C.c_long), _D, _I]),
 }
 
 PASS_KINDS = ("r2c", "c2r", "cols_fwd", "cols_inv", "tridag", "cols_unpad", "cols_pad")

@@ -177,59 +177,10 @@ int lg_download(lg_case* h, int field, double* host) {
     return lg_download_member(h, 0, field, host);
 }
 
-namespace {
-inline uint64_t splitmix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ULL;
-    uint64_t z = x;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-inline double uniform01(uint64_t seed, uint64_t fid, uint64_t i, uint64_t j, uint64_t k) {
-    uint64_t key = seed * 0x100000001B3ULL;
-    key = splitmix64(key ^ fid);
-    key = splitmix64(key ^ ((k << 42) ^ (j << 21) ^ i));
-    return (double)(key >> 11) * (1.0 / 9007199254740992.0);
-}
-}  // namespace
-
 int lg_seed_member(lg_case* h, int m, long seed, double noise) {
     return dispatch(h, [&](auto& c) {
         check_member(c, m);
-        const lg::Dims& d = c.d;
-        const double zo = c.zo[m];
-        using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
-        std::vector<std::vector<T>> f(4, std::vector<T>((size_t)d.field, (T)0));
-        for (int k = 1; k < d.nz; ++k) {
-            const double zk = (k - 0.5) * d.dz;
-            const double ubar = (1.0 / d.prm.vonk) * std::log(zk / zo);
-            const double amp = noise * ubar;
-            for (int j = 0; j < d.ny; ++j)
-                for (int i = 0; i < d.nx; ++i) {
-                    const size_t o = ((size_t)k * d.ny + j) * d.ld + i;
-                    for (int q = 0; q < 3; ++q) {
-                        const double r = uniform01((uint64_t)seed, q, i, j, k);
-                        const double base = (q == 0) ? ubar : 0.0;
-                        f[q][o] = (T)(base + amp * (2.0 * r - 1.0));
-                    }
-                }
-        }
-        const size_t P = (size_t)d.plane;
-        for (size_t o = 0; o < P; ++o) {
-            f[2][1 * P + o] = 0;
-            f[2][(size_t)d.nz * P + o] = 0;
-            f[0][(size_t)d.nz * P + o] = f[0][(size_t)(d.nz - 1) * P + o];
-            f[1][(size_t)d.nz * P + o] = f[1][(size_t)(d.nz - 1) * P + o];
-            for (int q = 0; q < 3; ++q) f[q][o] = (T)lg::BOGUS;
-        }
-        for (size_t o = P; o < (size_t)d.field; ++o) f[3][o] = (T)0.03;
-        for (int id = 0; id < LG_NFIELDS; ++id)
-            LG_HIP(hipMemsetAsync(c.mem(id, m), 0, d.field * sizeof(T), c.stream));
-        const int ids[4] = {LG_U, LG_V, LG_W, LG_CS_OPT2};
-        for (int q = 0; q < 4; ++q)
-            LG_HIP(hipMemcpyAsync(c.mem(ids[q], m), f[q].data(), d.field * sizeof(T),
-                                  hipMemcpyHostToDevice, c.stream));
-        LG_HIP(hipStreamSynchronize(c.stream));
+        c.seed_member(m, seed, noise);
     });
 }
 

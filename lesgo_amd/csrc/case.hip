@@ -3,6 +3,7 @@
 // kernel headers for the per-loop citations.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -213,6 +214,78 @@ Case<T>::~Case() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+namespace {
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t seed, uint64_t fid, uint64_t i, uint64_t j, uint64_t k) {
+    uint64_t key = seed * 0x100000001B3ULL;
+    key = splitmix64(key ^ fid);
+    key = splitmix64(key ^ ((k << 42) ^ (j << 21) ^ i));
+    return (double)(key >> 11) * (1.0 / 9007199254740992.0);
+}
+}  // namespace
+
+// Seeded start of member m (the vel.out-style state of initial.f90:226-239):
+// log-law u with the member's z0 plus uniform noise, w = 0 at wall and lid,
+// u(nz) = u(nz-1), Cs_opt2 = 0.03, everything else 0, level 0 BOGUS.
+template <typename T>
+void Case<T>::seed_member(int m, long seed, double noise) {
+    std::vector<std::vector<T>> f(4, std::vector<T>((size_t)d.field, (T)0));
+    for (int k = 1; k < d.nz; ++k) {
+        const double zk = (k - 0.5) * d.dz;
+        const double ubar = (1.0 / d.prm.vonk) * std::log(zk / zo[m]);
+        const double amp = noise * ubar;
+        for (int j = 0; j < d.ny; ++j)
+            for (int i = 0; i < d.nx; ++i) {
+                const size_t o = ((size_t)k * d.ny + j) * d.ld + i;
+                for (int q = 0; q < 3; ++q) {
+                    const double r = uniform01((uint64_t)seed, q, i, j, k);
+                    const double base = (q == 0) ? ubar : 0.0;
+                    f[q][o] = (T)(base + amp * (2.0 * r - 1.0));
+                }
+            }
+    }
+    const size_t P = (size_t)d.plane;
+    for (size_t o = 0; o < P; ++o) {
+        f[2][1 * P + o] = 0;
+        f[2][(size_t)d.nz * P + o] = 0;
+        f[0][(size_t)d.nz * P + o] = f[0][(size_t)(d.nz - 1) * P + o];
+        f[1][(size_t)d.nz * P + o] = f[1][(size_t)(d.nz - 1) * P + o];
+        for (int q = 0; q < 3; ++q) f[q][o] = (T)BOGUS;
+    }
+    for (size_t o = P; o < (size_t)d.field; ++o) f[3][o] = (T)0.03;
+    for (int id = 0; id < LG_NFIELDS; ++id)
+        LG_HIP(hipMemsetAsync(mem(id, m), 0, d.field * sizeof(T), stream));
+    const int ids[4] = {LG_U, LG_V, LG_W, LG_CS_OPT2};
+    for (int q = 0; q < 4; ++q)
+        LG_HIP(hipMemcpyAsync(mem(ids[q], m), f[q].data(), d.field * sizeof(T),
+                              hipMemcpyHostToDevice, stream));
+    LG_HIP(hipStreamSynchronize(stream));
+}
+
+// Bytes the constructor allocates for a batch of `members` (device arrays
+// only; plan tables and filter kernels add a few kB).
+size_t case_bytes(const lg_params& p, int members, int tsize) {
+    const Dims d(p);
+    const size_t M = (size_t)members, T = (size_t)tsize;
+    const size_t msp = (size_t)(d.nz + 1) * d.splane, mbp = (size_t)(d.nz + 1) * d.bplane;
+    const size_t mbsp = (size_t)(d.nz + 1) * d.bsplane;
+    const size_t chunk = (size_t)std::min(d.nz, 64);
+    size_t b = 0;
+    b += (size_t)(LG_NFIELDS + 9 + 4) * M * d.field * T;        // fields, tmp, Ftmp
+    b += (size_t)(8 + 3) * M * msp * 2 * T;                      // spec, sU/sV/sW
+    b += 9 * M * mbp * T + 6 * M * mbsp * 2 * T;                 // big, bigspec
+    b += M * msp * (sizeof(double) + sizeof(double2));           // gam, usc
+    b += (15 + 42) * M * chunk * d.plane * T + 21 * M * chunk * d.splane * 2 * T;
+    b += 3 * M * sizeof(double) + 1024 * sizeof(double) + 2 * d.splane * T;
+    return b;
+}
+
 template <typename T>
 void Case<T>::set_member_params(int m, double zo_m, double mpfx, double mpfy) {
     zo[m] = zo_m;
@@ -380,6 +453,24 @@ void Case<T>::copy_member(int src, int dst) {
         mturb[dst] = mturb[src];
         rebuild_turbines();
     }
+}
+
+template <typename T>
+void Case<T>::retire_member(int slot) {
+    const int last = Ma - 1;
+    const bool disks = mturb[slot].n() > 0 || mturb[last].n() > 0;
+    if (disks) pull_turbine_state();
+    if (slot != last) {
+        for (int id = 0; id < LG_NFIELDS; ++id)
+            LG_HIP(hipMemcpyAsync(mem(id, slot), mem(id, last), d.field * sizeof(T),
+                                  hipMemcpyDeviceToDevice, stream));
+        set_member_params(slot, zo[last], mpf[2 * last], mpf[2 * last + 1]);
+        mturb[slot] = mturb[last];
+    }
+    mturb[last] = MemberTurbines{};
+    Ma = last;
+    if (disks) rebuild_turbines();
+    LG_HIP(hipStreamSynchronize(stream));
 }
 
 // ---------------------------------------------------------------------------

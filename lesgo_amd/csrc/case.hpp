@@ -170,8 +170,14 @@ struct Case {
     void fill_level(int id, int k, double v);
     void copy_member(int src, int dst);
     void set_member_params(int m, double zo_m, double mpfx, double mpfy);
+    void seed_member(int m, long seed, double noise);
+    // Remove member `slot`: the last stepped member moves into it.
+    void retire_member(int slot);
     void check_device_error();
 };
+
+// Device bytes a batch of `members` allocates (tsize = 8 or 4).
+size_t case_bytes(const lg_params& p, int members, int tsize);
 
 // Time every launch choice of c's passes and record the fastest in the
 // tuning table; returns the number of passes tuned.
