@@ -3,9 +3,13 @@
 Workload (default): wind-farm LES, 256 x 256 x 192 grid on Lx = 2 pi,
 Ly = pi, Lz = 1, 4 x 4 aligned actuator-disk array (D = 0.1, hub 0.1,
 7D x 5D, Ct' = 1.33), Lagrangian scale-dependent SGS, equilibrium wall
-model z0 = 1e-4, dt = 1e-4, seeded log-law start (seed 7).  Both the plain
-FP64 step and the mixed-precision step are timed; `value` is the mixed step's
-grid-pts * steps / s, the FP64 figure and the ratio are reported beside it.
+model z0 = 1e-4, dt = 1e-4, seeded log-law start (seed 7) run as a fresh
+start (Euler first step, Smagorinsky Cs^2 = 0.03 until DYN_init = 100, then
+the dynamic model every 5 steps).  The first DYN_init steps are run before
+the warm-up (untimed), so the timed steps carry the dynamic model at its
+cadence.  Both the plain FP64 step and the mixed-precision step are timed;
+`value` is the mixed step's grid-pts * steps / s, the FP64 figure and the
+ratio are reported beside it.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs one case
 per GPU with no sharding.
@@ -39,11 +43,9 @@ def log(msg):
 def make_case(name, precision):
     import lesgo_amd
     nx, ny, Nz, dt, seed, turb = WORKLOADS[name]
-    cfg = lesgo_amd.Config(nx, ny, Nz, dt=dt, precision=precision)
+    cfg = lesgo_amd.Config(nx, ny, Nz, dt=dt, precision=precision, initu=False)
     c = lesgo_amd.Case(cfg)
-    for n, a in lesgo_amd.seeded_state(cfg, seed).items():
-        c.upload(n, a)
-    c.set_counters(0, 0)
+    c.seed_state(seed)
     if turb:
         c.turbines_init(lesgo_amd.aligned_farm(cfg))
     return cfg, c
@@ -52,6 +54,7 @@ def make_case(name, precision):
 def time_precision(name, precision, steps, warmup):
     cfg, c = make_case(name, precision)
     npts = cfg.nx * cfg.ny * cfg.Nz
+    c.step_graph(cfg.dyn_init)      # spin-up to the dynamic model (untimed)
     c.step(warmup)
     c.synchronize()
     c.timing(True)
@@ -63,12 +66,16 @@ def time_precision(name, precision, steps, warmup):
     rep = c.timing_report()
     c.timing(False)
     rms = c.rmsdiv()
+    ke = c.energy()
     c.close()
+    if not (ke == ke and ke < 1e6):
+        raise RuntimeError(f"{name} {precision}: the run diverged (energy {ke})")
     dt = t1 - t0
     res = {
         "value": npts * steps / dt / 1e6,
         "ms_per_step": 1e3 * dt / steps,
         "rmsdiv": rms,
+        "energy": ke,
         "kernels": {k: {"ms_total": v[0], "launches": v[1], "gbytes": v[2] / 1e9}
                     for k, v in rep.items()},
     }
@@ -108,7 +115,8 @@ def ensemble_members(n, rng_seed=0):
 def ensemble_case(precision, members):
     import lesgo_amd
     E = ENSEMBLE
-    cfg = lesgo_amd.Config(E["nx"], E["ny"], E["Nz"], dt=E["dt"], precision=precision)
+    cfg = lesgo_amd.Config(E["nx"], E["ny"], E["Nz"], dt=E["dt"], precision=precision,
+                           initu=False)
     c = lesgo_amd.Case(cfg, members=len(members))
     farm = lesgo_amd.aligned_farm(cfg, 2, 2, dia=E["dia"], hub=E["hub"], sx_d=7.0, sy_d=5.0,
                                   ct_prime=E["ct_prime"])

@@ -66,6 +66,14 @@ typedef struct {
     double mean_p_force_x, mean_p_force_y;
     double nu;
     int precision;          /* LG_FP64 or LG_MIXED */
+    /* initu (param.f90:171): 1 = the state comes from a file (initial.f90:
+     * 100-102), the dynamic model runs every cs_count steps from the start;
+     * 0 = fresh start (inilag): Euler first step (main.f90:273-280),
+     * Cs_opt2 = 0.03 until DYN_init, F_* initialised from MM, NN at the first
+     * dynamic call (sgs_stag_util.f90:187-193, lagrange_Sdep.f90:266-337).
+     * Counted per member from its own first step. */
+    int initu;
+    int dyn_init;           /* DYN_init (param.f90:123, default 100) */
 } lg_params;
 
 typedef struct lg_case lg_case;

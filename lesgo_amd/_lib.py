@@ -28,7 +28,8 @@ class LgParams(C.Structure):
                 ("dt", C.c_double), ("zo", C.c_double), ("vonk", C.c_double),
                 ("cs_count", C.c_int), ("use_mean_p_force", C.c_int),
                 ("mean_p_force_x", C.c_double), ("mean_p_force_y", C.c_double),
-                ("nu", C.c_double), ("precision", C.c_int)]
+                ("nu", C.c_double), ("precision", C.c_int), ("initu", C.c_int),
+                ("dyn_init", C.c_int)]
 
 
 # name -> (restype, argtypes)

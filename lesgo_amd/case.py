@@ -42,13 +42,18 @@ class Config:
     mean_p_force_y: float = 0.0
     nu: float = 0.0
     precision: str = "fp64"          # "fp64" | "mixed"
+    # initu: True = state from a file (dynamic model from the start); False =
+    # fresh start (Euler first step, Cs_opt2 = 0.03 until dyn_init steps)
+    initu: bool = True
+    dyn_init: int = 100
 
     def to_c(self) -> LgParams:
         return LgParams(self.nx, self.ny, self.Nz, self.lx, self.ly, self.lz,
                         self.dt, self.zo, self.vonk, self.cs_count,
                         int(self.use_mean_p_force), self.mean_p_force_x,
                         self.mean_p_force_y, self.nu,
-                        LG_FP64 if self.precision == "fp64" else LG_MIXED)
+                        LG_FP64 if self.precision == "fp64" else LG_MIXED,
+                        int(self.initu), self.dyn_init)
 
 
 def _fid(f) -> int:

@@ -194,6 +194,7 @@ int lg_set_counters(lg_case* h, int jt, int jt_total) {
     return dispatch(h, [&](auto& c) {
         c.jt = jt;
         c.jt_total = jt_total;
+        for (int m = 0; m < c.M; ++m) c.mjt[m] = jt;
     });
 }
 
@@ -267,9 +268,24 @@ int lg_ddz_w(lg_case* h, int f, int dfdz) {
 }
 
 int lg_wallstress(lg_case* h) { return dispatch(h, [&](auto& c) { c.wallstress(); }); }
-int lg_sgs_stag(lg_case* h) { return dispatch(h, [&](auto& c) { c.sgs_stag(); }); }
-int lg_lagrange_sdep(lg_case* h) { return dispatch(h, [&](auto& c) { c.lagrange_sdep(); }); }
-int lg_interpolag_sdep(lg_case* h) { return dispatch(h, [&](auto& c) { c.interpolag_sdep(); }); }
+int lg_sgs_stag(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        c.refresh_flags();
+        c.sgs_stag();
+    });
+}
+int lg_lagrange_sdep(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        c.refresh_flags(true);
+        c.lagrange_sdep();
+    });
+}
+int lg_interpolag_sdep(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        c.refresh_flags(true);
+        c.interpolag_sdep();
+    });
+}
 int lg_divstress_uv(lg_case* h) { return dispatch(h, [&](auto& c) { c.divstress_uv(); }); }
 int lg_divstress_w(lg_case* h) { return dispatch(h, [&](auto& c) { c.divstress_w(); }); }
 int lg_convec(lg_case* h) { return dispatch(h, [&](auto& c) { c.convec(); }); }
@@ -294,7 +310,12 @@ int lg_copy_field(lg_case* h, int src, int dst) {
     });
 }
 
-int lg_rhs_ab2(lg_case* h) { return dispatch(h, [&](auto& c) { c.rhs_ab2(); }); }
+int lg_rhs_ab2(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        c.refresh_flags();
+        c.rhs_ab2();
+    });
+}
 int lg_rhs_gradp(lg_case* h) { return dispatch(h, [&](auto& c) { c.rhs_gradp(); }); }
 
 int lg_step(lg_case* h, int nsteps) {

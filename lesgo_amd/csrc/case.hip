@@ -155,6 +155,11 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     mpf.resize(2 * M);
     for (int m = 0; m < M; ++m) set_member_params(m, p.zo, p.mean_p_force_x, p.mean_p_force_y);
     mturb.resize(M);
+    mjt.assign(M, 0);
+    lm_init.assign(M, 0);
+    hflags.assign(3 * M, 0);
+    dflags_now.assign(3 * M, 0);
+    d_flags = dalloc<int>(3L * M);
     px.build(d.nx);
     py.build(d.ny);
     pbx.build(d.nx2);
@@ -209,6 +214,7 @@ Case<T>::~Case() {
     fr(red);
     fr(d_denom);
     fr(d_mpf);
+    fr(d_flags);
     fr(G1);
     fr(G2);
     if (stream) (void)hipStreamDestroy(stream);
@@ -266,6 +272,8 @@ void Case<T>::seed_member(int m, long seed, double noise) {
         LG_HIP(hipMemcpyAsync(mem(ids[q], m), f[q].data(), d.field * sizeof(T),
                               hipMemcpyHostToDevice, stream));
     LG_HIP(hipStreamSynchronize(stream));
+    mjt[m] = 0;
+    lm_init[m] = 0;
 }
 
 // Bytes the constructor allocates for a batch of `members` (device arrays
@@ -448,6 +456,8 @@ void Case<T>::copy_member(int src, int dst) {
         LG_HIP(hipMemcpyAsync(mem(id, dst), mem(id, src), d.field * sizeof(T),
                               hipMemcpyDeviceToDevice, stream));
     set_member_params(dst, zo[src], mpf[2 * src], mpf[2 * src + 1]);
+    mjt[dst] = mjt[src];
+    lm_init[dst] = lm_init[src];
     if (turb || mturb[dst].n() || mturb[src].n()) {
         pull_turbine_state();
         mturb[dst] = mturb[src];
@@ -466,6 +476,8 @@ void Case<T>::retire_member(int slot) {
                                   hipMemcpyDeviceToDevice, stream));
         set_member_params(slot, zo[last], mpf[2 * last], mpf[2 * last + 1]);
         mturb[slot] = mturb[last];
+        mjt[slot] = mjt[last];
+        lm_init[slot] = lm_init[last];
     }
     mturb[last] = MemberTurbines{};
     Ma = last;
@@ -587,7 +599,15 @@ void Case<T>::calc_sij() {
 template <typename T>
 void Case<T>::sgs_stag() {
     calc_sij();
-    if (jt_total % d.prm.cs_count == 0) lagrange_sdep();
+    // Cs_opt2 = 0.03 at a fresh member's first step (sgs_stag_util.f90:187-189)
+    bool any_cs = false;
+    for (int m = 0; m < Ma; ++m) any_cs |= hflags[M + m] != 0;
+    if (any_cs)
+        hipLaunchKernelGGL(k_fill_members_if<T>,
+                           dim3((unsigned)((d.field + BLK - 1) / BLK), Ma), dim3(BLK), 0, stream,
+                           fld[LG_CS_OPT2], (const int*)(d_flags + M), d.field, d.field,
+                           (T)0.03);
+    if (lag_now) lagrange_sdep();
     TauArgs<T> a{fld[LG_S11], fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33],
                  fld[LG_CS_OPT2], fld[LG_NU_T], fld[LG_TXX], fld[LG_TXY], fld[LG_TYY],
                  fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ], (T)(d.delta * d.delta), (T)d.prm.nu};
@@ -609,7 +629,8 @@ void Case<T>::interpolag_sdep() {
                 d.prm.lx, d.prm.ly, d.prm.lz, d.lagran_dt, d.field};
     hipLaunchKernelGGL(k_interpolag<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
                        fld[LG_U], fld[LG_V], fld[LG_W], Ftmp, Ftmp + Fq, Ftmp + 2 * Fq,
-                       Ftmp + 3 * Fq, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN], fld[LG_F_NN], g);
+                       Ftmp + 3 * Fq, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN], fld[LG_F_NN], g,
+                       (const int*)(d_flags + 2 * M));
 }
 
 template <typename T>
@@ -669,8 +690,10 @@ void Case<T>::lagrange_sdep() {
         hipLaunchKernelGGL(k_lag_update<T>, grid_ij(d.ld, d.ny, nc, Ma), dim3(BLK), 0, stream,
                            (const T*)lag_filt, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN],
                            fld[LG_F_NN], fld[LG_CS_OPT2], fld[LG_BETA], fld[LG_TN_ALL], g, k0, nc,
-                           Ma, up);
+                           Ma, up, (const int*)(d_flags + 2 * M));
     }
+    for (int m = 0; m < Ma; ++m)
+        if (hflags[2 * M + m] == 2) lm_init[m] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -865,7 +888,7 @@ void Case<T>::rhs_ab2() {
                  fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F], fld[LG_DIVTX], fld[LG_DIVTY],
                  fld[LG_DIVTZ], tb ? fld[LG_FXA] : nullptr, tb ? fld[LG_FYA] : nullptr,
                  tb ? fld[LG_FZA] : nullptr, (T)d.prm.dt, (T)d.tadv1, (T)d.tadv2, d_mpf,
-                 d.prm.use_mean_p_force};
+                 (const int*)d_flags, d.prm.use_mean_p_force};
     hipLaunchKernelGGL(k_rhs_ab2<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
                        GEO(1, d.nz));
     for (int id : {LG_U, LG_V, LG_W}) fill_level(id, 0, BOGUS);
@@ -910,10 +933,53 @@ void Case<T>::step_body() {
         swapped = !swapped;                           \
     } while (0)
 
+// Per-member flags of the step the counters describe: Euler update and
+// Cs_opt2 = 0.03 at a fresh member's first step; dynamic-model mode (0 not
+// yet, 1 update, 2 first call with F_* initialisation) when jt_total is a
+// multiple of cs_count.  Uploads them if they changed (this synchronises the
+// stream, so it never happens inside a capture: steps that change flags are
+// eager).  Returns true when the step has a first-step or initialisation
+// event.  force_lag: a direct lagrange_Sdep call, which updates every member.
+template <typename T>
+bool Case<T>::refresh_flags(bool force_lag) {
+    const int cs = d.prm.cs_count;
+    const bool dyn_step = jt_total % cs == 0;
+    bool special = false, any_lag = false;
+    std::fill(hflags.begin(), hflags.end(), 0);
+    for (int m = 0; m < Ma; ++m) {
+        int eu = 0, ci = 0, lm = 0;
+        const int j = mjt[m];
+        if (d.prm.initu) {
+            lm = (dyn_step || force_lag) ? 1 : 0;
+        } else {
+            eu = ci = (j == 1) ? 1 : 0;
+            if ((!ci && j >= d.prm.dyn_init && dyn_step) || force_lag) {
+                lm = 1;
+                if (!lm_init[m] && (j == cs || j == d.prm.dyn_init)) lm = 2;
+            }
+        }
+        hflags[m] = eu;
+        hflags[M + m] = ci;
+        hflags[2 * M + m] = lm;
+        special |= eu || ci || lm == 2;
+        any_lag |= lm > 0;
+    }
+    lag_now = any_lag;
+    if (hflags != dflags_now) {
+        LG_HIP(hipStreamSynchronize(stream));
+        LG_HIP(hipMemcpy(d_flags, hflags.data(), hflags.size() * sizeof(int),
+                         hipMemcpyHostToDevice));
+        dflags_now = hflags;
+    }
+    return special;
+}
+
 template <typename T>
 void Case<T>::step() {
     ++jt;
     ++jt_total;
+    for (int m = 0; m < Ma; ++m) ++mjt[m];
+    refresh_flags();
     LG_SWAP_RHS();
     step_body();
 }
@@ -930,13 +996,20 @@ void Case<T>::drop_graphs() {
 // One step replayed from a captured graph.  A step's launches depend on the
 // RHS storage parity and on whether the dynamic model runs, so up to four
 // graphs are captured, each on first use (and again after the member count
-// or the disk set changes).
+// or the disk set changes).  A member's first step and its first dynamic
+// call run eagerly.
 template <typename T>
 void Case<T>::step_graph() {
     ++jt;
     ++jt_total;
+    for (int m = 0; m < Ma; ++m) ++mjt[m];
+    const bool special = refresh_flags();
     LG_SWAP_RHS();
-    const int key = (swapped ? 1 : 0) + ((jt_total % d.prm.cs_count == 0) ? 2 : 0);
+    if (special) {
+        step_body();
+        return;
+    }
+    const int key = (swapped ? 1 : 0) + (lag_now ? 2 : 0);
     if (!graphs[key] || graph_ma[key] != Ma) {
         if (graphs[key]) (void)hipGraphExecDestroy(graphs[key]);
         graphs[key] = nullptr;

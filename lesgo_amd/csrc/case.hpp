@@ -83,6 +83,13 @@ struct Case {
     double* d_denom = nullptr;       // per member ln(0.5 dz / z0)
     double* d_mpf = nullptr;         // per member mean_p_force_x, _y
     std::vector<MemberTurbines> mturb;
+    // fresh-start bookkeeping per member (lg_params.initu = 0)
+    std::vector<int> mjt;            // steps the member has taken (its jt)
+    std::vector<char> lm_init;       // F_* initialised by the first dynamic call
+    std::vector<int> hflags;         // euler[M], csinit[M], lagmode[M] of this step
+    std::vector<int> dflags_now;     // the copy on the device
+    int* d_flags = nullptr;
+    bool lag_now = false;            // the dynamic model runs this step
     hipGraphExec_t graphs[4] = {};   // captured steps keyed by (swapped, dynamic model)
     int graph_ma[4] = {};
 
@@ -156,6 +163,7 @@ struct Case {
     void project();
     void rhs_ab2();
     void rhs_gradp();
+    bool refresh_flags(bool force_lag = false);
     void step_body();
     void step();
     void step_graph();

@@ -80,10 +80,11 @@ __device__ double trilinear_w(const T* var, const InterpGeo& g, double px, doubl
 template <typename T>
 __global__ void k_interpolag(const T* u, const T* v, const T* w, const T* tLM, const T* tMM,
                              const T* tQN, const T* tNN, T* LM, T* MM, T* QN, T* NN,
-                             InterpGeo g) {
+                             InterpGeo g, const int* mode) {
     int j, i;
     if (!ij_of(g.nx, g.ny, j, i)) return;
     const int m = blockIdx.y / g.nz;
+    if (mode[m] == 0) return;      // member not yet dynamic
     const int k = blockIdx.y - m * g.nz + 1;
     const long mb = (long)m * g.mstride;
     u += mb;
@@ -176,13 +177,20 @@ struct LagUpd {
     double opftdelta, powcoeff, cnst, tf1_2, tf2_2, ldt, zero, expo, betamin;
 };
 
+// mode[m]: 0 = member not yet dynamic (untouched), 1 = update, 2 = first
+// dynamic call of a fresh start: F_MM = MM, F_LM = 0.03 MM, F_NN = NN,
+// F_QN = 0.03 NN before the update (lagrange_Sdep.f90:266-277, 326-337; the
+// padding columns of F_* are not kept, nothing reads them).
 template <typename T>
 __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, T* Cs,
-                             T* Beta, T* Tn_all, Geo g, int k0, int nc, int M, LagUpd p) {
+                             T* Beta, T* Tn_all, Geo g, int k0, int nc, int M, LagUpd p,
+                             const int* mode) {
     int j, i;
     if (!ij_of(g.ld, g.ny, j, i)) return;
     const int m = blockIdx.y / nc;
     const int c = blockIdx.y - m * nc;
+    const int md = mode[m];
+    if (md == 0) return;
     const int k = k0 + c;
     const long P = g.plane;
     const long o = (long)m * g.mstride + (long)k * P + (long)j * g.ld + i;
@@ -227,6 +235,13 @@ __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, 
     const double zero = p.zero;
     // 2-delta running averages (lagrange_Sdep.f90:297-315)
     double fLM = F_LM[o], fMM = F_MM[o];
+    double fQN = F_QN[o], fNN = F_NN[o];
+    if (md == 2) {
+        fMM = MM;
+        fLM = 0.03 * MM;
+        fNN = NN;
+        fQN = 0.03 * NN;
+    }
     double Tn = fmax(fLM * fMM, zero);
     Tn = p.opftdelta * pow(Tn, p.powcoeff);
     Tn = fmax(zero, Tn);
@@ -237,7 +252,6 @@ __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, 
     fLM = fmax(zero, fLM);
     const double cs2d = fmax(zero, fLM / (fMM + zero));
     // 4-delta running averages (lagrange_Sdep.f90:333-352)
-    double fQN = F_QN[o], fNN = F_NN[o];
     Tn = fmax(fQN * fNN, zero);
     Tn = p.opftdelta * pow(Tn, p.powcoeff);
     Tn = fmax(zero, Tn);

@@ -40,6 +40,14 @@ __global__ void k_fill_planes(T* f, long mstride, long n, T v) {
     if (idx < n) f[(long)blockIdx.y * mstride + idx] = v;
 }
 
+// f[m * mstride + q] = v for q < n on the members with flag[m] != 0
+template <typename T>
+__global__ void k_fill_members_if(T* f, const int* flag, long mstride, long n, T v) {
+    if (!flag[blockIdx.y]) return;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) f[(long)blockIdx.y * mstride + idx] = v;
+}
+
 template <typename T>
 __global__ void k_copy_plane(const T* src, T* dst, long mstride, long n) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -352,13 +360,17 @@ __global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz, lon
     a.cz[o] = z;
 }
 
-// RHS assembly, forcing and AB2 (main.f90:211-214, 228-232, 264-266, 287-296).
+// RHS assembly, forcing and AB2 (main.f90:211-214, 228-232, 264-266, 287-296);
+// members with euler[m] take the first-step Euler update with RHS_f = RHS
+// (main.f90:273-280).
 template <typename T>
 struct Ab2Args {
     T *u, *v, *w, *rx, *ry, *rz;
-    const T *rxf, *ryf, *rzf, *dx, *dy, *dz, *fx, *fy, *fz;
+    T *rxf, *ryf, *rzf;
+    const T *dx, *dy, *dz, *fx, *fy, *fz;
     T dt, t1, t2;
     const double* mpf;     // 2 per member: mean_p_force_x, mean_p_force_y
+    const int* euler;      // per member
     int use_mpf;
 };
 
@@ -384,13 +396,31 @@ __global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
         a.rx[o] = rx;
         a.ry[o] = ry;
         a.rz[o] = rz;
-        a.u[o] = a.u[o] + a.dt * (a.t1 * rx + a.t2 * a.rxf[o]);
-        a.v[o] = a.v[o] + a.dt * (a.t1 * ry + a.t2 * a.ryf[o]);
-        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * a.rzf[o]);
+        T fx = rx, fy = ry, fz = rz;
+        if (a.euler[m]) {
+            a.rxf[o] = rx;
+            a.ryf[o] = ry;
+            a.rzf[o] = rz;
+        } else {
+            fx = a.rxf[o];
+            fy = a.ryf[o];
+            fz = a.rzf[o];
+        }
+        a.u[o] = a.u[o] + a.dt * (a.t1 * rx + a.t2 * fx);
+        a.v[o] = a.v[o] + a.dt * (a.t1 * ry + a.t2 * fy);
+        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * fz);
     } else {
         const T rz = -a.rz[o] - a.dz[o];
         a.rz[o] = rz;
-        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * a.rzf[o]);
+        T fz = rz;
+        if (a.euler[m]) {
+            a.rzf[o] = rz;
+            a.rxf[o] = a.rx[o];
+            a.ryf[o] = a.ry[o];
+        } else {
+            fz = a.rzf[o];
+        }
+        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * fz);
     }
 }
 

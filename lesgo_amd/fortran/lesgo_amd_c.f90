@@ -31,6 +31,8 @@ type, bind(C) :: lg_params
     real(c_double) :: mean_p_force_x, mean_p_force_y
     real(c_double) :: nu
     integer(c_int) :: precision
+    integer(c_int) :: initu
+    integer(c_int) :: dyn_init
 end type lg_params
 
 interface

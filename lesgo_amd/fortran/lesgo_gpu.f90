@@ -42,6 +42,8 @@ prm%use_mean_p_force = merge(1, 0, use_mean_p_force)
 prm%mean_p_force_x = mean_p_force_x; prm%mean_p_force_y = mean_p_force_y
 prm%nu = 0._rprec
 prm%precision = precision
+prm%initu = merge(1, 0, initu)
+prm%dyn_init = DYN_init
 call gpu_check(lg_create(prm, gpu_case), 'lg_create')
 end subroutine gpu_init
 

@@ -24,6 +24,9 @@ real(rprec) :: L_x = 2._rprec*pi, L_y = pi, L_z = 1._rprec
 real(rprec) :: dx, dy, dz
 real(rprec) :: dt = 2.0e-4_rprec, zo = 1.0e-4_rprec, vonk = 0.4_rprec
 integer :: cs_count = 5
+integer :: DYN_init = 100
+! the seeded start stands in for a vel.out file (initial.f90:100-102)
+logical :: initu = .true.
 logical :: use_mean_p_force = .true.
 real(rprec) :: mean_p_force_x = 1._rprec, mean_p_force_y = 0._rprec
 real(rprec) :: tadv1 = 1.5_rprec, tadv2 = -0.5_rprec

@@ -18,7 +18,7 @@ use sgs_stag_util, only : sgs_stag
 use forcing, only : forcing_applied, project, turbines_on
 implicit none
 namelist /lesgo/ Nx, Ny, Nz, L_x, L_y, L_z, dt, zo, cs_count, mean_p_force_x, &
-    nsteps, wbase, domain_nskip, precision, seed, fused, path, turbines_on
+    nsteps, wbase, domain_nskip, precision, seed, fused, path, turbines_on, initu, DYN_init
 character(len=512) :: nml
 integer :: jt_step, fid, ios, count0, count1, rate
 real(rprec) :: rms, ke, tw

@@ -63,6 +63,12 @@ class Params:
     mean_p_force_x: float = 1.0
     mean_p_force_y: float = 0.0
     nu: float = 0.0                # molec = .false. -> nu = 0 (sgs_param.f90:187-192)
+    # initu = .true.: the state comes from a file (initial.f90:100-102) and the
+    # dynamic model runs from the first cs_count step; .false.: a fresh start
+    # (inilag = .true.): Euler first step, Cs_opt2 = 0.03 until DYN_init, F_*
+    # initialised from MM, NN at the first dynamic call (param.f90:123,171-173)
+    initu: bool = True
+    dyn_init: int = 100
 
     def __post_init__(self):
         assert self.nx % 2 == 0 and self.ny % 2 == 0
@@ -272,6 +278,8 @@ class State:
     jt_total: int = 0
     turbines: "Turbines | None" = None
     ustar_lbc: np.ndarray | None = None
+    lm_init: bool = False          # F_LM_MM_init (lagrange_Sdep.f90:266-277)
+    qn_init: bool = False          # F_QN_NN_init (lagrange_Sdep.f90:326-337)
 
     @classmethod
     def zeros(cls, p: Params):
@@ -288,7 +296,7 @@ class State:
 
     def copy(self):
         s = State(self.p, {k: v.copy() for k, v in self.f.items()}, self.jt,
-                  self.jt_total, self.turbines, None)
+                  self.jt_total, self.turbines, None, self.lm_init, self.qn_init)
         if self.turbines is not None:
             s.turbines = self.turbines.copy()
         return s
@@ -438,8 +446,9 @@ def smag_norm(S11, S12, S13, S22, S23, S33):
 def sgs_stag(sp: Spectral, s: State):
     """sgs_stag (sgs_stag_util.f90:33-464) for sgs_model = 5, one rank.
 
-    calc_Sij; lagran_dt = cs_count dt; lagrange_Sdep when (jt >= DYN_init or
-    initu) and mod(jt_total, cs_count) == 0 (initu is true for the file IC);
+    calc_Sij; lagran_dt = cs_count dt; Cs_opt2 = 0.03 at jt = 1 of a fresh
+    start (inilag, sgs_stag_util.f90:187-189); lagrange_Sdep when (jt >=
+    DYN_init or initu) and mod(jt_total, cs_count) == 0 (192-193);
     Nu_t = |S| Cs^2 delta^2; tau_ij with the wall row (lbc 1:) at k=1 and the
     stress-free top row at nz-1; BOGUS at 0 and nz.
     Pinned by: tests/test_oracle_ops.py::test_sgs_constant_cs.
@@ -447,7 +456,9 @@ def sgs_stag(sp: Spectral, s: State):
     p = sp.p
     nx, nz = p.nx, p.nz
     calc_Sij(p, s)
-    if s.jt_total % p.cs_count == 0:
+    if s.jt == 1 and not p.initu:
+        s.Cs_opt2[:] = 0.03
+    elif (s.jt >= p.dyn_init or p.initu) and s.jt_total % p.cs_count == 0:
         lagrange_Sdep(sp, s)
     X = np.s_[:, :nx]
     l2 = p.delta ** 2
@@ -590,9 +601,11 @@ def interpolag_Sdep(p: Params, s: State):
 # lagrange_Sdep.f90
 # --------------------------------------------------------------------------
 def lagrange_Sdep(sp: Spectral, s: State):
-    """lagrange_Sdep (lagrange_Sdep.f90:21-432) with inilag = .false.,
-    inflow_type = 0, no PPDYN_TN: test-filtered L,Q,M,N at 2 and 4 delta,
-    Lagrangian relaxation of F_LM/MM/QN/NN, beta, Cs_opt2.  All levels
+    """lagrange_Sdep (lagrange_Sdep.f90:21-432), inflow_type = 0, no
+    PPDYN_TN: test-filtered L,Q,M,N at 2 and 4 delta, Lagrangian relaxation
+    of F_LM/MM/QN/NN, beta, Cs_opt2.  On a fresh start (inilag) the first
+    call at jt = cs_count or DYN_init sets F_MM = MM, F_LM = 0.03 MM (and
+    F_NN, F_QN alike), padding columns 1 (266-277, 326-337).  All levels
     1..nz are processed at once (each level's work is independent).
     Parity unpinned by an external vector (no reference run exists);
     pinned by invariants in tests/test_oracle_ops.py::test_lagrange_*."""
@@ -651,6 +664,19 @@ def lagrange_Sdep(sp: Spectral, s: State):
 
     LM, MM, QN, NN = contract(L, M), sq(M), contract(Q, N), sq(N)
     F_LM, F_MM, F_QN, F_NN = (s.F_LM[K], s.F_MM[K], s.F_QN[K], s.F_NN[K])
+    first = (not p.initu) and (s.jt == p.cs_count or s.jt == p.dyn_init)
+    if first and not s.lm_init:
+        F_MM = MM.copy()
+        F_LM = 0.03 * MM
+        F_MM[:, :, p.ld - 2:] = 1.0
+        F_LM[:, :, p.ld - 2:] = 1.0
+        s.lm_init = True
+    if first and not s.qn_init:
+        F_NN = NN.copy()
+        F_QN = 0.03 * NN
+        F_NN[:, :, p.ld - 2:] = 1.0
+        F_QN[:, :, p.ld - 2:] = 1.0
+        s.qn_init = True
     zero = ZERO_LAG
     Tn = np.maximum(F_LM * F_MM, zero)
     Tn = opftdelta * (Tn ** powcoeff)
@@ -1203,6 +1229,11 @@ def step(sp: Spectral, s: State):
         s.RHSx[K] = s.RHSx[K] + s.fxa[K]
         s.RHSy[K] = s.RHSy[K] + s.fya[K]
         s.RHSz[K] = s.RHSz[K] + s.fza[K]
+    if s.jt_total == 1 and not p.initu:
+        # Euler first step of a fresh start (main.f90:273-280)
+        s.RHSx_f[:] = s.RHSx
+        s.RHSy_f[:] = s.RHSy
+        s.RHSz_f[:] = s.RHSz
     for n in ("x", "y", "z"):
         vel = {"x": s.u, "y": s.v, "z": s.w}[n]
         R, Rf = s.f["RHS" + n], s.f["RHS" + n + "_f"]

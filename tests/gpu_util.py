@@ -21,7 +21,8 @@ def gpu_case(p, precision="fp64"):
                            vonk=p.vonk, cs_count=p.cs_count,
                            use_mean_p_force=p.use_mean_p_force,
                            mean_p_force_x=p.mean_p_force_x,
-                           mean_p_force_y=p.mean_p_force_y, nu=p.nu, precision=precision)
+                           mean_p_force_y=p.mean_p_force_y, nu=p.nu, precision=precision,
+                           initu=p.initu, dyn_init=p.dyn_init)
     return lesgo_amd.Case(cfg)
 
 

@@ -100,10 +100,11 @@ def test_batch_member_matches_oracle():
             assert e <= 1e-10, (n, e)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_replay_equals_eager(precision):
+@pytest.mark.parametrize("precision,fresh", [("fp64", False), ("mixed", False),
+                                             ("mixed", True)])
+def test_graph_replay_equals_eager(precision, fresh):
     import lesgo_amd
-    cfg = cfg_of(precision=precision)
+    cfg = cfg_of(precision=precision, initu=not fresh, dyn_init=5)
     nsteps = 12       # all four (parity, dynamic-model) graphs, replayed
     with lesgo_amd.Case(cfg, members=2) as a, lesgo_amd.Case(cfg, members=2) as b:
         for c in (a, b):

@@ -19,9 +19,12 @@ JOBS = [
 ]
 
 
-def cfg_of(grid, zo):
+def cfg_of(grid, zo, fresh=False):
+    """fresh: the fresh-start schedule with DYN_init = 10, so jobs admitted
+    at different ticks sit in one pool at different points of it (first
+    Euler step, Smagorinsky phase, F_* initialisation, dynamic)."""
     import lesgo_amd
-    return lesgo_amd.Config(*grid, zo=zo, precision="mixed")
+    return lesgo_amd.Config(*grid, zo=zo, precision="mixed", initu=not fresh, dyn_init=10)
 
 
 def disks_of(cfg, on):
@@ -31,9 +34,9 @@ def disks_of(cfg, on):
     return lesgo_amd.aligned_farm(cfg, 2, 2, dia=0.3, hub=0.3, sx_d=7, sy_d=5)
 
 
-def alone(grid, seed, zo, nsteps, disks):
+def alone(grid, seed, zo, nsteps, disks, fresh=False):
     import lesgo_amd
-    cfg = cfg_of(grid, zo)
+    cfg = cfg_of(grid, zo, fresh)
     with lesgo_amd.Case(cfg) as c:
         c.seed_state(seed)
         if disks:
@@ -42,7 +45,7 @@ def alone(grid, seed, zo, nsteps, disks):
         return {n: c.download(n) for n in ("u", "v", "w", "p")}
 
 
-def run_server(budget, pool_members=2, max_ticks=400):
+def run_server(budget, pool_members=2, max_ticks=400, fresh=False):
     from lesgo_amd.serve import Server
     ids = {}
     with Server(budget, pool_members) as s:
@@ -50,7 +53,7 @@ def run_server(budget, pool_members=2, max_ticks=400):
         while True:
             for k, (grid, seed, zo, nsteps, disks, at) in enumerate(JOBS):
                 if at == tick:
-                    cfg = cfg_of(grid, zo)
+                    cfg = cfg_of(grid, zo, fresh)
                     ids[k] = s.submit(cfg, seed, nsteps, disks_of(cfg, disks), keep=True)
             _, running, queued = s.tick()
             tick += 1
@@ -69,19 +72,20 @@ def run_server(budget, pool_members=2, max_ticks=400):
         return out, tick
 
 
-def check_alone(out):
+def check_alone(out, fresh=False):
     for k, (grid, seed, zo, nsteps, disks, at) in enumerate(JOBS):
-        ref = alone(grid, seed, zo, nsteps, disks)
+        ref = alone(grid, seed, zo, nsteps, disks, fresh)
         for n in ("u", "v", "w", "p"):
             d = np.max(np.abs(out[k][n] - ref[n]))
             print(f"  job {k} {grid} {n}: max |served - alone| = {d:.3e}", flush=True)
             assert d == 0.0, (k, n, d)
 
 
-def test_served_jobs_equal_alone():
-    out, ticks = run_server(budget=50e9)
+@pytest.mark.parametrize("fresh", [False, True])
+def test_served_jobs_equal_alone(fresh):
+    out, ticks = run_server(budget=50e9, fresh=fresh)
     print(f"  drained in {ticks} ticks", flush=True)
-    check_alone(out)
+    check_alone(out, fresh)
 
 
 def test_budget_limits_admission():

@@ -51,6 +51,32 @@ def test_config0_100_steps_fp64():
     assert errs[-1]["rmsdiv_gpu"] < 1e-12
 
 
+def test_fresh_start_fp64():
+    """Fresh start (initu = .false.): Euler first step, Smagorinsky phase,
+    F_* initialisation at DYN_init = 10 and dynamic steps 15, 20: u, v, w, p
+    and Cs_opt2, F_LM against the oracle at 1e-10."""
+    import lesgo_amd
+    p = o.Params(32, 32, 16, initu=False, dyn_init=10)
+    sp = o.Spectral(p)
+    s = o.initial_condition(p, 2)
+    with gpu_case(p) as c:
+        for n, a in s.f.items():
+            c.upload(n, a)
+        c.set_counters(0, 0)
+        for chunk in (1, 8, 1, 5, 6):
+            for _ in range(chunk):
+                o.step(sp, s)
+            c.step(chunk)
+            X = np.s_[1:p.nz, :, :p.nx]
+            errs = {n: rel_l2(c.download(n)[X], s.f[n][X])
+                    for n in ("u", "v", "w", "p", "Cs_opt2", "F_LM", "F_NN")}
+            print(f"  step {s.jt_total}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()),
+                  flush=True)
+            for n, e in errs.items():
+                assert e <= 1e-10, (s.jt_total, n, e)
+    assert lesgo_amd
+
+
 def test_config0_mixed_precision_error():
     """Mixed step (FP32 FFT/convection/SGS, FP64 tridiagonal) against the FP64
     oracle: velocity rel L2 <= 1e-4 after 100 steps."""
