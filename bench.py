@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mpts·steps/s at 256³-class, mixed vs FP64; ensemble p50/p99 step latency"
+# launch choices recorded by the autotuner on gfx950 for the bench workloads
+TUNE_FILE = os.path.join(ROOT, "lesgo_amd", "tuned_gfx950.json")
+TUNE_MODE = "auto"
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
@@ -51,10 +54,39 @@ def make_case(name, precision):
     return cfg, c
 
 
-def time_precision(name, precision, steps, warmup):
+def measured_traffic(name, precision, kernel):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    PMC summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on a
+    plain copy), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{name}_{precision}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) >= 4 and f[0] == "traffic_per_launch_bytes" and f[1] == precision \
+                and f[2] == kernel:
+            return float(f[3])
+    return None
+
+
+def prepare_tuning(c):
+    """auto: the recorded launch choices when the file exists, else tune this
+    case now; tune: tune now; off: the built-in heuristics."""
+    from lesgo_amd.case import tune_load
+    if TUNE_MODE == "off":
+        return "off"
+    if TUNE_MODE == "auto" and os.path.exists(TUNE_FILE):
+        tune_load(TUNE_FILE)
+        return "file"
+    c.autotune(5)
+    return "live"
+
+
+def time_precision(name, precision, steps, warmup, spinup=None):
     cfg, c = make_case(name, precision)
     npts = cfg.nx * cfg.ny * cfg.Nz
-    c.step_graph(cfg.dyn_init)      # spin-up to the dynamic model (untimed)
+    tuning = prepare_tuning(c)
+    c.step_graph(cfg.dyn_init if spinup is None else spinup)   # untimed spin-up
     c.step(warmup)
     c.synchronize()
     c.timing(True)
@@ -74,6 +106,7 @@ def time_precision(name, precision, steps, warmup):
     res = {
         "value": npts * steps / dt / 1e6,
         "ms_per_step": 1e3 * dt / steps,
+        "tuning": tuning,
         "rmsdiv": rms,
         "energy": ke,
         "kernels": {k: {"ms_total": v[0], "launches": v[1], "gbytes": v[2] / 1e9}
@@ -89,7 +122,7 @@ def time_precision(name, precision, steps, warmup):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(name, precision, dom[0]),
         "launches": nl,
         "avg_launch_us": 1e3 * ms / nl if nl else 0.0,
         "share_of_step": ms / (1e3 * dt),
@@ -141,7 +174,7 @@ def ensemble_leg(steps):
     npts = E["nx"] * E["ny"] * E["Nz"] * len(members)
     cfg, c = ensemble_case("mixed", members)
     t0 = time.perf_counter()
-    npass = c.autotune(5)
+    tuning = prepare_tuning(c)
     t_tune = time.perf_counter() - t0
     c.synchronize()
     lat = []
@@ -180,10 +213,25 @@ def ensemble_leg(steps):
         "step_ms_p50": float(np.percentile(lat_ms, 50)),
         "step_ms_p99": float(np.percentile(lat_ms, 99)),
         "fp64_value": npts * steps / dt64 / 1e6,
-        "autotune_s": t_tune,
-        "autotuned_passes": npass,
+        "tuning": tuning,
+        "tuning_s": t_tune,
         "member_vel_rel_l2_vs_fp64": {"max": max(errs), "median": float(np.median(errs))},
     }
+
+
+def serving_leg(minutes):
+    """The serving loop on a Poisson stream (2 jobs/s, grids {64^3, 96x48x64,
+    128x64x96, 128^3}, 200-2000 steps, z0 log-uniform, 30 % with a 2x2 disk
+    array, mixed precision, 200 GB budget), drained; with the time the same
+    jobs would take one after another in plain FP64."""
+    from lesgo_amd import serve
+    jobs = serve.job_stream(minutes, 2.0, seed=0)
+    res = serve.run_stream(jobs, 200.0, 16, progress=log)
+    ser = serve.fp64_serial_estimate(jobs)
+    res["fp64_serial_s"] = ser["seconds"]
+    res["fp64_serial_Mpts_steps_s"] = ser["Mpts_steps_s"]
+    res["speedup_vs_fp64_serial"] = ser["seconds"] / res["wall_s"]
+    return res
 
 
 def cpu_baseline(name):
@@ -237,6 +285,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ensemble-steps", type=int, default=ENSEMBLE["steps"],
                     help="steps of the 48-member ensemble leg (0 skips it)")
+    ap.add_argument("--serve-minutes", type=float, default=1.0,
+                    help="length of the Poisson job stream of the serving leg (0 skips it)")
+    ap.add_argument("--tune", default="auto", choices=["auto", "tune", "off"],
+                    help="launch choices: recorded file if present (auto), tune now, or "
+                         "built-in heuristics")
+    ap.add_argument("--write-tune-file", action="store_true",
+                    help="only autotune the bench workloads and record the choices")
+    ap.add_argument("--spinup", type=int, default=None,
+                    help="untimed steps before the warm-up (default: DYN_init, so the timed "
+                         "steps include the dynamic model)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -249,6 +307,21 @@ def main():
         print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms.value,
                           "copy_GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
         return
+    global TUNE_MODE
+    TUNE_MODE = args.tune
+    if args.write_tune_file:
+        from lesgo_amd.case import tune_clear, tune_save
+        tune_clear()
+        for prec in ("fp64", "mixed"):
+            _, c = make_case("windfarm256", prec)
+            c.autotune(7)
+            c.close()
+            _, c = ensemble_case(prec, ensemble_members(ENSEMBLE["members"]))
+            c.autotune(7)
+            c.close()
+        tune_save(TUNE_FILE)
+        print(json.dumps({"tune_file": TUNE_FILE}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one case per GPU with no sharding; "
                                    "run with --gpus 1 (multi-GPU scaling is not built)"}))
@@ -257,11 +330,12 @@ def main():
     out = {}
     if args.precision in ("both", "fp64"):
         log(f"[bench] fp64 {args.workload}: warmup {args.warmup}, steps {args.steps}")
-        out["fp64"] = time_precision(args.workload, "fp64", args.steps, args.warmup)
+        out["fp64"] = time_precision(args.workload, "fp64", args.steps, args.warmup, args.spinup)
         log(f"[bench] fp64 done: {out['fp64']['value']:.1f} Mpts*steps/s")
     if args.precision in ("both", "mixed"):
         log(f"[bench] mixed {args.workload}: warmup {args.warmup}, steps {args.steps}")
-        out["mixed"] = time_precision(args.workload, "mixed", args.steps, args.warmup)
+        out["mixed"] = time_precision(args.workload, "mixed", args.steps, args.warmup,
+                                      args.spinup)
         log(f"[bench] mixed done: {out['mixed']['value']:.1f} Mpts*steps/s")
     head = out.get("mixed") or out["fp64"]
     line = {
@@ -296,6 +370,11 @@ def main():
         line["ensemble"] = ensemble_leg(args.ensemble_steps)
         log(f"[bench] ensemble done: {line['ensemble']['value']:.1f} Mpts*steps/s, step p50 "
             f"{line['ensemble']['step_ms_p50']:.2f} ms p99 {line['ensemble']['step_ms_p99']:.2f} ms")
+    if args.serve_minutes > 0:
+        log(f"[bench] serving: {args.serve_minutes} min Poisson stream at 2 jobs/s")
+        line["serving"] = serving_leg(args.serve_minutes)
+        log(f"[bench] serving done: job p50 {line['serving']['job_latency_s_p50']:.2f} s "
+            f"p99 {line['serving']['job_latency_s_p99']:.2f} s")
     if not args.no_cpu:
         log("[bench] cpu baseline (numpy oracle)")
         line["cpu_baseline"] = cpu_baseline(args.workload)

@@ -178,6 +178,11 @@ int lg_timing_report(lg_case* c, int kind, double* ms, double* launches, double*
  * times: calibrates HBM byte counters and measures copy bandwidth.  No
  * counterpart in the reference. */
 int lg_calibrate_copy(long nbytes, int reps, double* ms_per_copy);
+/* Device facts (hipGetDeviceProperties of the current device): out[0] LDS
+ * bytes per workgroup, [1] LDS bytes per CU, [2] CUs, [3] L2 bytes, [4] max
+ * threads per block, [5] wavefront size, [6] memory clock kHz, [7] memory bus
+ * width bits.  n >= 8. */
+int lg_device_info(long* out, int n);
 
 /* --- batches of independent cases (no counterpart in the reference, which
  * runs one case per process).  A batch holds `members` cases of one grid,
@@ -232,6 +237,11 @@ int lg_autotune(lg_case* c, int reps, int* npasses);
 int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg);
 int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg);
 int lg_tune_clear(void);
+/* Enumerate the table (to save it and lg_tune_set it back in a later
+ * process, as FFTW wisdom is kept). */
+int lg_tune_count(int* n);
+int lg_tune_entry(int i, int* kind, int* precision, int* N, int* other, int* n1, int* n2,
+                  int* per_wg);
 
 /* Device bytes lg_create_batch allocates for `members` members. */
 int lg_case_bytes(const lg_params* prm, int members, double* bytes);

@@ -96,8 +96,11 @@ SIGNATURES = {
     "lg_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _I, _I, _I]),
     "lg_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "lg_tune_clear": (C.c_int, []),
+    "lg_tune_count": (C.c_int, [_I]),
+    "lg_tune_entry": (C.c_int, [C.c_int, _I, _I, _I, _I, _I, _I, _I]),
     "lg_case_bytes": (C.c_int, [C.POINTER(LgParams), C.c_int, _D]),
     "lg_mem_info": (C.c_int, [_D, _D]),
+    "lg_device_info": (C.c_int, [C.POINTER(C.c_long), C.c_int]),
     "lg_server_create": (C.c_int, [C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "lg_server_destroy": (C.c_int, [_P]),
     "lg_server_now": (C.c_int, [_P, _D]),

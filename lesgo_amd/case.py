@@ -294,6 +294,36 @@ def tune_clear():
     check(lib().lg_tune_clear(), "tune_clear")
 
 
+def tune_table():
+    """The tuned launch choices as a list of dicts (JSON-ready)."""
+    n = C.c_int()
+    check(lib().lg_tune_count(C.byref(n)), "tune_count")
+    out = []
+    for i in range(n.value):
+        v = [C.c_int() for _ in range(7)]
+        check(lib().lg_tune_entry(i, *[C.byref(x) for x in v]), "tune_entry")
+        k, pr, N, o, a, b, s = (x.value for x in v)
+        out.append({"kind": _lib.PASS_KINDS[k], "precision": "fp64" if pr == LG_FP64 else "mixed",
+                    "N": N, "other": o, "n1": a, "n2": b, "per_wg": s})
+    return out
+
+
+def tune_save(path: str):
+    import json
+    with open(path, "w") as f:
+        json.dump(tune_table(), f, indent=1)
+
+
+def tune_load(path: str) -> int:
+    """Load choices saved by tune_save; returns how many were set."""
+    import json
+    with open(path) as f:
+        rows = json.load(f)
+    for r in rows:
+        tune_set(r["kind"], r["precision"], r["N"], r["other"], r["n1"], r["n2"], r["per_wg"])
+    return len(rows)
+
+
 def seeded_state(cfg: Config, seed: int, noise: float = 0.1):
     """The seeded vel.out-equivalent start (the ic_file path,
     initial.f90:226-239) as host arrays in the reference layout: log-law u

@@ -455,6 +455,29 @@ int lg_tune_clear(void) {
     return guarded([&] { lg::clear_tuned_choices(); });
 }
 
+int lg_tune_count(int* n) {
+    return guarded([&] {
+        REQUIRE(n, "null output");
+        *n = lg::tuned_count();
+    });
+}
+
+int lg_tune_entry(int i, int* kind, int* precision, int* N, int* other, int* n1, int* n2,
+                  int* per_wg) {
+    return guarded([&] {
+        int key[4];
+        lg::PassChoice c;
+        REQUIRE(lg::tuned_entry(i, key, c), "tuning entry out of range");
+        if (kind) *kind = key[0];
+        if (precision) *precision = key[1] == 8 ? LG_FP64 : LG_MIXED;
+        if (N) *N = key[2];
+        if (other) *other = key[3];
+        if (n1) *n1 = c.n1;
+        if (n2) *n2 = c.n2;
+        if (per_wg) *per_wg = c.S;
+    });
+}
+
 int lg_step_graph(lg_case* h, int nsteps) {
     return dispatch(h, [&](auto& c) {
         REQUIRE(nsteps >= 0, "nsteps must be >= 0");

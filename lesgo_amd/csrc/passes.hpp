@@ -48,6 +48,9 @@ bool tuned_choice(int kind, int tsize, int N, int other, PassChoice& out);
 void set_tuned_choice(int kind, int tsize, int N, int other, const PassChoice& c);
 
 void clear_tuned_choices();
+// Entries of the table in key order: key = (kind, element size, N, other).
+int tuned_count();
+bool tuned_entry(int i, int* key, PassChoice& c);
 
 // Launchable (LDS, split instantiated, r2c one-round rule) / heuristic choice.
 template <typename T>

@@ -41,3 +41,23 @@ extern "C" int lg_calibrate_copy(long nbytes, int reps, double* ms_per_copy) {
     (void)hipFree(b);
     return st == hipSuccess ? LG_OK : LG_ERR_HIP;
 }
+
+// Device facts used to size launches: out[0] LDS bytes per workgroup,
+// out[1] LDS bytes per CU, out[2] CUs, out[3] L2 bytes, out[4] max threads
+// per block, out[5] warp size, out[6] memory clock kHz, out[7] bus width bits.
+extern "C" int lg_device_info(long* out, int n) {
+    if (!out || n < 8) return LG_ERR_ARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LG_ERR_HIP;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return LG_ERR_HIP;
+    out[0] = (long)p.sharedMemPerBlock;
+    out[1] = (long)p.maxSharedMemoryPerMultiProcessor;
+    out[2] = p.multiProcessorCount;
+    out[3] = p.l2CacheSize;
+    out[4] = p.maxThreadsPerBlock;
+    out[5] = p.warpSize;
+    out[6] = p.memoryClockRate;
+    out[7] = p.memoryBusWidth;
+    return LG_OK;
+}

@@ -38,6 +38,24 @@ void clear_tuned_choices() {
     g_tune.clear();
 }
 
+int tuned_count() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return (int)g_tune.size();
+}
+
+bool tuned_entry(int i, int* key, PassChoice& c) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (i < 0 || i >= (int)g_tune.size()) return false;
+    auto it = g_tune.begin();
+    std::advance(it, i);
+    key[0] = std::get<0>(it->first);
+    key[1] = std::get<1>(it->first);
+    key[2] = std::get<2>(it->first);
+    key[3] = std::get<3>(it->first);
+    c = it->second;
+    return true;
+}
+
 namespace {
 
 // Candidate choices of one pass: generic kernel and every instantiated
