@@ -183,6 +183,11 @@ int lg_calibrate_copy(long nbytes, int reps, double* ms_per_copy);
  * threads per block, [5] wavefront size, [6] memory clock kHz, [7] memory bus
  * width bits.  n >= 8. */
 int lg_device_info(long* out, int n);
+/* Bandwidth probes over nbytes (multiple of 16), mean ms of reps launches:
+ * 1 copy one float4 per thread, 2 copy four float4 per thread, 3 as 2 with
+ * non-temporal loads/stores, 4 read only, 5 write only.  No counterpart in
+ * the reference. */
+int lg_bw_probe(int variant, long nbytes, int reps, double* ms);
 
 /* --- batches of independent cases (no counterpart in the reference, which
  * runs one case per process).  A batch holds `members` cases of one grid,
@@ -216,6 +221,33 @@ int lg_turbines_get_member(lg_case* c, int m, double* u_d_T, double* f_n, int nt
  * captured once per (RHS storage parity, dynamic-model step) and replayed;
  * recaptured after lg_set_active or a disk change. */
 int lg_step_graph(lg_case* c, int nsteps);
+
+/* --- running time averages (time_average.f90 tavg, driven as io.f90:558-582)
+ * and streamwise spectra, accumulated in FP64 on the device after each
+ * step.  Between nstart and nend (jt_total) every step adds dt to the
+ * pending weight; at nstart and every nskip steps after it the sums are
+ * updated (the first such step only starts the summation).  Quantities
+ * (tavg_t order, time_average.f90:176-319): */
+enum {
+    LG_TAVG_U, LG_TAVG_V, LG_TAVG_W, LG_TAVG_W_UV, LG_TAVG_U_W, LG_TAVG_V_W,
+    LG_TAVG_U2, LG_TAVG_V2, LG_TAVG_W2, LG_TAVG_UV, LG_TAVG_UW, LG_TAVG_VW,
+    LG_TAVG_TXX, LG_TAVG_TYY, LG_TAVG_TZZ, LG_TAVG_TXY, LG_TAVG_TXZ, LG_TAVG_TYZ,
+    LG_TAVG_P, LG_TAVG_FX, LG_TAVG_FY, LG_TAVG_FZ, LG_TAVG_CS_OPT2,
+    LG_TAVG_VORTX, LG_TAVG_VORTY, LG_TAVG_VORTZ, LG_TAVG_N
+};
+int lg_stats_enable(lg_case* c, int nstart, int nend, int nskip);
+int lg_stats_info(lg_case* c, double* total_time, int* initialized);
+/* Time average of one quantity of member m, host double [nz+1][ny][nx]. */
+int lg_stats_mean(lg_case* c, int m, int quantity, double* host);
+/* Streamwise spectrum (0 u, 1 v, 2 w) of member m, host double [nz+1][nx/2]:
+ * E(kx, z) = c_kx sum_ky |f_hat(kx, ky, z)|^2 of the normalised x-y
+ * transform on levels 1..nz-1, time-averaged; c_0 = 1, c_kx = 2 otherwise,
+ * so sum_kx E = plane mean of f^2 less its Nyquist part. */
+int lg_stats_spectrum(lg_case* c, int m, int component, double* host);
+/* Sums of (a - b)^2 and b^2 of one field of member ma of a and member mb of
+ * b (same grid, any precisions) over x < nx and levels k0..k1. */
+int lg_diff_l2(lg_case* a, int ma, lg_case* b, int mb, int field, int k0, int k1,
+               double* num, double* den);
 
 /* --- autotuner (no counterpart in the reference; FFTW_PATIENT planning,
  * fft.f90:114-121, is the nearest analogue).  Pass kinds: */

@@ -99,8 +99,14 @@ SIGNATURES = {
     "lg_tune_count": (C.c_int, [_I]),
     "lg_tune_entry": (C.c_int, [C.c_int, _I, _I, _I, _I, _I, _I, _I]),
     "lg_case_bytes": (C.c_int, [C.POINTER(LgParams), C.c_int, _D]),
+    "lg_stats_enable": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "lg_stats_info": (C.c_int, [_P, _D, _I]),
+    "lg_stats_mean": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_stats_spectrum": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_diff_l2": (C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
     "lg_mem_info": (C.c_int, [_D, _D]),
     "lg_device_info": (C.c_int, [C.POINTER(C.c_long), C.c_int]),
+    "lg_bw_probe": (C.c_int, [C.c_int, C.c_long, C.c_int, _D]),
     "lg_server_create": (C.c_int, [C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "lg_server_destroy": (C.c_int, [_P]),
     "lg_server_now": (C.c_int, [_P, _D]),
@@ -111,6 +117,10 @@ SIGNATURES = {
     "lg_server_result": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "lg_server_stats": (C.c_int, [_P, _D, C.POINTER(C.c_long), _D, _I]),
 }
+
+TAVG_NAMES = ("u", "v", "w", "w_uv", "u_w", "v_w", "u2", "v2", "w2", "uv", "uw", "vw",
+              "txx", "tyy", "tzz", "txy", "txz", "tyz", "p", "fx", "fy", "fz", "cs_opt2",
+              "vortx", "vorty", "vortz")
 
 PASS_KINDS = ("r2c", "c2r", "cols_fwd", "cols_inv", "tridag", "cols_unpad", "cols_pad")
 

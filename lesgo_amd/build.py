@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblesgo_amd.so")
 SOURCES = ["case.hip", "passes_f64.hip", "passes_f32.hip", "tune.hip", "turbines.hip", "capi.hip",
-           "server.hip", "probe.hip"]
+           "server.hip", "stats.hip", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -238,6 +238,40 @@ class Case:
     def tau_wall_bot(self, member: int = 0):
         return self._scalar(lib().lg_tau_wall_bot_member, member, "tau_wall_bot")
 
+    # -- running time averages (time_average.f90) and spectra ---------------
+    def stats_enable(self, nstart: int, nend: int, nskip: int = 1):
+        check(lib().lg_stats_enable(self._h, nstart, nend, nskip), "stats_enable")
+
+    def stats_info(self):
+        t, i = C.c_double(), C.c_int()
+        check(lib().lg_stats_info(self._h, C.byref(t), C.byref(i)), "stats_info")
+        return t.value, bool(i.value)
+
+    def stats_mean(self, name: str, member: int = 0):
+        """Time average of a tavg quantity, [nz+1, ny, nx] float64."""
+        a = np.empty((self.nz + 1, self.ny, self.cfg.nx))
+        check(lib().lg_stats_mean(self._h, member, _lib.TAVG_NAMES.index(name),
+                                  a.ctypes.data_as(_lib._D)), "stats_mean")
+        return a
+
+    def stats_profile(self, name: str, member: int = 0):
+        return self.stats_mean(name, member).mean(axis=(1, 2))
+
+    def stats_spectrum(self, component: int, member: int = 0):
+        """Streamwise spectrum E(kx, z) of u (0), v (1) or w (2), [nz+1, nx/2]."""
+        a = np.empty((self.nz + 1, self.cfg.nx // 2))
+        check(lib().lg_stats_spectrum(self._h, member, component, a.ctypes.data_as(_lib._D)),
+              "stats_spectrum")
+        return a
+
+    def diff_l2(self, other: "Case", field: str, k0: int, k1: int, member: int = 0,
+                other_member: int = 0):
+        """Relative L2 of this case's field against other's (x < nx, k0..k1)."""
+        n, d = C.c_double(), C.c_double()
+        check(lib().lg_diff_l2(self._h, member, other._h, other_member, _fid(field), k0, k1,
+                               C.byref(n), C.byref(d)), "diff_l2")
+        return math.sqrt(n.value / d.value) if d.value > 0 else math.sqrt(n.value)
+
     # -- per-kernel timing --------------------------------------------------
     def timing(self, on: bool):
         check(lib().lg_timing_enable(self._h, int(on)), "timing_enable")

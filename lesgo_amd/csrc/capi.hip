@@ -478,6 +478,58 @@ int lg_tune_entry(int i, int* kind, int* precision, int* N, int* other, int* n1,
     });
 }
 
+int lg_stats_enable(lg_case* h, int nstart, int nend, int nskip) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(nskip >= 1 && nend >= nstart, "bad statistics window");
+        c.stats_enable(nstart, nend, nskip);
+    });
+}
+
+int lg_stats_info(lg_case* h, double* total_time, int* initialized) {
+    return dispatch(h, [&](auto& c) {
+        if (total_time) *total_time = c.st.total_time;
+        if (initialized) *initialized = c.st.initialized ? 1 : 0;
+    });
+}
+
+int lg_stats_mean(lg_case* h, int m, int quantity, double* host) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(host, "null host pointer");
+        c.stats_mean(m, quantity, host);
+    });
+}
+
+int lg_stats_spectrum(lg_case* h, int m, int component, double* host) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(host, "null host pointer");
+        c.stats_spectrum(m, component, host);
+    });
+}
+
+int lg_diff_l2(lg_case* a, int ma, lg_case* b, int mb, int field, int k0, int k1, double* num,
+               double* den) {
+    if (!a || !b) {
+        lg::set_error("null case handle");
+        return LG_ERR_ARG;
+    }
+    return guarded([&] {
+        check_field(field);
+        REQUIRE(num && den, "null output");
+        auto run = [&](auto& ca, auto& cb) {
+            check_member(ca, ma);
+            check_member(cb, mb);
+            REQUIRE(k0 >= 0 && k1 <= ca.d.nz && k0 <= k1, "level range out of bounds");
+            lg::diff_l2(ca, ma, cb, mb, field, k0, k1, num, den);
+        };
+        if (a->c64 && b->c64) run(*a->c64, *b->c64);
+        else if (a->c64) run(*a->c64, *b->c32);
+        else if (b->c64) run(*a->c32, *b->c64);
+        else run(*a->c32, *b->c32);
+    });
+}
+
 int lg_step_graph(lg_case* h, int nsteps) {
     return dispatch(h, [&](auto& c) {
         REQUIRE(nsteps >= 0, "nsteps must be >= 0");

@@ -215,6 +215,8 @@ Case<T>::~Case() {
     fr(d_denom);
     fr(d_mpf);
     fr(d_flags);
+    fr(tavg);
+    fr(tspec);
     fr(G1);
     fr(G2);
     if (stream) (void)hipStreamDestroy(stream);
@@ -286,7 +288,7 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
     const size_t chunk = (size_t)std::min(d.nz, 64);
     size_t b = 0;
     b += (size_t)(LG_NFIELDS + 9 + 4) * M * d.field * T;        // fields, tmp, Ftmp
-    b += (size_t)(8 + 3) * M * msp * 2 * T;                      // spec, sU/sV/sW
+    b += (size_t)(12 + 3) * M * msp * 2 * T;                     // spec, sU/sV/sW
     b += 9 * M * mbp * T + 6 * M * mbsp * 2 * T;                 // big, bigspec
     b += M * msp * (sizeof(double) + sizeof(double2));           // gam, usc
     b += (15 + 42) * M * chunk * d.plane * T + 21 * M * chunk * d.splane * 2 * T;
@@ -442,6 +444,22 @@ void Case<T>::fill_level(int id, int k, double v) {
                        dim3(BLK), 0, stream, p, d.field, d.plane, (T)v);
 }
 
+template <typename T>
+void Case<T>::fill_levels(std::initializer_list<Fill> f) {
+    FillList<T> a{};
+    int n = 0;
+    for (const Fill& x : f) {
+        a.p[n] = fld[x.id] + (long)x.k * d.plane;
+        a.v[n] = (T)x.v;
+        if (++n == MAX_FILLS) break;
+    }
+    if (n == 0) return;
+    a.mstride = d.field;
+    a.plane = d.plane;
+    hipLaunchKernelGGL(k_fill_list<T>, dim3((unsigned)((d.plane + BLK - 1) / BLK), Ma, n),
+                       dim3(BLK), 0, stream, a);
+}
+
 // dst[q] = src[q], q < n (a kernel, so that captured steps hold no copy nodes)
 template <typename T>
 static void copy_dev(const T* src, T* dst, long n, hipStream_t s) {
@@ -502,6 +520,24 @@ void Case<T>::filt_da(int f, int dfdx, int dfdy, cplx<T>* keep) {
     inv2d(3, s3, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, d.nz + 1, tsp);
 }
 
+// filt_da (derivatives.f90:166-211) of u, v and w: the three forward
+// transforms in one pass pair, the nine inverses (filtered field, d/dx, d/dy
+// of each) in another; the spectra are kept in sU, sV, sW for convec.
+template <typename T>
+void Case<T>::filt_da_uvw() {
+    const T sc = (T)(1.0 / (d.nx * d.ny));
+    const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
+    const T scs[3] = {sc, sc, sc};
+    cplx<T>* S[3] = {sU, sV, sW};
+    fwd2d(3, src, scs, d.plane, 0, d.field, S, d.splane, 0, msp, d.nz + 1);
+    const cplx<T>* is[9] = {sU, sU, sU, sV, sV, sV, sW, sW, sW};
+    const int ops[9] = {OP_NONE, OP_IKX, OP_IKY, OP_NONE, OP_IKX, OP_IKY,
+                        OP_NONE, OP_IKX, OP_IKY};
+    T* dst[9] = {fld[LG_U], fld[LG_DUDX], fld[LG_DUDY], fld[LG_V], fld[LG_DVDX],
+                 fld[LG_DVDY], fld[LG_W], fld[LG_DWDX], fld[LG_DWDY]};
+    inv2d(9, is, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, d.nz + 1, spec);
+}
+
 template <typename T>
 void Case<T>::ddxy(int f, int dfdx, int dfdy) {
     const T sc = (T)(1.0 / (d.nx * d.ny));
@@ -533,17 +569,14 @@ template <typename T>
 void Case<T>::ddz_uv(int f, int dfdz) {
     hipLaunchKernelGGL(k_ddz_uv<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, fld[f],
                        fld[dfdz], GEO(1, d.nz), (T)(1.0 / d.dz));
-    fill_level(dfdz, 0, BOGUS);
-    fill_level(dfdz, 1, BOGUS);
-    fill_level(dfdz, d.nz, BOGUS);
+    fill_levels({{dfdz, 0, BOGUS}, {dfdz, 1, BOGUS}, {dfdz, d.nz, BOGUS}});
 }
 
 template <typename T>
 void Case<T>::ddz_w(int f, int dfdz) {
     hipLaunchKernelGGL(k_ddz_w<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, fld[f],
                        fld[dfdz], GEO(0, d.nz), (T)(1.0 / d.dz));
-    fill_level(dfdz, 0, BOGUS);
-    fill_level(dfdz, d.nz, BOGUS);
+    fill_levels({{dfdz, 0, BOGUS}, {dfdz, d.nz, BOGUS}});
 }
 
 // One plane per member (member stride = field) filtered by op.
@@ -578,10 +611,8 @@ void Case<T>::wallstress() {
                        (const T*)tmp[0], (const T*)tmp[1], d.field, (const T*)fld[LG_U],
                        (const T*)fld[LG_V], fld[LG_TXZ], fld[LG_TYZ], fld[LG_DUDZ], fld[LG_DVDZ],
                        GEO(1, 1), (const double*)d_denom, d.prm.vonk, d.dz);
-    fill_level(LG_TXZ, d.nz, 0.0);
-    fill_level(LG_TYZ, d.nz, 0.0);
-    fill_level(LG_DUDZ, d.nz, 0.0);
-    fill_level(LG_DVDZ, d.nz, 0.0);
+    fill_levels({{LG_TXZ, d.nz, 0.0}, {LG_TYZ, d.nz, 0.0}, {LG_DUDZ, d.nz, 0.0},
+                 {LG_DVDZ, d.nz, 0.0}});
 }
 
 // ---------------------------------------------------------------------------
@@ -613,8 +644,9 @@ void Case<T>::sgs_stag() {
                  fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ], (T)(d.delta * d.delta), (T)d.prm.nu};
     hipLaunchKernelGGL(k_sgs_tau<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
                        GEO(1, d.nz));
-    for (int id : {LG_TXX, LG_TXY, LG_TXZ, LG_TYY, LG_TYZ, LG_TZZ}) fill_level(id, 0, BOGUS);
-    for (int id : {LG_TXX, LG_TXY, LG_TYY, LG_TZZ}) fill_level(id, d.nz, BOGUS);
+    fill_levels({{LG_TXX, 0, BOGUS}, {LG_TXY, 0, BOGUS}, {LG_TXZ, 0, BOGUS}, {LG_TYY, 0, BOGUS},
+                 {LG_TYZ, 0, BOGUS}, {LG_TZZ, 0, BOGUS}, {LG_TXX, d.nz, BOGUS},
+                 {LG_TXY, d.nz, BOGUS}, {LG_TYY, d.nz, BOGUS}, {LG_TZZ, d.nz, BOGUS}});
 }
 
 // ---------------------------------------------------------------------------
@@ -720,10 +752,8 @@ void Case<T>::divstress_uv() {
                        (const T*)tmp[0], (const T*)fld[LG_TXZ], fld[LG_DIVTX], GEO(1, nl), rdz);
     hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl, Ma), dim3(BLK), 0, stream,
                        (const T*)tmp[2], (const T*)fld[LG_TYZ], fld[LG_DIVTY], GEO(1, nl), rdz);
-    for (int id : {LG_DIVTX, LG_DIVTY}) {
-        fill_level(id, 0, BOGUS);
-        fill_level(id, d.nz, BOGUS);
-    }
+    fill_levels({{LG_DIVTX, 0, BOGUS}, {LG_DIVTX, d.nz, BOGUS}, {LG_DIVTY, 0, BOGUS},
+                 {LG_DIVTY, d.nz, BOGUS}});
 }
 
 template <typename T>
@@ -742,7 +772,35 @@ void Case<T>::divstress_w() {
     hipLaunchKernelGGL(k_divt_w<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
                        (const T*)tmp[4], (const T*)fld[LG_TZZ], fld[LG_DIVTZ], GEO(1, d.nz),
                        (T)(1.0 / d.dz));
-    fill_level(LG_DIVTZ, 0, BOGUS);
+    fill_levels({{LG_DIVTZ, 0, BOGUS}});
+}
+
+// divstress_uv + divstress_w with the transforms of both in one forward and
+// one inverse pass over levels 1..nz (the uv sums' level nz is computed and
+// not used).
+template <typename T>
+void Case<T>::divstress_uvw() {
+    const T sc = (T)(1.0 / (d.nx * d.ny));
+    const T* src[5] = {fld[LG_TXX], fld[LG_TXY], fld[LG_TYY], fld[LG_TXZ], fld[LG_TYZ]};
+    const T scs[5] = {sc, sc, sc, sc, sc};
+    cplx<T>* S[5] = {spec[0], spec[1], spec[2], spec[3], spec[4]};
+    fwd2d(5, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
+    const cplx<T>* is[3] = {spec[0], spec[1], spec[3]};
+    const cplx<T>* is2[3] = {spec[1], spec[2], spec[4]};
+    const int ops[3] = {OP_DIVXY, OP_DIVXY, OP_DIVXY};
+    T* dst[3] = {tmp[0], tmp[2], tmp[4]};
+    cplx<T>* tsp[3] = {spec[5], spec[6], spec[7]};
+    inv2d(3, is, ops, d.splane, 1, msp, dst, d.plane, 1, d.field, d.nz, tsp, is2);
+    const int nl = d.nz - 1;
+    const T rdz = (T)(1.0 / d.dz);
+    hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[0], (const T*)fld[LG_TXZ], fld[LG_DIVTX], GEO(1, nl), rdz);
+    hipLaunchKernelGGL(k_divt_uv<T>, grid_ij(d.ld, d.ny, nl, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[2], (const T*)fld[LG_TYZ], fld[LG_DIVTY], GEO(1, nl), rdz);
+    hipLaunchKernelGGL(k_divt_w<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+                       (const T*)tmp[4], (const T*)fld[LG_TZZ], fld[LG_DIVTZ], GEO(1, d.nz), rdz);
+    fill_levels({{LG_DIVTX, 0, BOGUS}, {LG_DIVTX, d.nz, BOGUS}, {LG_DIVTY, 0, BOGUS},
+                 {LG_DIVTY, d.nz, BOGUS}, {LG_DIVTZ, 0, BOGUS}});
 }
 
 // ---------------------------------------------------------------------------
@@ -797,9 +855,8 @@ void Case<T>::convec(bool reuse_spectra) {
     T* dst[3] = {fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ]};
     cplx<T>* tsp[3] = {spec[3], spec[4], spec[5]};
     inv2d(3, is, ops, d.splane, 1, msp, dst, d.plane, 1, d.field, nl, tsp);
-    for (int id : {LG_RHSX, LG_RHSY, LG_RHSZ}) fill_level(id, 0, BOGUS);
-    fill_level(LG_RHSX, d.nz, BOGUS);
-    fill_level(LG_RHSY, d.nz, BOGUS);
+    fill_levels({{LG_RHSX, 0, BOGUS}, {LG_RHSY, 0, BOGUS}, {LG_RHSZ, 0, BOGUS},
+                 {LG_RHSX, d.nz, BOGUS}, {LG_RHSY, d.nz, BOGUS}});
 }
 
 // ---------------------------------------------------------------------------
@@ -810,15 +867,12 @@ void Case<T>::press_stag_array() {
     const double cst = 1.0 / (d.nx * d.ny);
     const T c2 = (T)(cst / d.tadv1 / d.prm.dt);
     {
-        const T* src[2] = {fld[LG_U], fld[LG_V]};
-        const T scs[2] = {c2, c2};
-        cplx<T>* S[2] = {spec[0], spec[1]};
-        fwd2d(2, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz - 1);
-    }
-    {
-        const T* src[1] = {fld[LG_W]};
-        cplx<T>* S[1] = {spec[2]};
-        fwd2d(1, src, &c2, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
+        // u*, v* are used on levels 1..nz-1 and w* on 1..nz: one pass over
+        // 1..nz (the u*, v* level nz spectra are not read)
+        const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
+        const T scs[3] = {c2, c2, c2};
+        cplx<T>* S[3] = {spec[0], spec[1], spec[2]};
+        fwd2d(3, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
     }
     {
         // divtz at k = 1 and k = nz into planes 0 and 1 of spec[3]
@@ -856,8 +910,7 @@ void Case<T>::press_stag_array() {
     }
     hipLaunchKernelGGL(k_dpdz<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
                        (const T*)fld[LG_P], fld[LG_DPDZ], GEO(1, d.nz), (T)d.dz);
-    fill_level(LG_DPDX, d.nz, BOGUS);
-    fill_level(LG_DPDY, d.nz, BOGUS);
+    fill_levels({{LG_DPDX, d.nz, BOGUS}, {LG_DPDY, d.nz, BOGUS}});
 }
 
 // ---------------------------------------------------------------------------
@@ -874,8 +927,7 @@ void Case<T>::project() {
                        lev(LG_U, d.nz), d.field, d.plane);
     hipLaunchKernelGGL(k_copy_plane<T>, gp, dim3(BLK), 0, stream, (const T*)lev(LG_V, d.nz - 1),
                        lev(LG_V, d.nz), d.field, d.plane);
-    fill_level(LG_W, d.nz, 0.0);
-    fill_level(LG_W, 1, 0.0);
+    fill_levels({{LG_W, d.nz, 0.0}, {LG_W, 1, 0.0}});
 }
 
 // ---------------------------------------------------------------------------
@@ -891,9 +943,8 @@ void Case<T>::rhs_ab2() {
                  (const int*)d_flags, d.prm.use_mean_p_force};
     hipLaunchKernelGGL(k_rhs_ab2<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
                        GEO(1, d.nz));
-    for (int id : {LG_U, LG_V, LG_W}) fill_level(id, 0, BOGUS);
-    fill_level(LG_U, d.nz, BOGUS);
-    fill_level(LG_V, d.nz, BOGUS);
+    fill_levels({{LG_U, 0, BOGUS}, {LG_V, 0, BOGUS}, {LG_W, 0, BOGUS}, {LG_U, d.nz, BOGUS},
+                 {LG_V, d.nz, BOGUS}});
 }
 
 template <typename T>
@@ -906,16 +957,13 @@ void Case<T>::rhs_gradp() {
 // The loop body after the counters advance and RHS_f = RHS (main.f90:155-326).
 template <typename T>
 void Case<T>::step_body() {
-    filt_da(LG_U, LG_DUDX, LG_DUDY, sU);
-    filt_da(LG_V, LG_DVDX, LG_DVDY, sV);
-    filt_da(LG_W, LG_DWDX, LG_DWDY, sW);
+    filt_da_uvw();
     ddz_uv(LG_U, LG_DUDZ);
     ddz_uv(LG_V, LG_DVDZ);
     ddz_w(LG_W, LG_DWDZ);
     wallstress();
     sgs_stag();
-    divstress_uv();
-    divstress_w();
+    divstress_uvw();
     convec(true);
     if (turb) turbines_forcing();
     rhs_ab2();
@@ -982,6 +1030,7 @@ void Case<T>::step() {
     refresh_flags();
     LG_SWAP_RHS();
     step_body();
+    if (st.on) stats_after_step();
 }
 
 template <typename T>
@@ -1007,6 +1056,7 @@ void Case<T>::step_graph() {
     LG_SWAP_RHS();
     if (special) {
         step_body();
+        if (st.on) stats_after_step();
         return;
     }
     const int key = (swapped ? 1 : 0) + (lag_now ? 2 : 0);
@@ -1035,6 +1085,7 @@ void Case<T>::step_graph() {
         graph_ma[key] = Ma;
     }
     LG_HIP(hipGraphLaunch(graphs[key], stream));
+    if (st.on) stats_after_step();
 }
 
 // ---------------------------------------------------------------------------

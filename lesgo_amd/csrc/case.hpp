@@ -1,6 +1,7 @@
 // Device-resident state of one LES case and the subsystem drivers.
 #pragma once
 
+#include <initializer_list>
 #include <memory>
 #include <vector>
 
@@ -90,12 +91,21 @@ struct Case {
     std::vector<int> dflags_now;     // the copy on the device
     int* d_flags = nullptr;
     bool lag_now = false;            // the dynamic model runs this step
+    // running time averages (time_average.f90) and streamwise spectra
+    struct StatsCfg {
+        int nstart = 0, nend = -1, nskip = 1;
+        bool on = false, initialized = false;
+        double dt = 0.0, total_time = 0.0;
+    };
+    StatsCfg st;
+    double* tavg = nullptr;          // 26 quantities x M x (nz+1) x ny x nx, FP64
+    double* tspec = nullptr;         // 3 components x M x (nz+1) x NK, FP64
     hipGraphExec_t graphs[4] = {};   // captured steps keyed by (swapped, dynamic model)
     int graph_ma[4] = {};
 
     T* fld[LG_NFIELDS] = {};
     // spectral scratch, each (nz+1) small spectral planes
-    static constexpr int NSPEC = 8;
+    static constexpr int NSPEC = 12;
     cplx<T>* spec[NSPEC] = {};
     // big-grid scratch (levels 0..nz)
     static constexpr int NBIG = 9;
@@ -158,6 +168,10 @@ struct Case {
     void lagrange_sdep();
     void divstress_uv();
     void divstress_w();
+    // the step's fused forms: filt_da of u, v, w in one pass pair; both
+    // divstress routines with one forward and one inverse pass
+    void filt_da_uvw();
+    void divstress_uvw();
     void convec(bool reuse_spectra = false);
     void press_stag_array();
     void project();
@@ -176,13 +190,29 @@ struct Case {
     double max_cfl(int m = 0);
     double tau_wall_bot(int m = 0);
     void fill_level(int id, int k, double v);
+    struct Fill {
+        int id, k;
+        double v;
+    };
+    // the planes (id, k) of every stepped member set to v, one launch
+    void fill_levels(std::initializer_list<Fill> f);
     void copy_member(int src, int dst);
     void set_member_params(int m, double zo_m, double mpfx, double mpfy);
     void seed_member(int m, long seed, double noise);
     // Remove member `slot`: the last stepped member moves into it.
     void retire_member(int slot);
     void check_device_error();
+    void stats_enable(int nstart, int nend, int nskip);
+    void stats_after_step();
+    void stats_mean(int m, int q, double* host);
+    void stats_spectrum(int m, int c, double* host);
 };
+
+// Sums of (a - b)^2 and b^2 of one field over i < nx, k0..k1 (two cases of
+// one grid, any precisions).
+template <typename TA, typename TB>
+void diff_l2(Case<TA>& a, int ma, Case<TB>& b, int mb, int field, int k0, int k1, double* num,
+             double* den);
 
 // Device bytes a batch of `members` allocates (tsize = 8 or 4).
 size_t case_bytes(const lg_params& p, int members, int tsize);

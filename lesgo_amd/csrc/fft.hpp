@@ -23,7 +23,7 @@
 namespace lg {
 
 constexpr int MAX_STAGES = 12;
-constexpr int MAX_JOBS = 8;
+constexpr int MAX_JOBS = 12;
 
 struct FftPlan {
     int N = 0;

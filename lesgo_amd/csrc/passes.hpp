@@ -11,7 +11,10 @@
 
 namespace lg {
 
-constexpr size_t LDS_MAX = 64 * 1024;
+// LDS a workgroup may take: the tuner's bound (gfx950 gives a workgroup the
+// CU's 160 KB) and the built-in heuristics' budget.
+constexpr size_t LDS_MAX = 160 * 1024;
+constexpr size_t LDS_DEFAULT = 64 * 1024;
 
 // Launch choice of one transform pass: the four-step split (n1 = 0: the
 // generic LDS kernel) and the rows (pairs) or columns per workgroup.

@@ -109,11 +109,11 @@ PassChoice default_choice(int kind, int N, int other) {
         if (c.n1) S = std::min(kind == PK_R2C ? std::min(PBLK / c.n1, 16) : 16, other / 2);
         else S = std::min(std::max(1, 2048 / N), other / 2);
         S = std::max(S, 1);
-        while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_MAX) --S;
+        while (S > 1 && (size_t)S * YS * sizeof(cplx<T>) > LDS_DEFAULT) --S;
         c.S = S;
     } else {
         int TC = 16;
-        while (TC > 1 && (size_t)TC * YS * sizeof(cplx<T>) > LDS_MAX) TC /= 2;
+        while (TC > 1 && (size_t)TC * YS * sizeof(cplx<T>) > LDS_DEFAULT) TC /= 2;
         c.S = std::min(TC, other);
     }
     return c;

@@ -40,6 +40,23 @@ __global__ void k_fill_planes(T* f, long mstride, long n, T v) {
     if (idx < n) f[(long)blockIdx.y * mstride + idx] = v;
 }
 
+// Up to 16 (plane, value) fills in one launch: grid (blocks per plane,
+// members, entries); entry q sets p[q][m * mstride + i] = v[q], i < plane.
+constexpr int MAX_FILLS = 16;
+template <typename T>
+struct FillList {
+    T* p[MAX_FILLS];
+    T v[MAX_FILLS];
+    long mstride, plane;
+};
+
+template <typename T>
+__global__ void k_fill_list(FillList<T> a) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.plane) return;
+    a.p[blockIdx.z][(long)blockIdx.y * a.mstride + idx] = a.v[blockIdx.z];
+}
+
 // f[m * mstride + q] = v for q < n on the members with flag[m] != 0
 template <typename T>
 __global__ void k_fill_members_if(T* f, const int* flag, long mstride, long n, T v) {

@@ -70,20 +70,25 @@ std::vector<PassChoice> candidates(int kind, int N, int other) {
     std::vector<std::pair<int, int>> sp = pass_splits(N);
     sp.insert(sp.begin(), {0, 0});
     for (auto& s : sp)
-        for (int S : {1, 2, 4, 8, 16, 32, 64}) {
+        for (int S : {1, 2, 4, 8, 12, 16, 24, 32, 48, 64}) {
             PassChoice c{s.first, s.second, S};
             if (choice_ok<T>(kind, c, N, other)) out.push_back(c);
         }
     return out;
 }
 
-// Median of reps timed launches (after one untimed) of launch().
+// Median of reps timed launches (after one untimed) of launch(); a choice
+// the runtime refuses to launch (resources) times as +inf.
 template <class F>
 float time_launch(hipStream_t s, int reps, F&& launch) {
+    launch();
+    if (hipGetLastError() != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        return 1e30f;
+    }
     hipEvent_t a, b;
     LG_HIP(hipEventCreate(&a));
     LG_HIP(hipEventCreate(&b));
-    launch();
     std::vector<float> t;
     for (int r = 0; r < reps; ++r) {
         LG_HIP(hipEventRecord(a, s));

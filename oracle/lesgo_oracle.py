@@ -9,8 +9,9 @@ It restates, in vectorised numpy, the body of the reference time loop
 build (``lbz = 0``, ``nz = Nz + 1``, every plane exchange a no-op because the
 1-D Cartesian communicator is non-periodic, mpi_defs.f90:77-85), with
 ``PPSAFETYMODE`` BOGUS fills, ``sgs_model = 5``, ``lbc_mom = 2``,
-``ubc_mom = 0``, a file initial condition (``initu = .true.``,
-``inilag = .false.``, initial.f90:96-110) and optionally ``PPTURBINES``.
+``ubc_mom = 0``, either start schedule of initial.f90:96-110 (file start
+``initu = .true.`` or fresh start ``inilag = .true.``), optionally
+``PPTURBINES``, and the running time averages of time_average.f90.
 
 Array layout mirrors the reference exactly: a 3-D field is a C-ordered
 ``(nz+1, ny, ld)`` float64 array indexed ``[k, j, i]`` for Fortran
@@ -983,6 +984,108 @@ def domain_output(p: Params, s: State):
     v = s.v[:, :, :nx]
     pr = s.f["p"][:, :, :nx] - 0.5 * (u ** 2 + w_uv ** 2 + v ** 2)
     return u[1:], v[1:], w_uv[1:], pr[1:]
+
+
+# --------------------------------------------------------------------------
+# time_average.f90 (tavg) and streamwise spectra
+# --------------------------------------------------------------------------
+TAVG_NAMES = ("u", "v", "w", "w_uv", "u_w", "v_w", "u2", "v2", "w2", "uv", "uw", "vw",
+              "txx", "tyy", "tzz", "txy", "txz", "tyz", "p", "fx", "fy", "fz", "cs_opt2",
+              "vortx", "vorty", "vortz")
+
+
+class Stats:
+    """Running time sums of time_average.f90 (``tavg%compute``, 176-319) on
+    the nx x ny x (nz+1) grid, driven as io.f90:558-582: between nstart and
+    nend every step adds dt to the pending weight; at nstart and every nskip
+    steps after it the sums are updated (the first such call only starts the
+    summation, as ``tavg%init`` does).  Levels the reference leaves
+    unassigned before summing (w_uv, u_w, v_w, vortz at k = 0) are 0 here.
+
+    Spectra (project-defined; the reference's spectra output is not part of
+    the build): for u, v, w on levels 1..nz-1 the y-summed power of the
+    normalised x-y transform, E(kx, z) = sum_ky |u_hat(kx, ky, z)|^2 for
+    kx < nx/2, time-weighted like the sums; ``spectrum`` doubles kx > 0 so
+    that sum_kx E = plane mean of u^2 (Parseval; the Nyquist mode excluded).
+    Pinned by tests/test_oracle_stats.py."""
+
+    def __init__(self, p: Params, nstart: int, nend: int, nskip: int):
+        self.p = p
+        self.nstart, self.nend, self.nskip = nstart, nend, nskip
+        shp = (p.nz + 1, p.ny, p.nx)
+        self.sum = {n: np.zeros(shp) for n in TAVG_NAMES}
+        self.spec = np.zeros((3, p.nz + 1, p.nx // 2))
+        self.total_time = 0.0
+        self.dt = 0.0
+        self.initialized = False
+
+    def after_step(self, sp: Spectral, s: State):
+        jt = s.jt_total
+        if self.nstart <= jt <= self.nend:
+            self.dt += self.p.dt
+            if (jt - self.nstart) % self.nskip == 0:
+                if not self.initialized:
+                    self.initialized = True
+                    self.dt = 0.0
+                else:
+                    self.compute(sp, s)
+
+    def compute(self, sp: Spectral, s: State):
+        p = self.p
+        nx, nz = p.nx, p.nz
+        X = np.s_[:, :, :nx]
+        u, v, w = s.u[X], s.v[X], s.w[X]
+        w_uv = np.zeros_like(u)
+        w_uv[1:nz] = 0.5 * (w[2:nz + 1] + w[1:nz])
+        w_uv[nz] = w_uv[nz - 1]
+        u_w = np.zeros_like(u)
+        v_w = np.zeros_like(u)
+        u_w[1:nz + 1] = 0.5 * (u[0:nz] + u[1:nz + 1])
+        v_w[1:nz + 1] = 0.5 * (v[0:nz] + v[1:nz + 1])
+        u_w[1] = 0.0
+        v_w[1] = 0.0
+        fz = s.fza[X]
+        fza_uv = np.zeros_like(u)
+        fza_uv[1:nz] = 0.5 * (fz[2:nz + 1] + fz[1:nz])
+        fza_uv[nz] = fza_uv[nz - 1]
+        pres = s.f["p"][X] - 0.5 * (u ** 2 + w_uv ** 2 + v ** 2)
+        vt = s.dvdx[X] - s.dudy[X]
+        vortz = np.zeros_like(u)
+        vortz[1:nz + 1] = 0.5 * (vt[0:nz] + vt[1:nz + 1])
+        vortz[1] = 0.0
+        vortx = s.dwdy[X] - s.dvdz[X]
+        vorty = s.dudz[X] - s.dwdx[X]
+        dt = self.dt
+        q = {"u": u, "v": v, "w": w, "w_uv": w_uv, "u_w": u_w, "v_w": v_w,
+             "u2": u * u, "v2": v * v, "w2": w * w, "uv": u * v, "uw": u_w * w,
+             "vw": v_w * w, "txx": s.txx[X], "tyy": s.tyy[X], "tzz": s.tzz[X],
+             "txy": s.txy[X], "txz": s.txz[X], "tyz": s.tyz[X], "p": pres,
+             "vortx": vortx, "vorty": vorty, "vortz": vortz}
+        for n, a in q.items():
+            self.sum[n] += a * dt
+        K = np.s_[1:nz + 1]
+        self.sum["fx"][K] += s.fxa[X][K] * dt
+        self.sum["fy"][K] += s.fya[X][K] * dt
+        self.sum["fz"][K] += fza_uv[K] * dt
+        self.sum["cs_opt2"][K] += s.Cs_opt2[X][K] * dt
+        c = 1.0 / (nx * p.ny)
+        for comp, f in enumerate((s.u, s.v, s.w)):
+            F = sp.r2c(f[1:nz] * c)[:, :, :nx // 2].copy()
+            F[:, p.ny // 2, :] = 0.0
+            self.spec[comp, 1:nz] += dt * np.sum(np.abs(F) ** 2, axis=1)
+        self.total_time += dt
+        self.dt = 0.0
+
+    def mean(self, name):
+        return self.sum[name] / self.total_time
+
+    def profile(self, name):
+        return self.mean(name).mean(axis=(1, 2))
+
+    def spectrum(self, comp):
+        E = self.spec[comp] / self.total_time
+        E[:, 1:] *= 2.0
+        return E
 
 
 # --------------------------------------------------------------------------
