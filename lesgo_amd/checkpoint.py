@@ -1,0 +1,103 @@
+"""The reference's restart files for a device-resident case.
+
+``write_checkpoint`` writes what ``checkpoint`` writes (io.f90:1173-1267) for
+one process: ``vel.out`` (one unformatted sequential record holding u, v, w,
+RHSx, RHSy, RHSz, Cs_opt2, F_LM, F_MM, F_QN, F_NN, each ``(ld, ny, 1:nz)``
+float64), ``grid.out`` (nproc, Nx, Ny, Nz, L_x, L_y, L_z) and
+``total_time.dat`` (jt_total, total_time, total_time_dim, dt, cfl).
+``read_checkpoint`` is ``ic_file`` (initial.f90:226-239) plus the
+``total_time.dat`` counters.  Unformatted records use 4-byte little-endian
+length markers; a record longer than 2**31 - 9 bytes is split into
+subrecords whose markers are negative except the last leading and the
+first trailing one (gfortran's convention).
+"""
+from __future__ import annotations
+
+import os
+import struct
+
+import numpy as np
+
+from .case import Case
+
+CHECKPOINT_FIELDS = ("u", "v", "w", "RHSx", "RHSy", "RHSz", "Cs_opt2", "F_LM", "F_MM",
+                     "F_QN", "F_NN")
+_MAXSUB = 2 ** 31 - 9
+
+
+def write_record(f, payload: bytes):
+    """One Fortran unformatted sequential record."""
+    n = len(payload)
+    if n <= _MAXSUB:
+        f.write(struct.pack("<i", n))
+        f.write(payload)
+        f.write(struct.pack("<i", n))
+        return
+    pos, first = 0, True
+    while pos < n:
+        chunk = payload[pos:pos + _MAXSUB]
+        last = pos + len(chunk) >= n
+        f.write(struct.pack("<i", len(chunk) if last else -len(chunk)))
+        f.write(chunk)
+        f.write(struct.pack("<i", len(chunk) if first else -len(chunk)))
+        pos += len(chunk)
+        first = False
+
+
+def read_record(f) -> bytes:
+    out = []
+    while True:
+        (lead,) = struct.unpack("<i", f.read(4))
+        out.append(f.read(abs(lead)))
+        f.read(4)
+        if lead >= 0:
+            return b"".join(out)
+
+
+def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "vel.out"):
+    """vel.out, grid.out and total_time.dat of member `member` into `path`."""
+    cfg = case.cfg
+    nz = case.nz
+    jt, jt_total = case.counters()
+    parts = [np.ascontiguousarray(case.download(n, member)[1:nz + 1]) for n in CHECKPOINT_FIELDS]
+    with open(os.path.join(path, fname), "wb") as f:
+        write_record(f, b"".join(a.astype("<f8").tobytes() for a in parts))
+    with open(os.path.join(path, "grid.out"), "wb") as f:
+        write_record(f, struct.pack("<4i3d", 1, cfg.nx, cfg.ny, cfg.Nz, cfg.lx, cfg.ly, cfg.lz))
+    cfl = case.max_cfl(member)
+    t = jt_total * cfg.dt
+    with open(os.path.join(path, "total_time.dat"), "w") as f:
+        f.write(f" {jt_total:d} {t!r} {t!r} {cfg.dt!r} {cfl!r}\n")
+
+
+def read_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "vel.out",
+                    set_counters: bool = True):
+    """Load a checkpoint into member `member`; returns (jt_total, total_time)
+    from total_time.dat (0, 0.0 if absent) and sets the case's counters.
+    A restart runs the file-start schedule (the reference sets initu =
+    .true. when the file exists), so the case must be made with initu."""
+    if not case.cfg.initu:
+        raise ValueError("a restarted case runs the file-start schedule: make it with initu=True")
+    nz = case.nz
+    shp = (nz, case.ny, case.ld)
+    with open(os.path.join(path, fname), "rb") as f:
+        raw = read_record(f)
+    n = int(np.prod(shp))
+    if len(raw) != 8 * n * len(CHECKPOINT_FIELDS):
+        raise ValueError(f"{fname}: {len(raw)} bytes, expected "
+                         f"{8 * n * len(CHECKPOINT_FIELDS)} for this grid")
+    allv = np.frombuffer(raw, dtype="<f8")
+    for q, name in enumerate(CHECKPOINT_FIELDS):
+        a = case.download(name, member)
+        a[1:nz + 1] = allv[q * n:(q + 1) * n].reshape(shp)
+        if name in ("u", "v", "w"):
+            a[0] = -1234567890.0
+        case.upload(name, a, member)
+    jt_total, t = 0, 0.0
+    tt = os.path.join(path, "total_time.dat")
+    if os.path.exists(tt):
+        w = open(tt).read().split()
+        jt_total, t = int(w[0]), float(w[1])
+    if set_counters:
+        case.set_counters(jt_total, jt_total)
+    return jt_total, t

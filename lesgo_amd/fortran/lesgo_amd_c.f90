@@ -144,6 +144,11 @@ interface
         type(c_ptr), value :: c
         real(c_double), intent(out) :: out
     end function
+    integer(c_int) function lg_max_cfl(c, out) bind(C, name='lg_max_cfl')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: c
+        real(c_double), intent(out) :: out
+    end function
     integer(c_int) function lg_tau_wall_bot(c, out) bind(C, name='lg_tau_wall_bot')
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: c

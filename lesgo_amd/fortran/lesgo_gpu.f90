@@ -49,6 +49,7 @@ end subroutine gpu_init
 
 integer(c_int) function field_id(f)
 use sim_param
+use sgs_param
 real(rprec), dimension(:,:,0:), target, intent(in) :: f
 type(c_ptr) :: a
 a = c_loc(f(1,1,0))
@@ -78,6 +79,11 @@ if (c_associated(a, c_loc(divtx))) field_id = LG_DIVTX
 if (c_associated(a, c_loc(divty))) field_id = LG_DIVTY
 if (c_associated(a, c_loc(divtz))) field_id = LG_DIVTZ
 if (c_associated(a, c_loc(p)))     field_id = LG_P
+if (c_associated(a, c_loc(Cs_opt2))) field_id = LG_CS_OPT2
+if (c_associated(a, c_loc(F_LM)))  field_id = LG_F_LM
+if (c_associated(a, c_loc(F_MM)))  field_id = LG_F_MM
+if (c_associated(a, c_loc(F_QN)))  field_id = LG_F_QN
+if (c_associated(a, c_loc(F_NN)))  field_id = LG_F_NN
 if (field_id < 0) then
     write(*,*) 'lesgo_gpu: array is not a sim_param field'
     stop 1

@@ -37,6 +37,10 @@ integer :: precision = 0                     ! 0 FP64, 1 mixed
 integer :: seed = 0
 logical :: fused = .false.                   ! .true.: whole steps via lg_step
 character(len=256) :: path = './'
+! checkpoint (io.f90:1173-1267): vel.out, grid.out, total_time.dat
+logical :: checkpoint_data = .false.
+integer :: checkpoint_nskip = 10000
+character(len=64) :: checkpoint_file = 'vel.out'
 end module param
 
 !*******************************************************************************
@@ -69,3 +73,20 @@ allocate(p(ld,ny,0:nz))
 u = 0; v = 0; w = 0; p = 0
 end subroutine sim_param_init
 end module sim_param
+
+!*******************************************************************************
+module sgs_param
+!*******************************************************************************
+! Host mirrors of the checkpointed sgs_param arrays (sgs_param.f90:29-184).
+use types, only : rprec
+implicit none
+save
+real(rprec), dimension(:,:,:), allocatable, target :: Cs_opt2, F_LM, F_MM, F_QN, F_NN
+contains
+subroutine sgs_param_init()
+use param, only : ld, ny, nz, lbz
+allocate(Cs_opt2(ld,ny,lbz:nz), F_LM(ld,ny,lbz:nz), F_MM(ld,ny,lbz:nz))
+allocate(F_QN(ld,ny,lbz:nz), F_NN(ld,ny,lbz:nz))
+Cs_opt2 = 0; F_LM = 0; F_MM = 0; F_QN = 0; F_NN = 0
+end subroutine sgs_param_init
+end module sgs_param

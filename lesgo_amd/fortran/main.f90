@@ -11,6 +11,7 @@ use iso_c_binding
 use types, only : rprec
 use param
 use sim_param
+use sgs_param
 use lesgo_amd_c
 use lesgo_gpu
 use derivatives, only : filt_da, ddz_uv, ddz_w
@@ -18,9 +19,11 @@ use sgs_stag_util, only : sgs_stag
 use forcing, only : forcing_applied, project, turbines_on
 implicit none
 namelist /lesgo/ Nx, Ny, Nz, L_x, L_y, L_z, dt, zo, cs_count, mean_p_force_x, &
-    nsteps, wbase, domain_nskip, precision, seed, fused, path, turbines_on, initu, DYN_init
+    nsteps, wbase, domain_nskip, precision, seed, fused, path, turbines_on, initu, DYN_init, &
+    checkpoint_data, checkpoint_nskip, checkpoint_file
 character(len=512) :: nml
 integer :: jt_step, fid, ios, count0, count1, rate
+logical :: restart
 real(rprec) :: rms, ke, tw
 real(rprec), allocatable :: farm(:)
 
@@ -43,8 +46,17 @@ lh_big = nx2/2 + 1; ld_big = 2*lh_big
 dx = L_x/Nx; dy = L_y/Ny; dz = L_z/(nz_tot - 1)
 
 call sim_param_init()
+call sgs_param_init()
+! a checkpoint in path is read as the reference reads it (initial.f90:96-110,
+! ic_file 226-239): initu = .true.; otherwise the seeded start
+inquire(file=trim(path)//trim(checkpoint_file), exist=restart)
+if (restart) initu = .true.
 call gpu_init()
-call gpu_check(lg_seed_state(gpu_case, int(seed, c_long), 0.1_rprec), 'lg_seed_state')
+if (restart) then
+    call ic_file()
+else
+    call gpu_check(lg_seed_state(gpu_case, int(seed, c_long), 0.1_rprec), 'lg_seed_state')
+end if
 if (turbines_on) then
     call aligned_farm(farm)
     call gpu_check(lg_turbines_init(gpu_case, int(size(farm)/8, c_int), farm,      &
@@ -54,7 +66,7 @@ call system('mkdir -p ' // trim(path) // 'output')
 
 call system_clock(count0, rate)
 time_loop: do jt_step = 1, nsteps
-    jt_total = jt_step
+    jt_total = jt_total + 1
     jt = jt + 1
     total_time = total_time + dt
     if (fused) then
@@ -83,6 +95,7 @@ time_loop: do jt_step = 1, nsteps
         call project()
     end if
     if (modulo(jt_total, domain_nskip) == 0) call write_domain()
+    if (checkpoint_data .and. modulo(jt_total, checkpoint_nskip) == 0) call checkpoint()
     if (modulo(jt_total, wbase) == 0) then
         call rmsdiv(rms)
         call gpu_check(lg_energy(gpu_case, ke), 'energy')
@@ -95,12 +108,62 @@ time_loop: do jt_step = 1, nsteps
     end if
 end do time_loop
 call gpu_check(lg_synchronize(gpu_case), 'synchronize')
+if (checkpoint_data) call checkpoint()
 call system_clock(count1)
 write(*,"(a,e15.7)") 'Simulation wall time (s) : ', real(count1 - count0, rprec)/rate
 call gpu_check(lg_destroy(gpu_case), 'lg_destroy')
 write(*,'(a)') 'Simulation complete'
 
 contains
+
+subroutine ic_file()
+! vel.out (initial.f90:226-239) and total_time.dat (jt_total, total_time)
+integer :: un, jt0
+real(rprec) :: t0, tdim, dt0, cfl0
+open(newunit=un, file=trim(path)//trim(checkpoint_file), form='unformatted', status='old')
+read(un) u(:, :, 1:nz), v(:, :, 1:nz), w(:, :, 1:nz),                          &
+    RHSx(:, :, 1:nz), RHSy(:, :, 1:nz), RHSz(:, :, 1:nz),                      &
+    Cs_opt2(:,:,1:nz), F_LM(:,:,1:nz), F_MM(:,:,1:nz),                         &
+    F_QN(:,:,1:nz), F_NN(:,:,1:nz)
+close(un)
+u(:,:,0) = BOGUS; v(:,:,0) = BOGUS; w(:,:,0) = BOGUS
+call gpu_upload(u); call gpu_upload(v); call gpu_upload(w)
+call gpu_upload(RHSx); call gpu_upload(RHSy); call gpu_upload(RHSz)
+call gpu_upload(Cs_opt2); call gpu_upload(F_LM); call gpu_upload(F_MM)
+call gpu_upload(F_QN); call gpu_upload(F_NN)
+open(newunit=un, file=trim(path)//'total_time.dat', status='old', iostat=ios)
+if (ios == 0) then
+    read(un, *) jt0, t0, tdim, dt0, cfl0
+    close(un)
+    jt_total = jt0
+    total_time = t0
+end if
+call gpu_counters()
+end subroutine ic_file
+
+subroutine checkpoint()
+! io.f90:1173-1267: vel.out, grid.out, total_time.dat (one process)
+integer :: un
+real(rprec) :: cfl_w
+call gpu_download(u); call gpu_download(v); call gpu_download(w)
+call gpu_download(RHSx); call gpu_download(RHSy); call gpu_download(RHSz)
+call gpu_download(Cs_opt2); call gpu_download(F_LM); call gpu_download(F_MM)
+call gpu_download(F_QN); call gpu_download(F_NN)
+open(newunit=un, file=trim(path)//trim(checkpoint_file), form='unformatted',      &
+    status='replace')
+write(un) u(:, :, 1:nz), v(:, :, 1:nz), w(:, :, 1:nz),                         &
+    RHSx(:, :, 1:nz), RHSy(:, :, 1:nz), RHSz(:, :, 1:nz),                      &
+    Cs_opt2(:,:,1:nz), F_LM(:,:,1:nz), F_MM(:,:,1:nz),                         &
+    F_QN(:,:,1:nz), F_NN(:,:,1:nz)
+close(un)
+open(newunit=un, file=trim(path)//'grid.out', form='unformatted', status='replace')
+write(un) 1, Nx, Ny, Nz, L_x, L_y, L_z
+close(un)
+call gpu_check(lg_max_cfl(gpu_case, cfl_w), 'max_cfl')
+open(newunit=un, file=trim(path)//'total_time.dat', status='replace')
+write(un, *) jt_total, total_time, total_time, dt, cfl_w
+close(un)
+end subroutine checkpoint
 
 subroutine write_domain()
 ! io.f90:786-884 (binary branch): u, v, w on the uv grid; p - |u|^2/2

@@ -1,0 +1,74 @@
+"""The restart file format: Fortran unformatted sequential records written
+and read by lesgo_amd.checkpoint, checked against the Fortran compiler's own
+I/O (a small program built with amdflang reads the Python-written record and
+writes one Python reads back), and the subrecord layout of long records."""
+import io
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from lesgo_amd import checkpoint as ck
+
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+PROG = """
+program rw
+implicit none
+integer, parameter :: ld = 10, ny = 6, nz = 5
+real(8) :: a(ld, ny, 0:nz), b(ld, ny, 0:nz)
+open(12, file='in.rec', form='unformatted', status='old')
+read(12) a(:, :, 1:nz), b(:, :, 1:nz)
+close(12)
+write(*, '(2es25.16)') sum(a(:, :, 1:nz)), sum(b(:, :, 1:nz) * 2)
+open(13, file='out.rec', form='unformatted', status='replace')
+write(13) b(:, :, 1:nz) * 3, a(:, :, 1:nz)
+close(13)
+end program rw
+"""
+
+
+def test_subrecords_round_trip(monkeypatch):
+    monkeypatch.setattr(ck, "_MAXSUB", 100)
+    payload = bytes(range(256)) + bytes(94)       # 350 bytes -> 100, 100, 100, 50
+    f = io.BytesIO()
+    ck.write_record(f, payload)
+    raw = f.getvalue()
+    marks = []
+    pos = 0
+    while pos < len(raw):
+        lead = int.from_bytes(raw[pos:pos + 4], "little", signed=True)
+        n = abs(lead)
+        trail = int.from_bytes(raw[pos + 4 + n:pos + 8 + n], "little", signed=True)
+        marks.append((lead, trail))
+        pos += n + 8
+    assert marks == [(-100, 100), (-100, -100), (-100, -100), (50, -50)]
+    f.seek(0)
+    assert ck.read_record(f) == payload
+
+
+@pytest.mark.skipif(not os.path.exists(FC), reason="no Fortran compiler")
+def test_records_interoperate_with_fortran_io(tmp_path):
+    src = tmp_path / "rw.f90"
+    src.write_text(PROG)
+    exe = tmp_path / "rw"
+    subprocess.run([FC, "-O1", str(src), "-o", str(exe)], check=True, cwd=tmp_path,
+                   capture_output=True, timeout=120)
+    rng = np.random.default_rng(0)
+    ld, ny, nz = 10, 6, 5
+    a = rng.standard_normal((nz, ny, ld))          # Fortran a(:, :, 1:nz)
+    b = rng.standard_normal((nz, ny, ld))
+    with open(tmp_path / "in.rec", "wb") as f:
+        ck.write_record(f, a.astype("<f8").tobytes() + b.astype("<f8").tobytes())
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    sa, sb = (float(x) for x in r.stdout.split())
+    np.testing.assert_allclose([sa, sb], [a.sum(), 2 * b.sum()], rtol=1e-13)
+    with open(tmp_path / "out.rec", "rb") as f:
+        raw = np.frombuffer(ck.read_record(f), dtype="<f8")
+    n = a.size
+    np.testing.assert_array_equal(raw[:n].reshape(a.shape), 3 * b)
+    np.testing.assert_array_equal(raw[n:].reshape(a.shape), a)
+    shutil.rmtree(tmp_path, ignore_errors=True)
