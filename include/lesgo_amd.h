@@ -266,14 +266,18 @@ enum {
  * keyed by (kind, precision, N, other).  The table depends on the grid only,
  * so every case of a grid, alone or batched, then runs the same kernels. */
 int lg_autotune(lg_case* c, int reps, int* npasses);
-int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg);
-int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg);
+/* variant (row passes): 0 = rows read and written in place, 1 = the
+ * workgroup's rows staged through LDS with 16-byte vector copies. */
+int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg,
+                int* variant);
+int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg,
+                int variant);
 int lg_tune_clear(void);
 /* Enumerate the table (to save it and lg_tune_set it back in a later
  * process, as FFTW wisdom is kept). */
 int lg_tune_count(int* n);
 int lg_tune_entry(int i, int* kind, int* precision, int* N, int* other, int* n1, int* n2,
-                  int* per_wg);
+                  int* per_wg, int* variant);
 
 /* Device bytes lg_create_batch allocates for `members` members. */
 int lg_case_bytes(const lg_params* prm, int members, double* bytes);

@@ -93,11 +93,12 @@ SIGNATURES = {
     "lg_turbines_get_member": (C.c_int, [_P, C.c_int, _D, _D, C.c_int]),
     "lg_step_graph": (C.c_int, [_P, C.c_int]),
     "lg_autotune": (C.c_int, [_P, C.c_int, _I]),
-    "lg_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _I, _I, _I]),
-    "lg_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "lg_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _I, _I, _I, _I]),
+    "lg_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_int]),
     "lg_tune_clear": (C.c_int, []),
     "lg_tune_count": (C.c_int, [_I]),
-    "lg_tune_entry": (C.c_int, [C.c_int, _I, _I, _I, _I, _I, _I, _I]),
+    "lg_tune_entry": (C.c_int, [C.c_int, _I, _I, _I, _I, _I, _I, _I, _I]),
     "lg_case_bytes": (C.c_int, [C.POINTER(LgParams), C.c_int, _D]),
     "lg_stats_enable": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "lg_stats_info": (C.c_int, [_P, _D, _I]),

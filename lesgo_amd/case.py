@@ -311,17 +311,18 @@ class Case:
 
 
 def tune_get(kind: str, precision: str, N: int, other: int):
-    """(n1, n2, per_wg) launch choice the library uses for a pass."""
-    a, b, s = C.c_int(), C.c_int(), C.c_int()
+    """(n1, n2, per_wg, variant) launch choice the library uses for a pass."""
+    a, b, s, v = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     check(lib().lg_tune_get(_lib.PASS_KINDS.index(kind), LG_FP64 if precision == "fp64"
-                            else LG_MIXED, N, other, C.byref(a), C.byref(b), C.byref(s)),
-          "tune_get")
-    return a.value, b.value, s.value
+                            else LG_MIXED, N, other, C.byref(a), C.byref(b), C.byref(s),
+                            C.byref(v)), "tune_get")
+    return a.value, b.value, s.value, v.value
 
 
-def tune_set(kind: str, precision: str, N: int, other: int, n1: int, n2: int, per_wg: int):
+def tune_set(kind: str, precision: str, N: int, other: int, n1: int, n2: int, per_wg: int,
+             variant: int = 0):
     check(lib().lg_tune_set(_lib.PASS_KINDS.index(kind), LG_FP64 if precision == "fp64"
-                            else LG_MIXED, N, other, n1, n2, per_wg), "tune_set")
+                            else LG_MIXED, N, other, n1, n2, per_wg, variant), "tune_set")
 
 
 def tune_clear():
@@ -334,11 +335,11 @@ def tune_table():
     check(lib().lg_tune_count(C.byref(n)), "tune_count")
     out = []
     for i in range(n.value):
-        v = [C.c_int() for _ in range(7)]
+        v = [C.c_int() for _ in range(8)]
         check(lib().lg_tune_entry(i, *[C.byref(x) for x in v]), "tune_entry")
-        k, pr, N, o, a, b, s = (x.value for x in v)
+        k, pr, N, o, a, b, s, var = (x.value for x in v)
         out.append({"kind": _lib.PASS_KINDS[k], "precision": "fp64" if pr == LG_FP64 else "mixed",
-                    "N": N, "other": o, "n1": a, "n2": b, "per_wg": s})
+                    "N": N, "other": o, "n1": a, "n2": b, "per_wg": s, "variant": var})
     return out
 
 
@@ -354,7 +355,8 @@ def tune_load(path: str) -> int:
     with open(path) as f:
         rows = json.load(f)
     for r in rows:
-        tune_set(r["kind"], r["precision"], r["N"], r["other"], r["n1"], r["n2"], r["per_wg"])
+        tune_set(r["kind"], r["precision"], r["N"], r["other"], r["n1"], r["n2"], r["per_wg"],
+                 r.get("variant", 0))
     return len(rows)
 
 

@@ -424,7 +424,8 @@ int lg_autotune(lg_case* h, int reps, int* npasses) {
     });
 }
 
-int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg) {
+int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int* per_wg,
+                int* variant) {
     return guarded([&] {
         REQUIRE(kind >= 0 && kind < lg::PK_NKINDS, "pass kind out of range");
         REQUIRE(precision == LG_FP64 || precision == LG_MIXED, "bad precision");
@@ -436,14 +437,16 @@ int lg_tune_get(int kind, int precision, int N, int other, int* n1, int* n2, int
         if (n1) *n1 = c.n1;
         if (n2) *n2 = c.n2;
         if (per_wg) *per_wg = c.S;
+        if (variant) *variant = c.v;
     });
 }
 
-int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg) {
+int lg_tune_set(int kind, int precision, int N, int other, int n1, int n2, int per_wg,
+                int variant) {
     return guarded([&] {
         REQUIRE(kind >= 0 && kind < lg::PK_NKINDS, "pass kind out of range");
         REQUIRE(precision == LG_FP64 || precision == LG_MIXED, "bad precision");
-        const lg::PassChoice c{n1, n2, per_wg};
+        const lg::PassChoice c{n1, n2, per_wg, variant};
         const bool ok = precision == LG_FP64 ? lg::choice_ok<double>(kind, c, N, other)
                                              : lg::choice_ok<float>(kind, c, N, other);
         REQUIRE(ok, "launch choice not launchable for this pass");
@@ -463,7 +466,7 @@ int lg_tune_count(int* n) {
 }
 
 int lg_tune_entry(int i, int* kind, int* precision, int* N, int* other, int* n1, int* n2,
-                  int* per_wg) {
+                  int* per_wg, int* variant) {
     return guarded([&] {
         int key[4];
         lg::PassChoice c;
@@ -475,6 +478,7 @@ int lg_tune_entry(int i, int* kind, int* precision, int* N, int* other, int* n1,
         if (n1) *n1 = c.n1;
         if (n2) *n2 = c.n2;
         if (per_wg) *per_wg = c.S;
+        if (variant) *variant = c.v;
     });
 }
 

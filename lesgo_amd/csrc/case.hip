@@ -889,7 +889,7 @@ void Case<T>::press_stag_array() {
     const PassChoice ch = choice_for<T>(PK_TRIDAG, d.ny, d.NK);
     const int tb = ch.S;
     timer.run(stream, KK_TRIDAG, tbytes, [&] {
-        hipLaunchKernelGGL(k_tridag<T>, dim3((nsys + tb - 1) / tb), dim3(tb), 0, stream,
+        LG_LAUNCH(k_tridag<T>, dim3((nsys + tb - 1) / tb), dim3(tb), 0, stream,
                            (const cplx<T>*)spec[0], (const cplx<T>*)spec[1],
                            (const cplx<T>*)spec[2], (const cplx<T>*)spec[3],
                            (const cplx<T>*)(spec[3] + d.splane), spec[4], gam, usc, tg, dev_err);

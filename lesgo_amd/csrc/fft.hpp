@@ -34,6 +34,50 @@ struct FftPlan {
     const float2* tw32 = nullptr;
 };
 
+// Global loads and stores of the transform passes (non-temporal when
+// LG_NT is set at build time).
+#ifndef LG_NT
+#define LG_NT 0
+#endif
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+#if LG_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> gld(const cplx<T>* p) {
+#if LG_NT
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return cplx<T>{v.x, v.y};
+#else
+    return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+#if LG_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void gst(cplx<T>* p, cplx<T> c) {
+#if LG_NT
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 v;
+    v.x = c.x;
+    v.y = c.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+#else
+    *p = c;
+#endif
+}
+
 template <typename T> struct TwT;
 template <> struct TwT<double> { using type = double2; };
 template <> struct TwT<float> { using type = float2; };

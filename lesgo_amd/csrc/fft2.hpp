@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_rows_r2c_v2(RowR2CArgs<T> a) {
         const T* rb = ra + a.src_ld;
         cplx<T> x[N1];
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) x[n1] = cplx<T>{sc * ra[n2 + N2 * n1], sc * rb[n2 + N2 * n1]};
+        for (int n1 = 0; n1 < N1; ++n1) x[n1] = cplx<T>{sc * gld(ra + n2 + N2 * n1), sc * gld(rb + n2 + N2 * n1)};
         dft_reg<T, N1, -1>(x);
         cplx<T>* Y = lds + s * SP::YS;
 #pragma unroll
@@ -175,8 +175,8 @@ __global__ void __launch_bounds__(256) k_rows_r2c_v2(RowR2CArgs<T> a) {
         const cplx<T> zc = X[(N - k) % N];
         const cplx<T> A{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y - zc.y)};
         const cplx<T> Bm{(T)0.5 * (z.x - zc.x), (T)0.5 * (z.y + zc.y)};
-        dst[(long)j * nc + k] = A;
-        dst[(long)(j + 1) * nc + k] = cplx<T>{Bm.y, -Bm.x};
+        gst(dst + (long)j * nc + k, A);
+        gst(dst + (long)(j + 1) * nc + k, cplx<T>{Bm.y, -Bm.x});
     }
 }
 
@@ -211,16 +211,16 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
             const int k = n2 + N2 * n1;
             cplx<T> z{(T)0, (T)0};
             if (k == 0) {
-                z = cplx<T>{ra[0].x, rb[0].x};
+                z = cplx<T>{gld(ra).x, gld(rb).x};
             } else if (k < half) {
                 if (k < nc) {
-                    const cplx<T> A = ra[k], B = rb[k];
+                    const cplx<T> A = gld(ra + k), B = gld(rb + k);
                     z = cplx<T>{A.x - B.y, A.y + B.x};
                 }
             } else if (k > half) {
                 const int kk = N - k;
                 if (kk < nc) {
-                    const cplx<T> A = ra[kk], B = rb[kk];
+                    const cplx<T> A = gld(ra + kk), B = gld(rb + kk);
                     z = cplx<T>{A.x + B.y, B.x - A.y};
                 }
             }
@@ -246,8 +246,8 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
         T* ob = oa + ld;
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
-            oa[k1 + N1 * k2] = y[k2].x;
-            ob[k1 + N1 * k2] = y[k2].y;
+            gst(oa + k1 + N1 * k2, y[k2].x);
+            gst(ob + k1 + N1 * k2, y[k2].y);
         }
     }
     // padding columns N..ld-1
@@ -294,13 +294,13 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 int j = -1;
                 if (jb < nyh) j = jb;
                 else if (jb > N - nyh) j = jb - (N - ny);
-                if (j >= 0) v = apply_op<T>(op, src[(long)j * a.ncols + c0 + c], c0 + c, j, a);
+                if (j >= 0) v = apply_op<T>(op, gld(src + (long)j * a.ncols + c0 + c), c0 + c, j, a);
             } else {
-                v = src[(long)jb * a.ncols + c0 + c];
+                v = gld(src + (long)jb * a.ncols + c0 + c);
                 if (DIR > 0 && a.map == MAP_ID) {
                     if (op == OP_DIVXY) {
                         const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
-                        v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);
+                        v = divxy<T>(v, gld(s2 + (long)jb * a.ncols + c0 + c), c0 + c, jb, a);
                     } else {
                         v = apply_op<T>(op, v, c0 + c, jb, a);
                     }
@@ -333,13 +333,13 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 else if (jb > N - nyh) j = jb - (N - ny);
                 else continue;
                 if (DIR < 0) v = apply_op<T>(op, v, c0 + c, j, a);
-                dst[(long)j * a.ncols + c0 + c] = v;
+                gst(dst + (long)j * a.ncols + c0 + c, v);
             } else {
                 if (DIR < 0 && a.map == MAP_ID) {
                     if (a.zero_nyq && jb == nyh) v = cplx<T>{(T)0, (T)0};
                     v = apply_op<T>(op, v, c0 + c, jb, a);
                 }
-                dst[(long)jb * a.ncols + c0 + c] = v;
+                gst(dst + (long)jb * a.ncols + c0 + c, v);
             }
         }
     }

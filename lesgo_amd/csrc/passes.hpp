@@ -20,6 +20,7 @@ constexpr size_t LDS_DEFAULT = 64 * 1024;
 // generic LDS kernel) and the rows (pairs) or columns per workgroup.
 struct PassChoice {
     int n1 = 0, n2 = 0, S = 0;
+    int v = 0;          // rows: 1 = the block's rows staged through LDS (fft3.hpp)
 };
 
 // Pass kinds of the tuning table (column passes by direction and map).
@@ -42,6 +43,17 @@ inline int cols_kind(int dir, int map) {
 
 // LDS complex elements per sequence (row or column) of a pass choice.
 inline int seq_stride(const PassChoice& c, int N) { return c.n1 ? c.n1 * (c.n2 + 1) + 1 : N + 1; }
+
+// LDS bytes of a row pass choice (ld_real: the real rows' stride).
+template <typename T>
+inline size_t row_lds_bytes(const PassChoice& c, int N, int ld_real, int ncols) {
+    size_t b = (size_t)c.S * seq_stride(c, N) * sizeof(cplx<T>);
+    if (c.v == 1) {
+        const int w = ld_real > 2 * ncols ? ld_real : 2 * ncols;
+        b += ((2ull * c.S * w * sizeof(T) + 15) & ~15ull);
+    }
+    return b;
+}
 
 // Process-wide table of tuned launch choices keyed by (kind, element size,
 // transform length, rows or columns).  Choices depend on the grid only,

@@ -5,7 +5,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "fft2.hpp"
+#include "fft3.hpp"
 #include "passes.hpp"
 
 namespace lg {
@@ -34,7 +34,7 @@ template <typename T>
 static bool v2_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        hipLaunchKernelGGL((k_rows_r2c_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        LG_LAUNCH((k_rows_r2c_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -46,7 +46,31 @@ template <typename T>
 static bool v2_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        hipLaunchKernelGGL((k_rows_c2r_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        LG_LAUNCH((k_rows_c2r_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+template <typename T>
+static bool v3_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        LG_LAUNCH((k_rows_r2c_v3<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
+template <typename T>
+static bool v3_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        LG_LAUNCH((k_rows_c2r_v3<T, A, B>), g, dim3(PBLK), lds, s, a);           \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -58,7 +82,7 @@ template <typename T, int DIR>
 static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        hipLaunchKernelGGL((k_cols_v2<T, A, B, DIR>), g, dim3(PBLK), lds, s, a);          \
+        LG_LAUNCH((k_cols_v2<T, A, B, DIR>), g, dim3(PBLK), lds, s, a);          \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -80,7 +104,9 @@ bool choice_ok(int kind, const PassChoice& c, int N, int other) {
         for (auto& p : splits_for(N)) known |= (p.first == c.n1 && p.second == c.n2);
         if (!known) return false;
     }
-    const size_t lds = (size_t)c.S * seq_stride(c, N) * sizeof(cplx<T>);
+    if (c.v != 0 && !(c.v == 1 && c.n1 && (kind == PK_R2C || kind == PK_C2R))) return false;
+    // real rows of a length-N row pass have stride N + 2; ncols <= N / 2
+    const size_t lds = row_lds_bytes<T>(c, N, N + 2, N / 2);
     if (lds > LDS_MAX) return false;
     if (kind == PK_R2C || kind == PK_C2R) {
         if (c.S > std::max(1, other / 2)) return false;
@@ -157,13 +183,14 @@ void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* con
         a.dst_mstride = dst_ms;
         a.S = ch.S;
         a.stride = seq_stride(ch, pl.N);
-        const size_t lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        const size_t lds = row_lds_bytes<T>(ch, pl.N, src_ld, ncols);
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
         const double bytes = (double)nj * nlev * M * nrows *
                              ((double)pl.N * sizeof(T) + (double)ncols * sizeof(cplx<T>));
         tm.run(s, KK_ROWS_R2C, bytes, [&] {
+            if (ch.n1 && ch.v == 1 && v3_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (!(ch.n1 && v2_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)))
-                hipLaunchKernelGGL(k_rows_r2c<T>, grid, dim3(PBLK), lds, s, a);
+                LG_LAUNCH(k_rows_r2c<T>, grid, dim3(PBLK), lds, s, a);
         });
     }
 }
@@ -194,13 +221,14 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
         a.dst_mstride = dst_ms;
         a.S = ch.S;
         a.stride = seq_stride(ch, pl.N);
-        const size_t lds = (size_t)a.S * a.stride * sizeof(cplx<T>);
+        const size_t lds = row_lds_bytes<T>(ch, pl.N, dst_ld, ncols);
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
         const double bytes = (double)nj * nlev * M * nrows *
                              ((double)ncols * sizeof(cplx<T>) + (double)dst_ld * sizeof(T));
         tm.run(s, KK_ROWS_C2R, bytes, [&] {
+            if (ch.n1 && ch.v == 1 && v3_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (!(ch.n1 && v2_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)))
-                hipLaunchKernelGGL(k_rows_c2r<T>, grid, dim3(PBLK), lds, s, a);
+                LG_LAUNCH(k_rows_c2r<T>, grid, dim3(PBLK), lds, s, a);
         });
     }
 }
@@ -220,7 +248,7 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
     const double bytes = (double)nj * nlev * M * c.ncols * sizeof(cplx<T>) * (ny_in + N);
     tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
         if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
-            hipLaunchKernelGGL((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
+            LG_LAUNCH((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
     });
 }
 

@@ -1,12 +1,31 @@
-// Optional per-kernel-class timing with HIP events recorded on the case's
-// stream around each launch (used by bench.py for the roofline figure).
+// Optional per-kernel-class timing (used by bench.py for the roofline
+// figure).  A timed launch goes through hipExtLaunchKernelGGL with a
+// start/stop event pair, which the dispatch itself records, so the interval
+// is the kernel's execution, not the queue gaps around it.
 #pragma once
+
+#include <hip/hip_ext.h>
 
 #include <vector>
 
 #include "common.hpp"
 
 namespace lg {
+
+// events for the next kernel launch made through LG_LAUNCH (none: untimed)
+struct LaunchEvents {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+inline thread_local LaunchEvents g_launch_events;
+
+#define LG_LAUNCH(K, G, B, L, S, ...)                                                         \
+    do {                                                                                      \
+        if (::lg::g_launch_events.a)                                                          \
+            hipExtLaunchKernelGGL(K, G, B, L, S, ::lg::g_launch_events.a,                     \
+                                  ::lg::g_launch_events.b, 0, __VA_ARGS__);                   \
+        else                                                                                  \
+            hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                   \
+    } while (0)
 
 enum KernelKind : int {
     KK_ROWS_R2C = 0,
@@ -44,9 +63,9 @@ struct KernelTimer {
             return;
         }
         hipEvent_t a = get(), b = get();
-        LG_HIP(hipEventRecord(a, s));
+        g_launch_events = LaunchEvents{a, b};
         launch();
-        LG_HIP(hipEventRecord(b, s));
+        g_launch_events = LaunchEvents{};
         recs.push_back({kind, a, b, bytes});
     }
 

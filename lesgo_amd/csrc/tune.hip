@@ -69,11 +69,13 @@ std::vector<PassChoice> candidates(int kind, int N, int other) {
     }
     std::vector<std::pair<int, int>> sp = pass_splits(N);
     sp.insert(sp.begin(), {0, 0});
+    const bool rows = kind == PK_R2C || kind == PK_C2R;
     for (auto& s : sp)
-        for (int S : {1, 2, 4, 8, 12, 16, 24, 32, 48, 64}) {
-            PassChoice c{s.first, s.second, S};
-            if (choice_ok<T>(kind, c, N, other)) out.push_back(c);
-        }
+        for (int S : {1, 2, 4, 8, 12, 16, 24, 32, 48, 64})
+            for (int v = 0; v <= (rows && s.first ? 1 : 0); ++v) {
+                PassChoice c{s.first, s.second, S, v};
+                if (choice_ok<T>(kind, c, N, other)) out.push_back(c);
+            }
     return out;
 }
 
