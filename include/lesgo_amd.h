@@ -52,8 +52,8 @@ enum {
  * Run parameters (param.f90).  Nz is the number of uv levels in the domain;
  * the one-rank layout has nz = Nz + 1 levels above the ghost level 0
  * (input_util.f90:196-235).  Physics: sgs_model = 5, lbc_mom = 2 (equilibrium
- * wall model), ubc_mom = 0 (stress-free lid), static dt with AB2
- * (tadv1 = 1.5), inflow_type = 0.
+ * wall model), ubc_mom = 0 (stress-free lid), AB2 with a static dt
+ * (tadv1 = 1.5) or a CFL-adaptive one (use_cfl_dt), inflow_type = 0.
  */
 typedef struct {
     int nx, ny, Nz;
@@ -74,6 +74,14 @@ typedef struct {
      * Counted per member from its own first step. */
     int initu;
     int dyn_init;           /* DYN_init (param.f90:123, default 100) */
+    /* use_cfl_dt (param.f90:151-152): 1 = each step takes dt = cfl / max(
+     * max|u|/dx, max|v|/dy, max|w|/dz) on the device before it starts, with
+     * tadv1 = 1 + dt / (2 dt_prev), tadv2 = 1 - tadv1 (cfl_util.f90:73-111,
+     * main.f90:135-144); a member whose clock holds no previous step takes
+     * an Euler step (initialize.f90:191-200).  dt above is then unused
+     * except as lg_set_clock_member's starting value. */
+    int use_cfl_dt;
+    double cfl;
 } lg_params;
 
 typedef struct lg_case lg_case;
@@ -100,6 +108,12 @@ int lg_seed_state(lg_case* c, long seed, double noise);
 int lg_set_counters(lg_case* c, int jt, int jt_total);
 int lg_get_counters(const lg_case* c, int* jt, int* jt_total);
 int lg_synchronize(lg_case* c);
+/* The head of the time loop (main.f90:135-151) for callers that step through
+ * the subsystem entry points: with use_cfl_dt, dt = get_cfl_dt() and the AB2
+ * weights of every stepped member, on the device; total_time += dt.  Call
+ * once per step, after lg_set_counters has advanced the counters (lg_step
+ * does this itself). */
+int lg_advance_clock(lg_case* c);
 
 /* --- replaced subroutines (device state in, device state out) ---------- */
 /* derivatives::filt_da (derivatives.f90:166-211), levels 0..nz */
@@ -211,6 +225,17 @@ int lg_copy_member(lg_case* c, int src, int dst);
 int lg_rmsdiv_member(lg_case* c, int m, double* out);
 int lg_energy_member(lg_case* c, int m, double* out);
 int lg_max_cfl_member(lg_case* c, int m, double* out);
+
+/* Time-stepping clock of member m (param: dt, tadv1, tadv2; sgs_param:
+ * lagran_dt; total_time, main.f90:149): out[0..4] = dt, tadv1, tadv2,
+ * lagran_dt, total_time after the last step (with use_cfl_dt, dt is the
+ * last step's; lagran_dt the displacement time pending for the next
+ * dynamic call, sgs_stag_util.f90:74-82, lagrange_Sdep.f90:430). */
+int lg_clock_member(lg_case* c, int m, double* out);
+/* Restart a member's clock from total_time.dat (io.f90:62-95): total_time,
+ * and with use_cfl_dt the previous step's dt; fresh != 0 makes the next step
+ * an Euler step (jt_total = 0 or a changed cfl, initialize.f90:191-200). */
+int lg_set_clock_member(lg_case* c, int m, double dt, double total_time, int fresh);
 int lg_tau_wall_bot_member(lg_case* c, int m, double* out);
 /* Replace member m's disks (nturb = 0 removes them). */
 int lg_turbines_init_member(lg_case* c, int m, int nturb, const double* spec8,
@@ -236,6 +261,8 @@ enum {
     LG_TAVG_VORTX, LG_TAVG_VORTY, LG_TAVG_VORTZ, LG_TAVG_N
 };
 int lg_stats_enable(lg_case* c, int nstart, int nend, int nskip);
+/* total_time: the averaging time so far (member 0's with use_cfl_dt, where
+ * every member weighs by its own dt) */
 int lg_stats_info(lg_case* c, double* total_time, int* initialized);
 /* Time average of one quantity of member m, host double [nz+1][ny][nx]. */
 int lg_stats_mean(lg_case* c, int m, int quantity, double* host);

@@ -29,7 +29,7 @@ class LgParams(C.Structure):
                 ("cs_count", C.c_int), ("use_mean_p_force", C.c_int),
                 ("mean_p_force_x", C.c_double), ("mean_p_force_y", C.c_double),
                 ("nu", C.c_double), ("precision", C.c_int), ("initu", C.c_int),
-                ("dyn_init", C.c_int)]
+                ("dyn_init", C.c_int), ("use_cfl_dt", C.c_int), ("cfl", C.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -88,6 +88,9 @@ SIGNATURES = {
     "lg_energy_member": (C.c_int, [_P, C.c_int, _D]),
     "lg_max_cfl_member": (C.c_int, [_P, C.c_int, _D]),
     "lg_tau_wall_bot_member": (C.c_int, [_P, C.c_int, _D]),
+    "lg_clock_member": (C.c_int, [_P, C.c_int, _D]),
+    "lg_advance_clock": (C.c_int, [_P]),
+    "lg_set_clock_member": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, C.c_int]),
     "lg_turbines_init_member": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_double, C.c_double,
                                           C.c_double, C.c_int]),
     "lg_turbines_get_member": (C.c_int, [_P, C.c_int, _D, _D, C.c_int]),

@@ -46,6 +46,10 @@ class Config:
     # fresh start (Euler first step, Cs_opt2 = 0.03 until dyn_init steps)
     initu: bool = True
     dyn_init: int = 100
+    # use_cfl_dt: dt = cfl / max(|u|/dx, |v|/dy, |w|/dz) each step, on the
+    # device (cfl_util.f90:73-111); dt above is then only the starting value
+    use_cfl_dt: bool = False
+    cfl: float = 0.05
 
     def to_c(self) -> LgParams:
         return LgParams(self.nx, self.ny, self.Nz, self.lx, self.ly, self.lz,
@@ -53,7 +57,8 @@ class Config:
                         int(self.use_mean_p_force), self.mean_p_force_x,
                         self.mean_p_force_y, self.nu,
                         LG_FP64 if self.precision == "fp64" else LG_MIXED,
-                        int(self.initu), self.dyn_init)
+                        int(self.initu), self.dyn_init, int(self.use_cfl_dt),
+                        self.cfl)
 
 
 def _fid(f) -> int:
@@ -237,6 +242,19 @@ class Case:
 
     def tau_wall_bot(self, member: int = 0):
         return self._scalar(lib().lg_tau_wall_bot_member, member, "tau_wall_bot")
+
+    def clock(self, member: int = 0) -> dict:
+        """dt, tadv1, tadv2, lagran_dt and total_time of a member after its
+        last step (with use_cfl_dt each member has its own)."""
+        out = (C.c_double * 5)()
+        check(lib().lg_clock_member(self._h, member, out), "clock")
+        return dict(zip(("dt", "tadv1", "tadv2", "lagran_dt", "total_time"), list(out)))
+
+    def set_clock(self, dt: float, total_time: float, fresh: bool = False, member: int = 0):
+        """Restart a member's clock (total_time.dat): with use_cfl_dt, dt is
+        the previous step's and fresh makes the next step an Euler step."""
+        check(lib().lg_set_clock_member(self._h, member, dt, total_time, int(fresh)),
+              "set_clock")
 
     # -- running time averages (time_average.f90) and spectra ---------------
     def stats_enable(self, nstart: int, nend: int, nskip: int = 1):

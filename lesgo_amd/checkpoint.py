@@ -6,7 +6,9 @@ RHSx, RHSy, RHSz, Cs_opt2, F_LM, F_MM, F_QN, F_NN, each ``(ld, ny, 1:nz)``
 float64), ``grid.out`` (nproc, Nx, Ny, Nz, L_x, L_y, L_z) and
 ``total_time.dat`` (jt_total, total_time, total_time_dim, dt, cfl).
 ``read_checkpoint`` is ``ic_file`` (initial.f90:226-239) plus the
-``total_time.dat`` counters.  Unformatted records use 4-byte little-endian
+``total_time.dat`` counters and clock (io.f90:62-95): with use_cfl_dt the
+file's dt is the previous step's, and a new run (jt_total = 0) or a cfl that
+changed by more than 1 % restarts with an Euler step (initialize.f90:191-200).  Unformatted records use 4-byte little-endian
 length markers; a record longer than 2**31 - 9 bytes is split into
 subrecords whose markers are negative except the last leading and the
 first trailing one (gfortran's convention).
@@ -64,10 +66,13 @@ def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = 
         write_record(f, b"".join(a.astype("<f8").tobytes() for a in parts))
     with open(os.path.join(path, "grid.out"), "wb") as f:
         write_record(f, struct.pack("<4i3d", 1, cfg.nx, cfg.ny, cfg.Nz, cfg.lx, cfg.ly, cfg.lz))
-    cfl = case.max_cfl(member)
-    t = jt_total * cfg.dt
+    # the cfl written is the target with use_cfl_dt, the measured one
+    # otherwise (io.f90:1232-1238)
+    clk = case.clock(member)
+    cfl = cfg.cfl if cfg.use_cfl_dt else case.max_cfl(member)
+    t, dt = clk["total_time"], clk["dt"]
     with open(os.path.join(path, "total_time.dat"), "w") as f:
-        f.write(f" {jt_total:d} {t!r} {t!r} {cfg.dt!r} {cfl!r}\n")
+        f.write(f" {jt_total:d} {t!r} {t!r} {dt!r} {cfl!r}\n")
 
 
 def read_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "vel.out",
@@ -93,11 +98,14 @@ def read_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "
         if name in ("u", "v", "w"):
             a[0] = -1234567890.0
         case.upload(name, a, member)
-    jt_total, t = 0, 0.0
+    cfg = case.cfg
+    jt_total, t, dt_r, cfl_r = 0, 0.0, cfg.dt, cfg.cfl
     tt = os.path.join(path, "total_time.dat")
     if os.path.exists(tt):
         w = open(tt).read().split()
-        jt_total, t = int(w[0]), float(w[1])
+        jt_total, t, dt_r, cfl_r = int(w[0]), float(w[1]), float(w[3]), float(w[4])
+    fresh = cfg.use_cfl_dt and (jt_total == 0 or abs((cfl_r - cfg.cfl) / cfg.cfl) > 1e-2)
+    case.set_clock(dt_r if cfg.use_cfl_dt else cfg.dt, t, fresh, member)
     if set_counters:
         case.set_counters(jt_total, jt_total)
     return jt_total, t

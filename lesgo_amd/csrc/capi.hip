@@ -96,6 +96,7 @@ int lg_create_batch(const lg_params* p, int members, lg_case** out) {
                 "lengths, dt and zo must be positive");
         REQUIRE(p->cs_count >= 1, "cs_count must be >= 1");
         REQUIRE(p->precision == LG_FP64 || p->precision == LG_MIXED, "bad precision");
+        REQUIRE(!p->use_cfl_dt || p->cfl > 0, "cfl must be > 0 with use_cfl_dt");
         auto* h = new lg_case();
         h->precision = p->precision;
         try {
@@ -195,6 +196,15 @@ int lg_set_counters(lg_case* h, int jt, int jt_total) {
         c.jt = jt;
         c.jt_total = jt_total;
         for (int m = 0; m < c.M; ++m) c.mjt[m] = jt;
+    });
+}
+
+int lg_advance_clock(lg_case* h) {
+    return dispatch(h, [&](auto& c) {
+        c.refresh_flags();
+        c.advance_clocks();
+        c.clock_begin();
+        LG_HIP(hipGetLastError());
     });
 }
 
@@ -340,6 +350,22 @@ LG_DIAG(lg_energy, energy)
 LG_DIAG(lg_max_cfl, max_cfl)
 LG_DIAG(lg_tau_wall_bot, tau_wall_bot)
 #undef LG_DIAG
+
+int lg_clock_member(lg_case* h, int m, double* out) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(out, "null output");
+        check_member(c, m);
+        c.clock_get(m, out);
+    });
+}
+
+int lg_set_clock_member(lg_case* h, int m, double dt, double total_time, int fresh) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(dt > 0 && std::isfinite(total_time), "dt must be > 0 and total_time finite");
+        c.clock_set(m, dt, total_time, fresh != 0);
+    });
+}
 
 int lg_turbines_init_member(lg_case* h, int m, int nturb, const double* spec8, double alpha1,
                             double alpha2, double filter_cutoff, int adm_correction) {
@@ -491,7 +517,7 @@ int lg_stats_enable(lg_case* h, int nstart, int nend, int nskip) {
 
 int lg_stats_info(lg_case* h, double* total_time, int* initialized) {
     return dispatch(h, [&](auto& c) {
-        if (total_time) *total_time = c.st.total_time;
+        if (total_time) *total_time = c.stats_time(0);
         if (initialized) *initialized = c.st.initialized ? 1 : 0;
     });
 }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "case.hpp"
+#include "clock.hpp"
 #include "passes.hpp"
 #include "lagrange.hpp"
 #include "pointwise.hpp"
@@ -157,9 +158,15 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     mturb.resize(M);
     mjt.assign(M, 0);
     lm_init.assign(M, 0);
-    hflags.assign(3 * M, 0);
-    dflags_now.assign(3 * M, 0);
-    d_flags = dalloc<int>(3L * M);
+    hflags.assign(5 * M, 0);
+    dflags_now.assign(5 * M, 0);
+    d_flags = dalloc<int>(5L * M);
+    cfl_mode = p.use_cfl_dt != 0;
+    d_clock = dalloc<double>((long)CK_N * M);
+    d_cmax = dalloc<unsigned long long>(3L * M);
+    mtime.assign(M, 0.0);
+    clk_fresh.assign(M, 1);
+    for (int m = 0; m < M; ++m) clock_reset(m);
     px.build(d.nx);
     py.build(d.ny);
     pbx.build(d.nx2);
@@ -215,6 +222,8 @@ Case<T>::~Case() {
     fr(d_denom);
     fr(d_mpf);
     fr(d_flags);
+    fr(d_clock);
+    fr(d_cmax);
     fr(tavg);
     fr(tspec);
     fr(G1);
@@ -276,6 +285,75 @@ void Case<T>::seed_member(int m, long seed, double noise) {
     LG_HIP(hipStreamSynchronize(stream));
     mjt[m] = 0;
     lm_init[m] = 0;
+    clock_reset(m);
+}
+
+// ---------------------------------------------------------------------------
+// Time-stepping clock (param: dt, tadv1, tadv2; sgs_param: lagran_dt)
+// ---------------------------------------------------------------------------
+// A member's clock before its first step: the run's dt with the static AB2
+// weights (input_util.f90:403-409), lagran_dt = cs_count dt for a fixed dt
+// and 0 for use_cfl_dt (sgs_param.f90:51), total_time 0.
+template <typename T>
+void Case<T>::clock_reset(int m) {
+    const double dt = d.prm.dt;
+    double c[CK_N] = {};
+    c[CK_DT] = dt;
+    c[CK_TADV1] = d.tadv1;
+    c[CK_TADV2] = d.tadv2;
+    c[CK_LAGDT] = cfl_mode ? 0.0 : d.lagran_dt;
+    c[CK_PSCALE] = 1.0 / (d.nx * d.ny) / d.tadv1 / dt;
+    LG_HIP(hipMemcpyAsync(d_clock + (long)CK_N * m, c, sizeof(c), hipMemcpyHostToDevice, stream));
+    LG_HIP(hipStreamSynchronize(stream));
+    mtime[m] = 0.0;
+    clk_fresh[m] = 1;
+}
+
+template <typename T>
+void Case<T>::clock_get(int m, double* out5) {
+    if (!cfl_mode) {
+        out5[0] = d.prm.dt;
+        out5[1] = d.tadv1;
+        out5[2] = d.tadv2;
+        out5[3] = d.lagran_dt;
+        out5[4] = mtime[m];
+        return;
+    }
+    double c[CK_N];
+    LG_HIP(hipMemcpyAsync(c, d_clock + (long)CK_N * m, sizeof(c), hipMemcpyDeviceToHost, stream));
+    LG_HIP(hipStreamSynchronize(stream));
+    for (int q = 0; q < 5; ++q) out5[q] = c[q];
+}
+
+// Restart (io.f90:62-95): total_time always; with use_cfl_dt the previous
+// step's dt (the next step's dt_f) and whether the next step is Euler.
+template <typename T>
+void Case<T>::clock_set(int m, double dt, double total_time, bool fresh) {
+    mtime[m] = total_time;
+    if (!cfl_mode) return;
+    double c[CK_N];
+    LG_HIP(hipMemcpyAsync(c, d_clock + (long)CK_N * m, sizeof(c), hipMemcpyDeviceToHost, stream));
+    LG_HIP(hipStreamSynchronize(stream));
+    c[CK_DT] = dt;
+    c[CK_TIME] = total_time;
+    LG_HIP(hipMemcpyAsync(d_clock + (long)CK_N * m, c, sizeof(c), hipMemcpyHostToDevice, stream));
+    LG_HIP(hipStreamSynchronize(stream));
+    clk_fresh[m] = fresh ? 1 : 0;
+}
+
+// The use_cfl_dt head of a step: the velocity maxima of every stepped member,
+// then the clock update (clock.hpp).
+template <typename T>
+void Case<T>::clock_begin() {
+    if (!cfl_mode) return;
+    const long n = (long)(d.nz - 1) * d.ny * d.nx;
+    const int nb = (int)std::max(1L, std::min(1024L, n / (BLK * 32L)));
+    hipLaunchKernelGGL(k_cfl_max<T>, dim3(nb, Ma), dim3(BLK), 0, stream, (const T*)fld[LG_U],
+                       (const T*)fld[LG_V], (const T*)fld[LG_W], d.nx, d.ny, d.ld, d.plane,
+                       d.field, d.nz, d_cmax);
+    hipLaunchKernelGGL(k_clock_step, dim3(1), dim3(64), 0, stream, d_clock, d_cmax,
+                       (const int*)(d_flags + 3 * M), (const int*)(d_flags + 4 * M), Ma,
+                       d.prm.cfl, d.dx, d.dy, d.dz, 1.0 / (d.nx * d.ny));
 }
 
 // Bytes the constructor allocates for a batch of `members` (device arrays
@@ -293,6 +371,7 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
     b += M * msp * (sizeof(double) + sizeof(double2));           // gam, usc
     b += (15 + 42) * M * chunk * d.plane * T + 21 * M * chunk * d.splane * 2 * T;
     b += 3 * M * sizeof(double) + 1024 * sizeof(double) + 2 * d.splane * T;
+    b += M * (CK_N * sizeof(double) + 3 * sizeof(unsigned long long) + 2 * sizeof(int));
     return b;
 }
 
@@ -324,10 +403,11 @@ static void col_common(ColArgs<T>& c, const Dims& d, const T* G1, const T* G2) {
 template <typename T>
 void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
                     long src_ms, cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms,
-                    int nlev) {
+                    int nlev, const double* mscale) {
     if (nlev <= 0) return;
     launch_rows_r2c<T>(timer, px.plan, njobs, src, scales, src_plane, d.ld, src_lev0, src_ms, dst,
-                       dst_plane, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr);
+                       dst_plane, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr,
+                       mscale, CK_N);
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
         ColArgs<T> c{};
@@ -476,6 +556,10 @@ void Case<T>::copy_member(int src, int dst) {
     set_member_params(dst, zo[src], mpf[2 * src], mpf[2 * src + 1]);
     mjt[dst] = mjt[src];
     lm_init[dst] = lm_init[src];
+    LG_HIP(hipMemcpyAsync(d_clock + (long)CK_N * dst, d_clock + (long)CK_N * src,
+                          CK_N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    mtime[dst] = mtime[src];
+    clk_fresh[dst] = clk_fresh[src];
     if (turb || mturb[dst].n() || mturb[src].n()) {
         pull_turbine_state();
         mturb[dst] = mturb[src];
@@ -496,6 +580,10 @@ void Case<T>::retire_member(int slot) {
         mturb[slot] = mturb[last];
         mjt[slot] = mjt[last];
         lm_init[slot] = lm_init[last];
+        LG_HIP(hipMemcpyAsync(d_clock + (long)CK_N * slot, d_clock + (long)CK_N * last,
+                              CK_N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        mtime[slot] = mtime[last];
+        clk_fresh[slot] = clk_fresh[last];
     }
     mturb[last] = MemberTurbines{};
     Ma = last;
@@ -658,7 +746,7 @@ void Case<T>::interpolag_sdep() {
     const long Fq = (long)M * d.field;
     for (int q = 0; q < 4; ++q) copy_dev<T>(fld[ids[q]], Ftmp + q * Fq, Ma * d.field, stream);
     InterpGeo g{d.nx, d.ny, d.nz, d.ld, d.plane, d.dx, d.dy, d.dz,
-                d.prm.lx, d.prm.ly, d.prm.lz, d.lagran_dt, d.field};
+                d.prm.lx, d.prm.ly, d.prm.lz, d.lagran_dt, d.field, clk()};
     hipLaunchKernelGGL(k_interpolag<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
                        fld[LG_U], fld[LG_V], fld[LG_W], Ftmp, Ftmp + Fq, Ftmp + 2 * Fq,
                        Ftmp + 3 * Fq, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN], fld[LG_F_NN], g,
@@ -677,6 +765,7 @@ void Case<T>::lagrange_sdep() {
     up.tf1_2 = tf1 * tf1;
     up.tf2_2 = tf2 * tf2;
     up.ldt = d.lagran_dt;
+    up.clk = clk();
     up.zero = 1.0e-24;
     up.expo = std::log(tf1) / (std::log(tf2) - std::log(tf1));
     up.betamin = 1.0 / (tf1 * tf2);
@@ -724,6 +813,9 @@ void Case<T>::lagrange_sdep() {
                            fld[LG_F_NN], fld[LG_CS_OPT2], fld[LG_BETA], fld[LG_TN_ALL], g, k0, nc,
                            Ma, up, (const int*)(d_flags + 2 * M));
     }
+    if (cfl_mode)
+        hipLaunchKernelGGL(k_clock_lag_reset, dim3(1), dim3(64), 0, stream, d_clock,
+                           (const int*)(d_flags + 2 * M), Ma);
     for (int m = 0; m < Ma; ++m)
         if (hflags[2 * M + m] == 2) lm_init[m] = 1;
 }
@@ -869,10 +961,12 @@ void Case<T>::press_stag_array() {
     {
         // u*, v* are used on levels 1..nz-1 and w* on 1..nz: one pass over
         // 1..nz (the u*, v* level nz spectra are not read)
+        // with use_cfl_dt the scale is each member's (CK_PSCALE of its clock)
         const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
         const T scs[3] = {c2, c2, c2};
         cplx<T>* S[3] = {spec[0], spec[1], spec[2]};
-        fwd2d(3, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
+        fwd2d(3, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz,
+              cfl_mode ? d_clock + CK_PSCALE : nullptr);
     }
     {
         // divtz at k = 1 and k = nz into planes 0 and 1 of spec[3]
@@ -921,7 +1015,7 @@ void Case<T>::project() {
     hipLaunchKernelGGL(k_project<T>, grid_ij(d.nx, d.ny, d.nz - 1, Ma), dim3(BLK), 0, stream,
                        fld[LG_U], fld[LG_V], fld[LG_W], (const T*)fld[LG_DPDX],
                        (const T*)fld[LG_DPDY], (const T*)fld[LG_DPDZ], GEO(1, d.nz - 1),
-                       (T)d.prm.dt, (T)d.tadv1);
+                       (T)d.prm.dt, (T)d.tadv1, clk());
     const dim3 gp((unsigned)((d.plane + BLK - 1) / BLK), Ma);
     hipLaunchKernelGGL(k_copy_plane<T>, gp, dim3(BLK), 0, stream, (const T*)lev(LG_U, d.nz - 1),
                        lev(LG_U, d.nz), d.field, d.plane);
@@ -940,7 +1034,7 @@ void Case<T>::rhs_ab2() {
                  fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F], fld[LG_DIVTX], fld[LG_DIVTY],
                  fld[LG_DIVTZ], tb ? fld[LG_FXA] : nullptr, tb ? fld[LG_FYA] : nullptr,
                  tb ? fld[LG_FZA] : nullptr, (T)d.prm.dt, (T)d.tadv1, (T)d.tadv2, d_mpf,
-                 (const int*)d_flags, d.prm.use_mean_p_force};
+                 (const int*)d_flags, d.prm.use_mean_p_force, clk()};
     hipLaunchKernelGGL(k_rhs_ab2<T>, grid_ij(d.ld, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
                        GEO(1, d.nz));
     fill_levels({{LG_U, 0, BOGUS}, {LG_V, 0, BOGUS}, {LG_W, 0, BOGUS}, {LG_U, d.nz, BOGUS},
@@ -957,6 +1051,7 @@ void Case<T>::rhs_gradp() {
 // The loop body after the counters advance and RHS_f = RHS (main.f90:155-326).
 template <typename T>
 void Case<T>::step_body() {
+    clock_begin();
     filt_da_uvw();
     ddz_uv(LG_U, LG_DUDZ);
     ddz_uv(LG_V, LG_DVDZ);
@@ -1009,6 +1104,10 @@ bool Case<T>::refresh_flags(bool force_lag) {
         hflags[m] = eu;
         hflags[M + m] = ci;
         hflags[2 * M + m] = lm;
+        if (cfl_mode) {
+            hflags[3 * M + m] = clk_fresh[m] ? 1 : 0;
+            hflags[4 * M + m] = (d.prm.initu || j >= d.prm.dyn_init - cs + 1) ? 1 : 0;
+        }
         special |= eu || ci || lm == 2;
         any_lag |= lm > 0;
     }
@@ -1022,12 +1121,23 @@ bool Case<T>::refresh_flags(bool force_lag) {
     return special;
 }
 
+// Host side of the clocks for the step the flags describe: total_time of a
+// fixed dt (main.f90:149); a CFL clock is fresh for one step only.
+template <typename T>
+void Case<T>::advance_clocks() {
+    for (int m = 0; m < Ma; ++m) {
+        if (!cfl_mode) mtime[m] = mtime[m] + d.prm.dt;
+        clk_fresh[m] = 0;
+    }
+}
+
 template <typename T>
 void Case<T>::step() {
     ++jt;
     ++jt_total;
     for (int m = 0; m < Ma; ++m) ++mjt[m];
     refresh_flags();
+    advance_clocks();
     LG_SWAP_RHS();
     step_body();
     if (st.on) stats_after_step();
@@ -1053,6 +1163,7 @@ void Case<T>::step_graph() {
     ++jt_total;
     for (int m = 0; m < Ma; ++m) ++mjt[m];
     const bool special = refresh_flags();
+    advance_clocks();
     LG_SWAP_RHS();
     if (special) {
         step_body();
@@ -1135,7 +1246,9 @@ double Case<T>::max_cfl(int m) {
         for (double x : h) mx = std::max(mx, x);
         mx3[q] = mx / h3[q];
     }
-    return d.prm.dt * std::max(mx3[0], std::max(mx3[1], mx3[2]));
+    double ck[5];
+    clock_get(m, ck);
+    return ck[0] * std::max(mx3[0], std::max(mx3[1], mx3[2]));
 }
 
 template <typename T>

@@ -87,10 +87,20 @@ struct Case {
     // fresh-start bookkeeping per member (lg_params.initu = 0)
     std::vector<int> mjt;            // steps the member has taken (its jt)
     std::vector<char> lm_init;       // F_* initialised by the first dynamic call
-    std::vector<int> hflags;         // euler[M], csinit[M], lagmode[M] of this step
+    // euler[M], csinit[M], lagmode[M], clock fresh[M], lagran_dt grows[M]
+    // of this step
+    std::vector<int> hflags;
     std::vector<int> dflags_now;     // the copy on the device
     int* d_flags = nullptr;
     bool lag_now = false;            // the dynamic model runs this step
+    // time-stepping clock.  use_cfl_dt: CK_N doubles per member on the
+    // device (clock.hpp), advanced by the step itself; fixed dt: only
+    // total_time changes, kept on the host.
+    bool cfl_mode = false;
+    double* d_clock = nullptr;
+    unsigned long long* d_cmax = nullptr;  // 3 per member: max |u|, |v|, |w|
+    std::vector<double> mtime;       // total_time per member (fixed dt)
+    std::vector<char> clk_fresh;     // the member's clock holds no previous step
     // running time averages (time_average.f90) and streamwise spectra
     struct StatsCfg {
         int nstart = 0, nend = -1, nskip = 1;
@@ -146,7 +156,7 @@ struct Case {
     // 2-D transform helpers over planes [lev0, lev0+nlev) of every member
     void fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
                long src_ms, cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms,
-               int nlev);
+               int nlev, const double* mscale = nullptr);
     void inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane, int src_lev0,
                long src_ms, T* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nlev,
                cplx<T>* const* tmpspec, const cplx<T>* const* src2 = nullptr);
@@ -178,6 +188,12 @@ struct Case {
     void rhs_ab2();
     void rhs_gradp();
     bool refresh_flags(bool force_lag = false);
+    const double* clk() const { return cfl_mode ? d_clock : nullptr; }
+    void clock_begin();
+    void clock_reset(int m);
+    void clock_get(int m, double* out5);
+    void clock_set(int m, double dt, double total_time, bool fresh);
+    void advance_clocks();
     void step_body();
     void step();
     void step_graph();
@@ -204,6 +220,7 @@ struct Case {
     void check_device_error();
     void stats_enable(int nstart, int nend, int nskip);
     void stats_after_step();
+    double stats_time(int m);
     void stats_mean(int m, int q, double* host);
     void stats_spectrum(int m, int c, double* host);
 };

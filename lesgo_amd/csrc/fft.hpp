@@ -190,6 +190,9 @@ struct RowR2CArgs {
     int nlev;              // planes per member (grid.y = nlev * members)
     long src_mstride;      // elements between members of src
     long dst_mstride;      // complex elements between members of dst
+    // per-member input scale (replaces scale[] when set): mscale[m * stride]
+    const double* mscale;
+    int mscale_stride;
 };
 
 template <typename T>
@@ -205,7 +208,7 @@ __global__ void __launch_bounds__(256) k_rows_r2c(RowR2CArgs<T> a) {
     const int S = min(a.S, npairs - pair0);
     const T* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
     cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
-    const T sc = a.scale[job];
+    const T sc = a.mscale ? (T)a.mscale[(long)mem * a.mscale_stride] : a.scale[job];
     const int* perm = a.plan.perm;
     for (int idx = threadIdx.x; idx < S * N; idx += blockDim.x) {
         const int s = idx / N;

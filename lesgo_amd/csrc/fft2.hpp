@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_rows_r2c_v2(RowR2CArgs<T> a) {
     const int S = min(a.S, npairs - pair0);
     const T* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
     cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
-    const T sc = a.scale[job];
+    const T sc = a.mscale ? (T)a.mscale[(long)mem * a.mscale_stride] : a.scale[job];
     for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
         const int s = t / N2, n2 = t - s * N2;
         const int j = 2 * (pair0 + s);

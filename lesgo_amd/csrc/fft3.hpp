@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) k_rows_r2c_v3(RowR2CArgs<T> a) {
     // rows 2 pair0 .. 2 (pair0 + S) - 1 are contiguous: stage them as they are
     block_copy16<T>(src, stage, 2L * S * sld);
     __syncthreads();
-    const T sc = a.scale[job];
+    const T sc = a.mscale ? (T)a.mscale[(long)mem * a.mscale_stride] : a.scale[job];
     for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
         const int s = t / N2, n2 = t - s * N2;
         const T* ra = stage + (long)(2 * s) * sld;

@@ -12,6 +12,7 @@ struct InterpGeo {
     long plane;
     double dx, dy, dz, lx, ly, lz, ldt;
     long mstride;          // elements between members (all arrays)
+    const double* clk;     // per-member clock (CFL-adaptive dt: lagran_dt) or null
 };
 
 // cell_indx_w for 'i'/'j' (functions.f90:188-237): px is wrapped with
@@ -87,6 +88,7 @@ __global__ void k_interpolag(const T* u, const T* v, const T* w, const T* tLM, c
     if (mode[m] == 0) return;      // member not yet dynamic
     const int k = blockIdx.y - m * g.nz + 1;
     const long mb = (long)m * g.mstride;
+    if (g.clk) g.ldt = g.clk[(long)CK_N * m + CK_LAGDT];
     u += mb;
     v += mb;
     w += mb;
@@ -175,6 +177,7 @@ __global__ void k_lag_prep(const T* u, const T* v, const T* w, const T* S11, con
 // 0-2 ubar,vbar,wbar; 3-8 products 11,12,13,22,23,33; 9-14 S_ij; 15-20 |S|S_ij.
 struct LagUpd {
     double opftdelta, powcoeff, cnst, tf1_2, tf2_2, ldt, zero, expo, betamin;
+    const double* clk;     // per-member clock (CFL-adaptive dt: lagran_dt) or null
 };
 
 // mode[m]: 0 = member not yet dynamic (untouched), 1 = update, 2 = first
@@ -191,6 +194,7 @@ __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, 
     const int c = blockIdx.y - m * nc;
     const int md = mode[m];
     if (md == 0) return;
+    if (p.clk) p.ldt = p.clk[(long)CK_N * m + CK_LAGDT];
     const int k = k0 + c;
     const long P = g.plane;
     const long o = (long)m * g.mstride + (long)k * P + (long)j * g.ld + i;

@@ -84,7 +84,8 @@ template <typename T>
 void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* const* src,
                      const T* scales, long src_plane, int src_ld, int src_lev0, long src_ms,
                      cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nrows,
-                     int ncols, int nlev, int M, hipStream_t s, const PassChoice* force);
+                     int ncols, int nlev, int M, hipStream_t s, const PassChoice* force,
+                     const double* mscale = nullptr, int mscale_stride = 0);
 template <typename T>
 void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T>* const* src,
                      long src_plane, int src_lev0, long src_ms, T* const* dst, long dst_plane,

@@ -160,7 +160,8 @@ template <typename T>
 void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* const* src,
                      const T* scales, long src_plane, int src_ld, int src_lev0, long src_ms,
                      cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nrows,
-                     int ncols, int nlev, int M, hipStream_t s, const PassChoice* force) {
+                     int ncols, int nlev, int M, hipStream_t s, const PassChoice* force,
+                     const double* mscale, int mscale_stride) {
     const PassChoice ch = force ? *force : choice_for<T>(PK_R2C, pl.N, nrows);
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
@@ -181,6 +182,8 @@ void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* con
         a.nlev = nlev;
         a.src_mstride = src_ms;
         a.dst_mstride = dst_ms;
+        a.mscale = mscale;
+        a.mscale_stride = mscale_stride;
         a.S = ch.S;
         a.stride = seq_stride(ch, pl.N);
         const size_t lds = row_lds_bytes<T>(ch, pl.N, src_ld, ncols);

@@ -3,6 +3,7 @@
 // ((k * ny) + j) * ld + i; k = 0..nz (lbz = 0, the one-rank PPMPI layout).
 #pragma once
 
+#include "clock.hpp"
 #include "common.hpp"
 
 namespace lg {
@@ -389,6 +390,7 @@ struct Ab2Args {
     const double* mpf;     // 2 per member: mean_p_force_x, mean_p_force_y
     const int* euler;      // per member
     int use_mpf;
+    const double* clk;     // per-member clock (CFL-adaptive dt) or null: dt, t1, t2
 };
 
 template <typename T>
@@ -397,6 +399,13 @@ __global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
     long mb;
     if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
     const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
+    T dt = a.dt, t1 = a.t1, t2 = a.t2;
+    if (a.clk) {
+        const double* c = a.clk + (long)CK_N * m;
+        dt = (T)c[CK_DT];
+        t1 = (T)c[CK_TADV1];
+        t2 = (T)c[CK_TADV2];
+    }
     if (k <= g.nz - 1) {
         T rx = -a.rx[o] - a.dx[o];
         T ry = -a.ry[o] - a.dy[o];
@@ -423,9 +432,9 @@ __global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
             fy = a.ryf[o];
             fz = a.rzf[o];
         }
-        a.u[o] = a.u[o] + a.dt * (a.t1 * rx + a.t2 * fx);
-        a.v[o] = a.v[o] + a.dt * (a.t1 * ry + a.t2 * fy);
-        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * fz);
+        a.u[o] = a.u[o] + dt * (t1 * rx + t2 * fx);
+        a.v[o] = a.v[o] + dt * (t1 * ry + t2 * fy);
+        a.w[o] = a.w[o] + dt * (t1 * rz + t2 * fz);
     } else {
         const T rz = -a.rz[o] - a.dz[o];
         a.rz[o] = rz;
@@ -437,7 +446,7 @@ __global__ void k_rhs_ab2(Ab2Args<T> a, Geo g) {
         } else {
             fz = a.rzf[o];
         }
-        a.w[o] = a.w[o] + a.dt * (a.t1 * rz + a.t2 * fz);
+        a.w[o] = a.w[o] + dt * (t1 * rz + t2 * fz);
     }
 }
 
@@ -468,11 +477,15 @@ __global__ void k_dpdz(const T* p, T* dpdz, Geo g, T dzv) {
 // project (forcing.f90:149-244), interior update; lid/wall BCs follow.
 template <typename T>
 __global__ void k_project(T* u, T* v, T* w, const T* px, const T* py, const T* pz, Geo g, T dt,
-                          T t1) {
+                          T t1, const double* clk) {
     int m, k, j, i;
     long mb;
     if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
     const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
+    if (clk) {
+        dt = (T)clk[(long)CK_N * m + CK_DT];
+        t1 = (T)clk[(long)CK_N * m + CK_TADV1];
+    }
     u[o] = u[o] + dt * (-(t1 * px[o]));
     v[o] = v[o] + dt * (-(t1 * py[o]));
     if (k >= 2) w[o] = w[o] + dt * (-(t1 * pz[o]));

@@ -55,7 +55,8 @@ static bool same_class(const lg_params& a, const lg_params& b) {
     return a.nx == b.nx && a.ny == b.ny && a.Nz == b.Nz && a.lx == b.lx && a.ly == b.ly &&
            a.lz == b.lz && a.dt == b.dt && a.vonk == b.vonk && a.cs_count == b.cs_count &&
            a.use_mean_p_force == b.use_mean_p_force && a.nu == b.nu &&
-           a.precision == b.precision;
+           a.precision == b.precision && a.initu == b.initu && a.dyn_init == b.dyn_init &&
+           a.use_cfl_dt == b.use_cfl_dt && (!a.use_cfl_dt || a.cfl == b.cfl);
 }
 
 struct Server {
@@ -300,7 +301,8 @@ int lg_server_submit(lg_server* s, const lg_params* prm, long seed, double noise
         SREQ(nsteps >= 1, "nsteps must be >= 1");
         SREQ(nturb >= 0 && (nturb == 0 || spec8), "bad turbine list");
         SREQ(prm->nx >= 4 && prm->ny >= 4 && prm->nx % 4 == 0 && prm->ny % 4 == 0 &&
-                 prm->Nz >= 3 && prm->zo > 0 && prm->dt > 0 && prm->cs_count >= 1,
+                 prm->Nz >= 3 && prm->zo > 0 && prm->dt > 0 && prm->cs_count >= 1 &&
+                 (!prm->use_cfl_dt || prm->cfl > 0),
              "bad job parameters");
         lg::Job j;
         j.prm = *prm;

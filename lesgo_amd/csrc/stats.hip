@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "case.hpp"
+#include "clock.hpp"
 #include "pointwise.hpp"
 
 namespace lg {
@@ -24,6 +25,7 @@ struct TavgArgs {
     double* acc;       // quantity q of member m at q * qstride + m * pm
     long qstride, pm;
     double dt;
+    const double* clk;     // per-member weights (CFL-adaptive dt) or null: dt
 };
 
 // One point (i < nx, j, k = 0..nz) of one member.  Interpolations as
@@ -39,7 +41,7 @@ __global__ void k_tavg(TavgArgs<T> a, Geo g) {
     const long P = g.plane;
     const long o = mb + (long)k * P + (long)j * g.ld + i;
     const int nz = g.nz;
-    const double dt = a.dt;
+    const double dt = a.clk ? a.clk[(long)CK_N * m + CK_SWAIT] : a.dt;
     double* acc = a.acc + (long)m * a.pm + ((long)k * g.ny + j) * g.nx + i;
     auto add = [&](int q, double val) { acc[(long)q * a.qstride] += val * dt; };
     const double u = a.u[o], v = a.v[o], w = a.w[o];
@@ -92,13 +94,15 @@ __global__ void k_tavg(TavgArgs<T> a, Geo g) {
 // y-summed spectral power of levels 1..nz-1: grid (kx blocks, (nz-1) * M, 3)
 template <typename T>
 __global__ void k_spec_acc(const cplx<T>* su, const cplx<T>* sv, const cplx<T>* sw, long msp,
-                           long splane, int ny, int NK, int nz, int M, double* acc, double dt) {
+                           long splane, int ny, int NK, int nz, int M, double* acc, double dt,
+                           const double* clk) {
     const int kx = blockIdx.x * blockDim.x + threadIdx.x;
     if (kx >= NK) return;
     const int m = blockIdx.y / (nz - 1);
     const int k = blockIdx.y - m * (nz - 1) + 1;
     const int c = blockIdx.z;
     const cplx<T>* s = (c == 0 ? su : (c == 1 ? sv : sw)) + (long)m * msp + (long)k * splane;
+    if (clk) dt = clk[(long)CK_N * m + CK_SWAIT];
     double sum = 0.0;
     for (int ky = 0; ky < ny; ++ky) {
         const cplx<T> z = s[(long)ky * NK + kx];
@@ -117,6 +121,8 @@ void Case<T>::stats_enable(int nstart, int nend, int nskip) {
     }
     LG_HIP(hipMemset(tavg, 0, (size_t)TQ_N * M * pm * sizeof(double)));
     LG_HIP(hipMemset(tspec, 0, (size_t)3 * M * (d.nz + 1) * d.NK * sizeof(double)));
+    if (cfl_mode)
+        hipLaunchKernelGGL(k_clock_stats, dim3(1), dim3(64), 0, stream, d_clock, M, 3);
     st = StatsCfg{};
     st.nstart = nstart;
     st.nend = nend;
@@ -127,11 +133,16 @@ void Case<T>::stats_enable(int nstart, int nend, int nskip) {
 template <typename T>
 void Case<T>::stats_after_step() {
     if (jt_total < st.nstart || jt_total > st.nend) return;
+    // with use_cfl_dt the weights are each member's, kept in its clock
+    if (cfl_mode)
+        hipLaunchKernelGGL(k_clock_stats, dim3(1), dim3(64), 0, stream, d_clock, Ma, 0);
     st.dt += d.prm.dt;
     if ((jt_total - st.nstart) % st.nskip != 0) return;
     if (!st.initialized) {
         st.initialized = true;
         st.dt = 0.0;
+        if (cfl_mode)
+            hipLaunchKernelGGL(k_clock_stats, dim3(1), dim3(64), 0, stream, d_clock, Ma, 1);
         return;
     }
     const long pm = (long)(d.nz + 1) * d.ny * d.nx;
@@ -139,7 +150,8 @@ void Case<T>::stats_after_step() {
                   fld[LG_TYY],  fld[LG_TZZ],  fld[LG_TXY],  fld[LG_TXZ],  fld[LG_TYZ],
                   fld[LG_FXA],  fld[LG_FYA],  fld[LG_FZA],  fld[LG_CS_OPT2],
                   fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDZ], fld[LG_DWDX],
-                  fld[LG_DWDY], tavg,         (long)M * pm, pm,         st.dt};
+                  fld[LG_DWDY], tavg,         (long)M * pm, pm,         st.dt,
+                  clk()};
     const Geo g{d.nx, d.ny, d.nz, d.ld, d.plane, 0, d.nz + 1, d.field};
     hipLaunchKernelGGL(k_tavg<T>, dim3((unsigned)((d.nx * d.ny + SBLK - 1) / SBLK), (d.nz + 1) * Ma),
                        dim3(SBLK), 0, stream, a, g);
@@ -151,9 +163,22 @@ void Case<T>::stats_after_step() {
     fwd2d(3, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz - 1);
     hipLaunchKernelGGL(k_spec_acc<T>, dim3((unsigned)((d.NK + 63) / 64), (d.nz - 1) * Ma, 3),
                        dim3(64), 0, stream, (const cplx<T>*)spec[0], (const cplx<T>*)spec[1],
-                       (const cplx<T>*)spec[2], msp, d.splane, d.ny, d.NK, d.nz, M, tspec, st.dt);
+                       (const cplx<T>*)spec[2], msp, d.splane, d.ny, d.NK, d.nz, M, tspec, st.dt,
+                       clk());
+    if (cfl_mode)
+        hipLaunchKernelGGL(k_clock_stats, dim3(1), dim3(64), 0, stream, d_clock, Ma, 2);
     st.total_time += st.dt;
     st.dt = 0.0;
+}
+
+// Accumulated averaging time of member m (each member's own with use_cfl_dt).
+template <typename T>
+double Case<T>::stats_time(int m) {
+    if (!cfl_mode) return st.total_time;
+    double c[CK_N];
+    LG_HIP(hipMemcpyAsync(c, d_clock + (long)CK_N * m, sizeof(c), hipMemcpyDeviceToHost, stream));
+    LG_HIP(hipStreamSynchronize(stream));
+    return c[CK_STIME];
 }
 
 template <typename T>
@@ -166,7 +191,7 @@ void Case<T>::stats_mean(int m, int q, double* host) {
     LG_HIP(hipStreamSynchronize(stream));
     LG_HIP(hipMemcpy(host, tavg + (long)q * M * pm + (long)m * pm, pm * sizeof(double),
                      hipMemcpyDeviceToHost));
-    const double tt = st.total_time;
+    const double tt = stats_time(m);
     for (long i = 0; i < pm; ++i) host[i] = tt > 0 ? host[i] / tt : 0.0;
 }
 
@@ -180,7 +205,7 @@ void Case<T>::stats_spectrum(int m, int c, double* host) {
     LG_HIP(hipStreamSynchronize(stream));
     LG_HIP(hipMemcpy(host, tspec + ((long)c * M + m) * n, n * sizeof(double),
                      hipMemcpyDeviceToHost));
-    const double tt = st.total_time;
+    const double tt = stats_time(m);
     for (long r = 0; r < d.nz + 1; ++r)
         for (int kx = 0; kx < d.NK; ++kx) {
             double& e = host[r * d.NK + kx];

@@ -33,6 +33,8 @@ type, bind(C) :: lg_params
     integer(c_int) :: precision
     integer(c_int) :: initu
     integer(c_int) :: dyn_init
+    integer(c_int) :: use_cfl_dt
+    real(c_double) :: cfl
 end type lg_params
 
 interface
@@ -143,6 +145,23 @@ interface
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: c
         real(c_double), intent(out) :: out
+    end function
+    integer(c_int) function lg_advance_clock(c) bind(C, name='lg_advance_clock')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+    end function
+    integer(c_int) function lg_clock_member(c, m, out) bind(C, name='lg_clock_member')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: c
+        integer(c_int), value :: m
+        real(c_double), intent(out) :: out(5)
+    end function
+    integer(c_int) function lg_set_clock_member(c, m, dt, total_time, fresh)             &
+        bind(C, name='lg_set_clock_member')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: c
+        integer(c_int), value :: m, fresh
+        real(c_double), value :: dt, total_time
     end function
     integer(c_int) function lg_max_cfl(c, out) bind(C, name='lg_max_cfl')
         import :: c_int, c_ptr, c_double

@@ -44,6 +44,8 @@ prm%nu = 0._rprec
 prm%precision = precision
 prm%initu = merge(1, 0, initu)
 prm%dyn_init = DYN_init
+prm%use_cfl_dt = merge(1, 0, use_cfl_dt)
+prm%cfl = cfl
 call gpu_check(lg_create(prm, gpu_case), 'lg_create')
 end subroutine gpu_init
 

@@ -30,6 +30,10 @@ logical :: initu = .true.
 logical :: use_mean_p_force = .true.
 real(rprec) :: mean_p_force_x = 1._rprec, mean_p_force_y = 0._rprec
 real(rprec) :: tadv1 = 1.5_rprec, tadv2 = -0.5_rprec
+! CFL-adaptive dt (param.f90:151-152): dt, tadv1, tadv2 then live on the
+! device, per step (lg_advance_clock); clock_sync copies them here
+logical :: use_cfl_dt = .false.
+real(rprec) :: cfl = 0.05_rprec
 integer :: nsteps = 100, wbase = 10, domain_nskip = 10
 integer :: jt = 0, jt_total = 0
 real(rprec) :: total_time = 0._rprec

@@ -20,7 +20,7 @@ use forcing, only : forcing_applied, project, turbines_on
 implicit none
 namelist /lesgo/ Nx, Ny, Nz, L_x, L_y, L_z, dt, zo, cs_count, mean_p_force_x, &
     nsteps, wbase, domain_nskip, precision, seed, fused, path, turbines_on, initu, DYN_init, &
-    checkpoint_data, checkpoint_nskip, checkpoint_file
+    checkpoint_data, checkpoint_nskip, checkpoint_file, use_cfl_dt, cfl
 character(len=512) :: nml
 integer :: jt_step, fid, ios, count0, count1, rate
 logical :: restart
@@ -68,11 +68,13 @@ call system_clock(count0, rate)
 time_loop: do jt_step = 1, nsteps
     jt_total = jt_total + 1
     jt = jt + 1
-    total_time = total_time + dt
+    if (.not. use_cfl_dt) total_time = total_time + dt
     if (fused) then
         call gpu_check(lg_step(gpu_case, 1_c_int), 'lg_step')
     else
         call gpu_counters()
+        ! dt = get_cfl_dt(), tadv1, tadv2, total_time (main.f90:135-151)
+        call gpu_check(lg_advance_clock(gpu_case), 'lg_advance_clock')
         ! RHS_f <- RHS (main.f90:155-157)
         call gpu_check(lg_copy_field(gpu_case, LG_RHSX, LG_RHSX_F), 'copy')
         call gpu_check(lg_copy_field(gpu_case, LG_RHSY, LG_RHSY_F), 'copy')
@@ -97,6 +99,7 @@ time_loop: do jt_step = 1, nsteps
     if (modulo(jt_total, domain_nskip) == 0) call write_domain()
     if (checkpoint_data .and. modulo(jt_total, checkpoint_nskip) == 0) call checkpoint()
     if (modulo(jt_total, wbase) == 0) then
+        call clock_sync()
         call rmsdiv(rms)
         call gpu_check(lg_energy(gpu_case, ke), 'energy')
         call gpu_check(lg_tau_wall_bot(gpu_case, tw), 'tau_wall')
@@ -105,6 +108,8 @@ time_loop: do jt_step = 1, nsteps
         write(*,'(a,E15.7)') '  Velocity divergence metric: ', rms
         write(*,'(a,E15.7)') '  Kinetic energy: ', ke
         write(*,'(a,E15.7)') '  Bot wall stress: ', tw
+        write(*,'(a,E15.7)') '  Time step: ', dt
+        write(*,'(a,2E15.7)') '  AB2 TADV1, TADV2: ', tadv1, tadv2
     end if
 end do time_loop
 call gpu_check(lg_synchronize(gpu_case), 'synchronize')
@@ -137,9 +142,24 @@ if (ios == 0) then
     close(un)
     jt_total = jt0
     total_time = t0
+    ! with use_cfl_dt the file's dt is the previous step's; a new run or a
+    ! changed cfl restarts with an Euler step (io.f90:91-94,
+    ! initialize.f90:191-200)
+    if (use_cfl_dt) dt = dt0
+    call gpu_check(lg_set_clock_member(gpu_case, 0_c_int, dt, total_time,                 &
+        merge(1_c_int, 0_c_int, use_cfl_dt .and. (jt0 == 0 .or.                          &
+        abs((cfl0 - cfl)/cfl) > 1.e-2_rprec))), 'lg_set_clock_member')
 end if
 call gpu_counters()
 end subroutine ic_file
+
+subroutine clock_sync()
+! dt, tadv1, tadv2, total_time of the last step from the device clock
+real(c_double) :: ck(5)
+if (.not. use_cfl_dt) return
+call gpu_check(lg_clock_member(gpu_case, 0_c_int, ck), 'lg_clock_member')
+dt = ck(1); tadv1 = ck(2); tadv2 = ck(3); total_time = ck(5)
+end subroutine clock_sync
 
 subroutine checkpoint()
 ! io.f90:1173-1267: vel.out, grid.out, total_time.dat (one process)
@@ -159,7 +179,14 @@ close(un)
 open(newunit=un, file=trim(path)//'grid.out', form='unformatted', status='replace')
 write(un) 1, Nx, Ny, Nz, L_x, L_y, L_z
 close(un)
-call gpu_check(lg_max_cfl(gpu_case, cfl_w), 'max_cfl')
+! the cfl written is the target with use_cfl_dt, the measured one otherwise
+! (io.f90:1232-1238)
+call clock_sync()
+if (use_cfl_dt) then
+    cfl_w = cfl
+else
+    call gpu_check(lg_max_cfl(gpu_case, cfl_w), 'max_cfl')
+end if
 open(newunit=un, file=trim(path)//'total_time.dat', status='replace')
 write(un, *) jt_total, total_time, total_time, dt, cfl_w
 close(un)

@@ -30,6 +30,7 @@ names the test that fixes it independently of this code).
 from __future__ import annotations
 
 import math
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -70,6 +71,10 @@ class Params:
     # initialised from MM, NN at the first dynamic call (param.f90:123,171-173)
     initu: bool = True
     dyn_init: int = 100
+    # use_cfl_dt = .true.: dt from the velocity each step (cfl_util.f90:73-111,
+    # main.f90:135-144); dt above is then only the value before the first step
+    use_cfl_dt: bool = False
+    cfl: float = 0.05
 
     def __post_init__(self):
         assert self.nx % 2 == 0 and self.ny % 2 == 0
@@ -281,10 +286,21 @@ class State:
     ustar_lbc: np.ndarray | None = None
     lm_init: bool = False          # F_LM_MM_init (lagrange_Sdep.f90:266-277)
     qn_init: bool = False          # F_QN_NN_init (lagrange_Sdep.f90:326-337)
+    # the time-stepping clock (param: dt, tadv1, tadv2; sgs_param: lagran_dt;
+    # total_time); clock_set: dt holds a previous step's value (with
+    # use_cfl_dt, initialize.f90:191-200 forces an Euler step otherwise)
+    dt: float = 0.0
+    tadv1: float = 1.5
+    tadv2: float = -0.5
+    lagran_dt: float = 0.0
+    total_time: float = 0.0
+    clock_set: bool = False
 
     @classmethod
     def zeros(cls, p: Params):
         s = cls(p)
+        s.dt, s.tadv1, s.tadv2 = p.dt, p.tadv1, p.tadv2
+        s.lagran_dt = 0.0 if p.use_cfl_dt else p.lagran_dt
         for n in FIELDS:
             s.f[n] = np.zeros((p.nz + 1, p.ny, p.ld))
         return s
@@ -297,7 +313,9 @@ class State:
 
     def copy(self):
         s = State(self.p, {k: v.copy() for k, v in self.f.items()}, self.jt,
-                  self.jt_total, self.turbines, None, self.lm_init, self.qn_init)
+                  self.jt_total, self.turbines, None, self.lm_init, self.qn_init,
+                  self.dt, self.tadv1, self.tadv2, self.lagran_dt, self.total_time,
+                  self.clock_set)
         if self.turbines is not None:
             s.turbines = self.turbines.copy()
         return s
@@ -457,6 +475,12 @@ def sgs_stag(sp: Spectral, s: State):
     p = sp.p
     nx, nz = p.nx, p.nz
     calc_Sij(p, s)
+    if p.use_cfl_dt:
+        # displacement time since the last dynamic call (sgs_stag_util.f90:74-79)
+        if s.jt >= p.dyn_init - p.cs_count + 1 or p.initu:
+            s.lagran_dt = s.lagran_dt + s.dt
+    else:
+        s.lagran_dt = p.cs_count * s.dt
     if s.jt == 1 and not p.initu:
         s.Cs_opt2[:] = 0.03
     elif (s.jt >= p.dyn_init or p.initu) and s.jt_total % p.cs_count == 0:
@@ -571,7 +595,7 @@ def interpolag_Sdep(p: Params, s: State):
     0.25 w(2) at the uvp level 1, the w=0 lid at nz) and trilinear gathers
     of F_LM, F_MM, F_QN, F_NN from their pre-step copies."""
     nx, ny, nz = p.nx, p.ny, p.nz
-    ldt = p.lagran_dt
+    ldt = s.lagran_dt
     x = (np.arange(nx) * p.dx)[None, None, :]
     y = (np.arange(ny) * p.dy)[None, :, None]
     u, v, w = s.u[:, :, :nx], s.v[:, :, :nx], s.w[:, :, :nx]
@@ -682,7 +706,7 @@ def lagrange_Sdep(sp: Spectral, s: State):
     Tn = np.maximum(F_LM * F_MM, zero)
     Tn = opftdelta * (Tn ** powcoeff)
     Tn = np.maximum(zero, Tn)
-    dumfac = p.lagran_dt / Tn
+    dumfac = s.lagran_dt / Tn
     epsi = dumfac / (1.0 + dumfac)
     F_LM = epsi * LM + (1.0 - epsi) * F_LM
     F_MM = epsi * MM + (1.0 - epsi) * F_MM
@@ -693,7 +717,7 @@ def lagrange_Sdep(sp: Spectral, s: State):
     Tn = np.maximum(F_QN * F_NN, zero)
     Tn = opftdelta * (Tn ** powcoeff)
     Tn = np.maximum(zero, Tn)
-    dumfac = p.lagran_dt / Tn
+    dumfac = s.lagran_dt / Tn
     epsi = dumfac / (1.0 + dumfac)
     F_QN = epsi * QN + (1.0 - epsi) * F_QN
     F_NN = epsi * NN + (1.0 - epsi) * F_NN
@@ -712,6 +736,8 @@ def lagrange_Sdep(sp: Spectral, s: State):
     s.Cs_opt2[K] = cs
     s.Beta[K] = beta
     s.Tn_all[K] = Tn
+    if p.use_cfl_dt:
+        s.lagran_dt = 0.0                  # lagrange_Sdep.f90:430
 
 
 # --------------------------------------------------------------------------
@@ -867,7 +893,7 @@ def press_stag_array(sp: Spectral, s: State):
     p = sp.p
     nz, ny, lh = p.nz, p.ny, p.lh
     const = 1.0 / (p.nx * p.ny)
-    const2 = const / p.tadv1 / p.dt
+    const2 = const / s.tadv1 / s.dt
     const3 = 1.0 / (p.dz ** 2)
     const4 = 1.0 / p.dz
     Hx = np.zeros((nz + 1, ny, lh), dtype=np.complex128)
@@ -929,10 +955,10 @@ def project(p: Params, s: State):
     w(1) = 0."""
     nx, nz = p.nx, p.nz
     K = np.s_[1:nz, :, :nx]
-    s.u[K] = s.u[K] + p.dt * (-p.tadv1 * s.dpdx[K])
-    s.v[K] = s.v[K] + p.dt * (-p.tadv1 * s.dpdy[K])
+    s.u[K] = s.u[K] + s.dt * (-s.tadv1 * s.dpdx[K])
+    s.v[K] = s.v[K] + s.dt * (-s.tadv1 * s.dpdy[K])
     K = np.s_[2:nz, :, :nx]
-    s.w[K] = s.w[K] + p.dt * (-p.tadv1 * s.dpdz[K])
+    s.w[K] = s.w[K] + s.dt * (-s.tadv1 * s.dpdz[K])
     s.u[nz] = s.u[nz - 1]
     s.v[nz] = s.v[nz - 1]
     s.w[nz] = 0.0
@@ -960,9 +986,33 @@ def energy(p: Params, s: State):
 def max_cfl(p: Params, s: State):
     """get_max_cfl (cfl_util.f90:35-70)."""
     K = np.s_[1:p.nz, :, :p.nx]
-    return p.dt * max(np.max(np.abs(s.u[K])) / p.dx,
+    return s.dt * max(np.max(np.abs(s.u[K])) / p.dx,
                       np.max(np.abs(s.v[K])) / p.dy,
                       np.max(np.abs(s.w[K])) / p.dz)
+
+
+def get_cfl_dt(p: Params, s: State):
+    """get_cfl_dt (cfl_util.f90:73-111): cfl / max(max|u|/dx, max|v|/dy,
+    max|w|/dz) over i < nx, levels 1..nz-1."""
+    K = np.s_[1:p.nz, :, :p.nx]
+    return p.cfl / max(float(np.max(np.abs(s.u[K]))) / p.dx,
+                       float(np.max(np.abs(s.v[K]))) / p.dy,
+                       float(np.max(np.abs(s.w[K]))) / p.dz)
+
+
+def cfl_clock(p: Params, s: State):
+    """The use_cfl_dt head of the time loop (main.f90:135-144): dt_f = dt,
+    dt = get_cfl_dt(), tadv1 = 1 + dt/(2 dt_f), tadv2 = 1 - tadv1.  A clock
+    that holds no previous step first gets dt = get_cfl_dt() * huge
+    (initialize.f90:191-200), which makes the step Euler (tadv1 = 1,
+    tadv2 = 0)."""
+    if not s.clock_set:
+        s.dt = get_cfl_dt(p, s) * sys.float_info.max
+        s.clock_set = True
+    dt_f = s.dt
+    s.dt = get_cfl_dt(p, s)
+    s.tadv1 = 1.0 + 0.5 * s.dt / dt_f
+    s.tadv2 = 1.0 - s.tadv1
 
 
 def tau_wall_bot(p: Params, s: State):
@@ -1022,7 +1072,7 @@ class Stats:
     def after_step(self, sp: Spectral, s: State):
         jt = s.jt_total
         if self.nstart <= jt <= self.nend:
-            self.dt += self.p.dt
+            self.dt += s.dt
             if (jt - self.nstart) % self.nskip == 0:
                 if not self.initialized:
                     self.initialized = True
@@ -1303,8 +1353,11 @@ def step(sp: Spectral, s: State):
     derivative fields (rmsdiv uses them, main.f90:367)."""
     p = sp.p
     nz = p.nz
+    if p.use_cfl_dt:
+        cfl_clock(p, s)
     s.jt += 1
     s.jt_total += 1
+    s.total_time = s.total_time + s.dt
     s.RHSx_f[:] = s.RHSx
     s.RHSy_f[:] = s.RHSy
     s.RHSz_f[:] = s.RHSz
@@ -1340,8 +1393,8 @@ def step(sp: Spectral, s: State):
     for n in ("x", "y", "z"):
         vel = {"x": s.u, "y": s.v, "z": s.w}[n]
         R, Rf = s.f["RHS" + n], s.f["RHS" + n + "_f"]
-        vel[K] = vel[K] + p.dt * (p.tadv1 * R[K] + p.tadv2 * Rf[K])
-    s.w[nz] = s.w[nz] + p.dt * (p.tadv1 * s.RHSz[nz] + p.tadv2 * s.RHSz_f[nz])
+        vel[K] = vel[K] + s.dt * (s.tadv1 * R[K] + s.tadv2 * Rf[K])
+    s.w[nz] = s.w[nz] + s.dt * (s.tadv1 * s.RHSz[nz] + s.tadv2 * s.RHSz_f[nz])
     s.u[0] = BOGUS
     s.v[0] = BOGUS
     s.w[0] = BOGUS

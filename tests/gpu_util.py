@@ -22,7 +22,8 @@ def gpu_case(p, precision="fp64"):
                            use_mean_p_force=p.use_mean_p_force,
                            mean_p_force_x=p.mean_p_force_x,
                            mean_p_force_y=p.mean_p_force_y, nu=p.nu, precision=precision,
-                           initu=p.initu, dyn_init=p.dyn_init)
+                           initu=p.initu, dyn_init=p.dyn_init, use_cfl_dt=p.use_cfl_dt,
+                           cfl=p.cfl)
     return lesgo_amd.Case(cfg)
 
 

@@ -1,0 +1,189 @@
+"""CFL-adaptive time step on the device (cfl_util.f90:73-111, main.f90:135-144,
+initialize.f90:191-200, sgs_stag_util.f90:74-82, lagrange_Sdep.f90:430)
+against the oracle: the dt sequence, the flow, members of one batch with
+their own clocks, graph replay, restart through total_time.dat, time-weighted
+statistics and the Fortran driver's subsystem path."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import lesgo_oracle as o
+from gpu_util import gpu_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "lesgo_amd", "lesgo_amd_main")
+X = np.s_[1:-1, :, :32]
+
+
+def _errs(c, s, names=("u", "v", "w", "p")):
+    p = s.p
+    K = np.s_[1:p.nz, :, :p.nx]
+    return {n: rel_l2(c.download(n)[K], s.f[n][K]) for n in names}
+
+
+@pytest.mark.parametrize("initu,dyn_init,steps", [(True, 100, 22), (False, 10, 17)])
+def test_cfl_run_matches_oracle(initu, dyn_init, steps):
+    """32x32x16, cfl = 0.05: dt, tadv1, tadv2, lagran_dt and total_time after
+    every step at 1e-12, the flow and Cs_opt2 at 1e-10 (file start: Euler by
+    the fresh clock, dynamic model every 5 steps; fresh start: Euler,
+    Smagorinsky, F_* initialised at DYN_init = 10)."""
+    p = o.Params(32, 32, 16, use_cfl_dt=True, cfl=0.05, initu=initu, dyn_init=dyn_init)
+    sp = o.Spectral(p)
+    s = o.initial_condition(p, 3)
+    with gpu_case(p) as c:
+        for n, a in s.f.items():
+            c.upload(n, a)
+        c.set_counters(0, 0)
+        for k in range(steps):
+            o.step(sp, s)
+            c.step(1)
+            ck = c.clock()
+            ref = {"dt": s.dt, "tadv1": s.tadv1, "tadv2": s.tadv2, "total_time": s.total_time}
+            for n, v in ref.items():
+                assert abs(ck[n] - v) <= 1e-12 * abs(v) + 1e-300, (k, n, ck[n], v)
+            assert abs(ck["lagran_dt"] - s.lagran_dt) <= 1e-12 * max(s.lagran_dt, 1e-300)
+            if k == 0:
+                assert ck["tadv1"] == 1.0 and ck["tadv2"] == 0.0
+        errs = _errs(c, s, ("u", "v", "w", "p", "Cs_opt2"))
+        print("  " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+        for n, e in errs.items():
+            assert e <= 1e-10, (n, e)
+        # get_max_cfl with the last dt reads back close to the target cfl
+        assert abs(c.max_cfl() - o.max_cfl(p, s)) <= 1e-12
+
+
+def test_cfl_members_graph_and_alone():
+    """Two members with different z0 and forcing take their own dt sequences;
+    the batch stepped through captured graphs equals each member stepped
+    alone, bit for bit, clocks included."""
+    import lesgo_amd
+    cfg = lesgo_amd.Config(32, 32, 16, use_cfl_dt=True, cfl=0.08, initu=False, dyn_init=10)
+    prm = [(1e-4, 1.0), (5e-4, 1.6)]
+    alone = []
+    for m, (zo, fx) in enumerate(prm):
+        with lesgo_amd.Case(cfg) as a:
+            a.set_member(0, zo, fx, 0.0)
+            a.seed_state(10 + m)
+            a.step(23)
+            alone.append(({n: a.download(n) for n in ("u", "v", "w", "p", "F_LM")}, a.clock()))
+    with lesgo_amd.Case(cfg, members=2) as b:
+        for m, (zo, fx) in enumerate(prm):
+            b.set_member(m, zo, fx, 0.0)
+            b.seed_state(10 + m, member=m)
+        b.step_graph(23)
+        for m in range(2):
+            assert b.clock(m) == alone[m][1], (m, b.clock(m), alone[m][1])
+            for n, ref in alone[m][0].items():
+                assert np.array_equal(b.download(n, m)[1:17], ref[1:17]), (m, n)
+    assert alone[0][1]["dt"] != alone[1][1]["dt"]
+
+
+def test_cfl_restart_is_exact(tmp_path):
+    """A CFL run written after 10 steps and read back continues bit for bit
+    (the file's dt is the AB2 history; lagran_dt is not in the files, as in
+    the reference, and is 0 right after the dynamic call of step 10), while a
+    changed cfl restarts with an Euler step."""
+    import lesgo_amd
+    from lesgo_amd.checkpoint import read_checkpoint, write_checkpoint
+    cfg = lesgo_amd.Config(32, 32, 16, use_cfl_dt=True, cfl=0.06)
+    with lesgo_amd.Case(cfg) as a:
+        a.seed_state(6)
+        a.step(10)
+        assert a.clock()["lagran_dt"] == 0.0
+        write_checkpoint(a, str(tmp_path))
+        a.step(8)
+        ref = {n: a.download(n) for n in ("u", "v", "w", "p")}
+        ref_clock = a.clock()
+    with lesgo_amd.Case(cfg) as b:
+        jt, _ = read_checkpoint(b, str(tmp_path))
+        assert jt == 10
+        b.step(8)
+        assert b.clock() == ref_clock
+        for n, r in ref.items():
+            assert np.array_equal(b.download(n)[1:17], r[1:17]), n
+    cfg2 = lesgo_amd.Config(32, 32, 16, use_cfl_dt=True, cfl=0.03)
+    with lesgo_amd.Case(cfg2) as e:
+        read_checkpoint(e, str(tmp_path))
+        e.step(1)
+        ck = e.clock()
+        assert ck["tadv1"] == 1.0 and ck["tadv2"] == 0.0
+
+
+def test_cfl_statistics_weights():
+    """Time averages weighted by each step's own dt match the oracle's."""
+    p = o.Params(32, 32, 16, use_cfl_dt=True, cfl=0.05)
+    sp = o.Spectral(p)
+    s = o.initial_condition(p, 8)
+    st = o.Stats(p, 2, 11, 1)
+    with gpu_case(p) as c:
+        for n, a in s.f.items():
+            c.upload(n, a)
+        c.set_counters(0, 0)
+        c.stats_enable(2, 11, 1)
+        for _ in range(11):
+            o.step(sp, s)
+            st.after_step(sp, s)
+        c.step(11)
+        tt, init = c.stats_info()
+        assert init and abs(tt - st.total_time) <= 1e-12 * st.total_time
+        K = np.s_[1:p.nz]
+        for n in ("u", "u2", "uw", "txz", "p", "cs_opt2"):
+            e = rel_l2(c.stats_mean(n)[K], st.mean(n)[K])
+            assert e <= 1e-10, (n, e)
+        e = rel_l2(c.stats_spectrum(0)[K], st.spectrum(0)[K])
+        assert e <= 1e-10, e
+
+
+def test_cfl_mixed_precision_dt_close():
+    """Mixed precision takes nearly the FP64 dt sequence: dt follows the
+    velocity maximum at one point, which the FP32 fields carry to about
+    1e-4; within 1e-3 relative over 40 steps."""
+    p = o.Params(32, 32, 16, use_cfl_dt=True, cfl=0.05)
+    s = o.initial_condition(p, 1)
+    with gpu_case(p, "fp64") as a, gpu_case(p, "mixed") as b:
+        for c in (a, b):
+            for n, arr in s.f.items():
+                c.upload(n, arr)
+            c.set_counters(0, 0)
+        worst = 0.0
+        for _ in range(40):
+            a.step(1)
+            b.step(1)
+            da, db = a.clock()["dt"], b.clock()["dt"]
+            worst = max(worst, abs(da - db) / da)
+        print(f"  max relative dt difference {worst:.2e}", flush=True)
+        assert worst <= 1e-3
+
+
+def test_fortran_driver_cfl_subsystem_path(tmp_path):
+    """The Fortran driver stepping through the subsystem entry points with
+    use_cfl_dt (lg_advance_clock at the head of each step) matches the
+    oracle's CFL run at step 10 (domain output u, 1e-10) and reports the
+    oracle's total_time and dt in total_time.dat (cfl = 0.03: at 0.05 this
+    32^3 flow blows up within 10 steps, in the oracle alike)."""
+    p = o.Params(32, 32, 32, use_cfl_dt=True, cfl=0.03)
+    sp = o.Spectral(p)
+    s = o.initial_condition(p, 0)
+    for _ in range(10):
+        o.step(sp, s)
+    with open(tmp_path / "lesgo_amd.nml", "w") as f:
+        f.write(f"&lesgo Nx=32, Ny=32, Nz=32, nsteps=10, wbase=10, domain_nskip=10, "
+                f"use_cfl_dt=.true., cfl=0.03, checkpoint_data=.true., "
+                f"path='{tmp_path}/' /\n")
+    os.makedirs(tmp_path / "output", exist_ok=True)
+    r = subprocess.run([EXE, str(tmp_path / "lesgo_amd.nml")], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=180)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0 and "Simulation complete" in r.stdout
+    n = 32 * 32 * 33
+    vel = np.fromfile(tmp_path / "output" / "vel.10.c0.bin", dtype=np.float64)
+    u = vel[:n].reshape(33, 32, 32)
+    e = rel_l2(u[:32], s.u[1:33, :, :32])
+    assert e <= 1e-10, e
+    w = open(tmp_path / "total_time.dat").read().split()
+    assert int(w[0]) == 10
+    assert abs(float(w[1]) - s.total_time) <= 1e-10 * s.total_time
+    assert abs(float(w[3]) - s.dt) <= 1e-10 * s.dt and float(w[4]) == 0.03
