@@ -171,6 +171,18 @@ int lg_turbines_init(lg_case* c, int nturb, const double* spec8,
 int lg_turbines_forcing(lg_case* c);
 /* u_d_T per turbine (turbines.f90:657-679 checkpoint content). */
 int lg_turbines_get(lg_case* c, double* u_d_T, double* f_n, int nturb);
+/* Time filter of the disk velocity for every disk of the case (turbines.f90:
+ * 561-566, 583): u_d_T = (1 - eps) u_d_T + eps u_d with eps = r / (1 + r),
+ * r = dt z_i / u_star / T_avg_dim; T_avg_dim <= 0 gives eps = 1 (the
+ * default). */
+int lg_turbines_time_filter(lg_case* c, double T_avg_dim, double z_i, double u_star);
+/* Dynamic thrust coefficients (dyn_Ct_prime, turbines.f90:504-515): Ct' of
+ * member m's nturb disks, with the ADM correction that follows from it; the
+ * disks' node lists are kept. */
+int lg_turbines_set_ct_member(lg_case* c, int m, const double* ct_prime, int nturb);
+/* u_d_T of member m's disks (turbine_vel_init reading u_d_T.dat,
+ * turbines.f90:229-250). */
+int lg_turbines_set_udt_member(lg_case* c, int m, const double* u_d_T, int nturb);
 
 /* --- per-kernel timing (no reference counterpart: the reference only has
  * whole-iteration clocks, clocks.f90:47-83).  When enabled, HIP events are

@@ -327,6 +327,23 @@ class Case:
                                            fn.ctypes.data_as(_lib._D), n), "turbines_get")
         return ud, fn
 
+    def turbines_time_filter(self, T_avg_dim: float, z_i: float = 1.0, u_star: float = 1.0):
+        """u_d_T filter of every disk (turbines.f90:561-566); T_avg_dim <= 0 is
+        none."""
+        check(lib().lg_turbines_time_filter(self._h, T_avg_dim, z_i, u_star), "time_filter")
+
+    def turbines_set_ct(self, ct_prime, member: int = 0):
+        """Dynamic Ct' of a member's disks (turbines.f90:504-515)."""
+        a = np.ascontiguousarray(np.asarray(ct_prime, dtype=np.float64).ravel())
+        check(lib().lg_turbines_set_ct_member(self._h, member, a.ctypes.data_as(_lib._D),
+                                              a.size), "turbines_set_ct")
+
+    def turbines_set_udt(self, u_d_T, member: int = 0):
+        """u_d_T of a member's disks (u_d_T.dat, turbines.f90:229-250)."""
+        a = np.ascontiguousarray(np.asarray(u_d_T, dtype=np.float64).ravel())
+        check(lib().lg_turbines_set_udt_member(self._h, member, a.ctypes.data_as(_lib._D),
+                                               a.size), "turbines_set_udt")
+
 
 def tune_get(kind: str, precision: str, N: int, other: int):
     """(n1, n2, per_wg, variant) launch choice the library uses for a pass."""

@@ -56,8 +56,11 @@ def read_record(f) -> bytes:
             return b"".join(out)
 
 
-def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "vel.out"):
-    """vel.out, grid.out and total_time.dat of member `member` into `path`."""
+def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "vel.out",
+                     total_time_dim: float | None = None):
+    """vel.out, grid.out and total_time.dat of member `member` into `path`
+    (total_time_dim: the dimensional time column, total_time when not
+    given)."""
     cfg = case.cfg
     nz = case.nz
     jt, jt_total = case.counters()
@@ -71,8 +74,9 @@ def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = 
     clk = case.clock(member)
     cfl = cfg.cfl if cfg.use_cfl_dt else case.max_cfl(member)
     t, dt = clk["total_time"], clk["dt"]
+    td = t if total_time_dim is None else total_time_dim
     with open(os.path.join(path, "total_time.dat"), "w") as f:
-        f.write(f" {jt_total:d} {t!r} {t!r} {dt!r} {cfl!r}\n")
+        f.write(f" {jt_total:d} {t!r} {td!r} {dt!r} {cfl!r}\n")
 
 
 def read_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "vel.out",

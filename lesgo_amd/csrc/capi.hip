@@ -408,6 +408,32 @@ int lg_turbines_get(lg_case* h, double* u_d_T, double* f_n, int nturb) {
     return lg_turbines_get_member(h, 0, u_d_T, f_n, nturb);
 }
 
+int lg_turbines_time_filter(lg_case* h, double T_avg_dim, double z_i, double u_star) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(z_i > 0 && u_star > 0, "z_i and u_star must be > 0");
+        c.t_avg_dim = T_avg_dim;
+        c.z_i = z_i;
+        c.u_star = u_star;
+        c.drop_graphs();
+    });
+}
+
+int lg_turbines_set_ct_member(lg_case* h, int m, const double* ct_prime, int nturb) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(ct_prime && nturb == c.mturb[m].n() && nturb > 0, "turbine count mismatch");
+        lg::turbines_set_ct(c, m, ct_prime);
+    });
+}
+
+int lg_turbines_set_udt_member(lg_case* h, int m, const double* u_d_T, int nturb) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(u_d_T && nturb == c.mturb[m].n() && nturb > 0, "turbine count mismatch");
+        lg::turbines_set_udt(c, m, u_d_T);
+    });
+}
+
 int lg_batch_info(const lg_case* h, int* members, int* active) {
     if (!h) return LG_ERR_ARG;
     const int M = h->c64 ? h->c64->M : h->c32->M;

@@ -47,10 +47,12 @@ struct MemberTurbines {
     std::vector<long> lin;           // node offset (k*ny + j)*ld + i within the member
     std::vector<double> ind;         // normalised indicator weight
     std::vector<int> off;            // n+1 offsets into lin / ind
-    std::vector<double> coef;        // per turbine: n1,n2,n3,ct',dia,corrfac
+    std::vector<double> coef;        // per turbine TC_N: n1,n2,n3,ct',dia,corrfac,member,-
     std::vector<double> state;       // per turbine: u_d, u_d_T, f_n (last download)
+    bool adm = true;                 // ADM correction (turbines.f90:578-581)
     int n() const { return (int)spec.size() / 8; }
 };
+constexpr int TC_N = 8;              // coefficients per turbine
 
 // Device node lists of every member's disks, concatenated member by member.
 struct TurbineSet {
@@ -62,7 +64,7 @@ struct TurbineSet {
     int* off = nullptr;              // n+1 offsets
     int* node_tid = nullptr;         // turbine of each node
     bool has_fz = false;
-    double* coef = nullptr;          // per turbine: n1,n2,n3,ct',dia,corrfac
+    double* coef = nullptr;          // per turbine: TC_N coefficients
     double* state = nullptr;         // per turbine: u_d, u_d_T, f_n
     ~TurbineSet();
 };
@@ -142,6 +144,8 @@ struct Case {
     T* G2 = nullptr;
     PlanTables px, py, pbx, pby;     // nx, ny, nx2, ny2
     std::unique_ptr<TurbineSet> turb;
+    // u_d_T time filter (turbines.f90:561-566): T_avg_dim <= 0 is none
+    double t_avg_dim = -1.0, z_i = 1.0, u_star = 1.0;
     KernelTimer timer;
 
     Case(const lg_params& p, int members);
@@ -243,5 +247,9 @@ int autotune(Case<T>& c, int reps);
 template <typename T>
 void turbines_init(Case<T>& c, int m, int nturb, const double* spec8, double alpha1,
                    double alpha2, double filter_cutoff, int adm_correction);
+template <typename T>
+void turbines_set_ct(Case<T>& c, int m, const double* ct);
+template <typename T>
+void turbines_set_udt(Case<T>& c, int m, const double* udt);
 
 }  // namespace lg

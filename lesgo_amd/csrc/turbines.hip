@@ -96,14 +96,21 @@ static int fnint(double x) {
 }
 
 // Disk-averaged velocity, thrust (turbines.f90:517-580): one block per
-// turbine; T_avg_dim <= 0 so eps = 1.
+// turbine.  The u_d_T filter weight eps = (dt_dim/T)/(1 + dt_dim/T),
+// dt_dim = dt z_i / u_star, for T = T_avg_dim > 0, else 1 (561-566); dt is
+// the member's clock with a CFL-adaptive dt.
+struct TurbFilter {
+    double T_avg_dim, z_i, u_star, dt;
+    const double* clk;
+};
+
 template <typename T>
 __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* node_lin,
                               const double* node_ind, const int* off, const double* coef,
-                              double* state, long plane, double dV) {
+                              double* state, long plane, double dV, TurbFilter tf) {
     __shared__ double sh[256];
     const int s = blockIdx.x;
-    const double n1 = coef[6 * s + 0], n2 = coef[6 * s + 1], n3 = coef[6 * s + 2];
+    const double n1 = coef[TC_N * s + 0], n2 = coef[TC_N * s + 1], n3 = coef[TC_N * s + 2];
     double acc = 0.0;
     for (int l = off[s] + threadIdx.x; l < off[s + 1]; l += blockDim.x) {
         const long o = node_lin[l];
@@ -117,10 +124,17 @@ __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* no
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const double ct = coef[6 * s + 3], dia = coef[6 * s + 4], corr = coef[6 * s + 5];
+        const double ct = coef[TC_N * s + 3], dia = coef[TC_N * s + 4],
+                     corr = coef[TC_N * s + 5];
         double ud = sh[0];
         ud = ud / corr;
-        const double eps = 1.0;
+        double eps = 1.0;
+        if (tf.T_avg_dim > 0.0) {
+            const int mem = (int)coef[TC_N * s + 6];
+            const double dt = tf.clk ? tf.clk[(long)CK_N * mem + CK_DT] : tf.dt;
+            const double dt_dim = dt * tf.z_i / tf.u_star;
+            eps = (dt_dim / tf.T_avg_dim) / (1. + dt_dim / tf.T_avg_dim);
+        }
         const double udt = (1. - eps) * state[3 * s + 1] + eps * ud;
         const double fn = -0.5 * ct * fabs(udt) * udt * 0.25 * 3.1415926535897932384626433 * (dia * dia);
         state[3 * s + 0] = ud;
@@ -139,9 +153,9 @@ __global__ void k_turb_scatter(T* fx, T* fy, T* fzu, const long* node_lin, const
     const int s = node_tid[l];
     const double fn = state[3 * s + 2];
     const long o = node_lin[l];
-    fx[o] = (T)(fn * coef[6 * s + 0] * node_ind[l]);
-    fy[o] = (T)(fn * coef[6 * s + 1] * node_ind[l]);
-    fzu[o] = (T)(fn * coef[6 * s + 2] * node_ind[l]);
+    fx[o] = (T)(fn * coef[TC_N * s + 0] * node_ind[l]);
+    fy[o] = (T)(fn * coef[TC_N * s + 1] * node_ind[l]);
+    fzu[o] = (T)(fn * coef[TC_N * s + 2] * node_ind[l]);
 }
 
 // fza = interp_to_w_grid(fza) (functions.f90:131-185), k = 1..nz.
@@ -227,10 +241,11 @@ void turbines_init(Case<T>& c, int m, int nturb, const double* spec8, double alp
         }
         mt.off.push_back((int)mt.lin.size());
         const double corr = adm_correction ? (1 + 0.25 * (1 - itf.M) * ctp) : 1.0;
-        mt.coef.insert(mt.coef.end(), {n1, n2, n3, ctp, dia, corr});
+        mt.coef.insert(mt.coef.end(), {n1, n2, n3, ctp, dia, corr, (double)m, 0.0});
         // u_d_T without a u_d_T.dat (turbines.f90:245-250)
         mt.state.insert(mt.state.end(), {0.0, -1.0, 0.0});
     }
+    mt.adm = adm_correction != 0;
     c.pull_turbine_state();     // the other members keep their disk state
     c.mturb[m] = std::move(mt);
     c.rebuild_turbines();
@@ -273,9 +288,10 @@ void Case<T>::rebuild_turbines() {
                 tid.push_back(s);
             }
             off.push_back((int)lin.size());
-            coef.insert(coef.end(), mt.coef.begin() + 6 * q, mt.coef.begin() + 6 * q + 6);
+            coef.insert(coef.end(), mt.coef.begin() + TC_N * q, mt.coef.begin() + TC_N * (q + 1));
+            coef[coef.size() - TC_N + 6] = (double)m;
             st.insert(st.end(), mt.state.begin() + 3 * q, mt.state.begin() + 3 * q + 3);
-            if (mt.coef[6 * q + 2] != 0.0) ts->has_fz = true;
+            if (mt.coef[TC_N * q + 2] != 0.0) ts->has_fz = true;
         }
         ts->first.push_back(s);
     }
@@ -309,7 +325,8 @@ void Case<T>::turbines_forcing() {
     hipLaunchKernelGGL(k_turb_reduce<T>, dim3(t.n), dim3(256), 0, stream, (const T*)fld[LG_U],
                        (const T*)fld[LG_V], (const T*)fld[LG_W], (const long*)t.node_lin,
                        (const double*)t.node_ind, (const int*)t.off, (const double*)t.coef,
-                       t.state, d.plane, d.dx * d.dy * d.dz);
+                       t.state, d.plane, d.dx * d.dy * d.dz,
+                       TurbFilter{t_avg_dim, z_i, u_star, d.prm.dt, clk()});
     // the node set is static, so fxa/fya/fza are zero off the nodes
     T* fz_uv = t.has_fz ? tmp[8] : fld[LG_FZA];
     hipLaunchKernelGGL(k_turb_scatter<T>, dim3((t.total_nodes + 255) / 256), dim3(256), 0, stream,
@@ -323,6 +340,41 @@ void Case<T>::turbines_forcing() {
     }
 }
 
+// Dynamic Ct' (turbines.f90:504-515): new thrust coefficients of member m's
+// disks in place (and the ADM correction that depends on them); the node
+// lists stay.
+template <typename T>
+void turbines_set_ct(Case<T>& c, int m, const double* ct) {
+    MemberTurbines& mt = c.mturb[m];
+    const int n = mt.n();
+    if (n == 0) return;
+    for (int q = 0; q < n; ++q) {
+        double* cf = &mt.coef[TC_N * q];
+        cf[3] = ct[q];
+        cf[5] = mt.adm ? (1 + 0.25 * (1 - mt.M[q]) * ct[q]) : 1.0;
+        mt.spec[8 * q + 7] = ct[q];
+    }
+    LG_HIP(hipStreamSynchronize(c.stream));
+    LG_HIP(hipMemcpy(c.turb->coef + (long)TC_N * c.turb->first[m], mt.coef.data(),
+                     mt.coef.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// u_d_T of member m's disks (turbine_vel_init from u_d_T.dat, turbines.f90:
+// 229-250).
+template <typename T>
+void turbines_set_udt(Case<T>& c, int m, const double* udt) {
+    const int n = c.mturb[m].n();
+    if (n == 0) return;
+    c.pull_turbine_state();
+    for (int q = 0; q < n; ++q) c.mturb[m].state[3 * q + 1] = udt[q];
+    LG_HIP(hipMemcpy(c.turb->state + 3L * c.turb->first[m], c.mturb[m].state.data(),
+                     3 * n * sizeof(double), hipMemcpyHostToDevice));
+}
+
+template void turbines_set_ct<double>(Case<double>&, int, const double*);
+template void turbines_set_ct<float>(Case<float>&, int, const double*);
+template void turbines_set_udt<double>(Case<double>&, int, const double*);
+template void turbines_set_udt<float>(Case<float>&, int, const double*);
 template void turbines_init<double>(Case<double>&, int, int, const double*, double, double,
                                     double, int);
 template void turbines_init<float>(Case<float>&, int, int, const double*, double, double, double,

@@ -1216,8 +1216,10 @@ class Turbines:
     normalised indicator weights, and u_d_T per turbine."""
 
     def __init__(self, p: Params, specs, alpha1=1.5, alpha2=1.5,
-                 filter_cutoff=1e-2, adm_correction=True, N=2048):
+                 filter_cutoff=1e-2, adm_correction=True, N=2048, T_avg_dim=-1.0,
+                 z_i=1.0, u_star=1.0):
         self.p = p
+        self.T_avg_dim, self.z_i, self.u_star = T_avg_dim, z_i, u_star
         self.specs = [TurbineSpec(**vars(t)) for t in specs]
         self.adm_correction = adm_correction
         dx, dy, dz = p.dx, p.dy, p.dz
@@ -1284,8 +1286,10 @@ class Turbines:
         return t
 
     def forcing(self, s: State):
-        """turbines_forcing (turbines.f90:465-638), T_avg_dim <= 0 (eps = 1),
-        no rotation, static yaw/Ct'."""
+        """turbines_forcing (turbines.f90:465-638), no rotation; the u_d_T
+        filter eps = (dt_dim/T)/(1 + dt_dim/T), dt_dim = dt z_i/u_star, for
+        T = T_avg_dim > 0, else 1 (561-566).  Ct' is each spec's current one
+        (a dynamic schedule sets it between steps, 504-515)."""
         p = self.p
         nz = p.nz
         w_uv = np.empty_like(s.w)
@@ -1294,7 +1298,11 @@ class Turbines:
         s.fxa[:] = 0.0
         s.fya[:] = 0.0
         fza = np.zeros_like(s.fza)
-        eps = 1.0
+        if self.T_avg_dim > 0.0:
+            dt_dim = s.dt * self.z_i / self.u_star
+            eps = (dt_dim / self.T_avg_dim) / (1. + dt_dim / self.T_avg_dim)
+        else:
+            eps = 1.0
         dV = p.dx * p.dy * p.dz
         for n, t in enumerate(self.specs):
             I, J, K = self.nodes[n]
