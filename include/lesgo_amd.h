@@ -114,6 +114,13 @@ int lg_synchronize(lg_case* c);
  * once per step, after lg_set_counters has advanced the counters (lg_step
  * does this itself). */
 int lg_advance_clock(lg_case* c);
+/* Make the physical fields the case's state, as a restart from files does:
+ * lg_step keeps the fields between steps as (kx, ky) spectra and makes the
+ * physical ones when they are read; after lg_resync the next step starts
+ * from the physical fields.  Writing a checkpoint calls it, so that the run
+ * that wrote the files and a run restarted from them continue alike, bit for
+ * bit (no counterpart in the reference, whose state is always physical). */
+int lg_resync(lg_case* c);
 
 /* --- replaced subroutines (device state in, device state out) ---------- */
 /* derivatives::filt_da (derivatives.f90:166-211), levels 0..nz */

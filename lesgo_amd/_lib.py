@@ -90,6 +90,7 @@ SIGNATURES = {
     "lg_tau_wall_bot_member": (C.c_int, [_P, C.c_int, _D]),
     "lg_clock_member": (C.c_int, [_P, C.c_int, _D]),
     "lg_advance_clock": (C.c_int, [_P]),
+    "lg_resync": (C.c_int, [_P]),
     "lg_set_clock_member": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, C.c_int]),
     "lg_turbines_init_member": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_double, C.c_double,
                                           C.c_double, C.c_int]),

@@ -151,6 +151,11 @@ class Case:
     def synchronize(self):
         check(lib().lg_synchronize(self._h), "synchronize")
 
+    def resync(self):
+        """Make the physical fields the state (the next step starts from them,
+        as a restart from files does)."""
+        check(lib().lg_resync(self._h), "resync")
+
     def upload_state(self, fields: dict, jt: int = 0, jt_total: int = 0):
         for n, a in fields.items():
             if n in FIELD_ID:

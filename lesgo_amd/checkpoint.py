@@ -63,6 +63,8 @@ def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = 
     given)."""
     cfg = case.cfg
     nz = case.nz
+    # the files are the state from here on, in this run as in a restart
+    case.resync()
     jt, jt_total = case.counters()
     parts = [np.ascontiguousarray(case.download(n, member)[1:nz + 1]) for n in CHECKPOINT_FIELDS]
     with open(os.path.join(path, fname), "wb") as f:

@@ -62,6 +62,25 @@ int dispatch(lg_case* h, F&& fn) {
     });
 }
 
+// Entry points that read physical fields make the stale ones first; those
+// that change them also end the spectral state (the next step starts from
+// the physical fields).
+template <typename F>
+int dispatch_mod(lg_case* h, F&& fn) {
+    return dispatch(h, [&](auto& c) {
+        c.touch(true);
+        fn(c);
+    });
+}
+
+template <typename F>
+int dispatch_read(lg_case* h, F&& fn) {
+    return dispatch(h, [&](auto& c) {
+        c.touch(false);
+        fn(c);
+    });
+}
+
 bool smooth(int n) {
     if (n <= 0) return false;
     while (n % 2 == 0) n /= 2;
@@ -132,7 +151,7 @@ int lg_sizes(const lg_case* h, int* nz, int* ld, int* ny) {
 }
 
 int lg_upload_member(lg_case* h, int m, int field, const double* host) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(field);
         check_member(c, m);
         REQUIRE(host, "null host pointer");
@@ -151,7 +170,7 @@ int lg_upload_member(lg_case* h, int m, int field, const double* host) {
 }
 
 int lg_download_member(lg_case* h, int m, int field, double* host) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_read(h, [&](auto& c) {
         check_field(field);
         check_member(c, m);
         REQUIRE(host, "null host pointer");
@@ -200,12 +219,16 @@ int lg_set_counters(lg_case* h, int jt, int jt_total) {
 }
 
 int lg_advance_clock(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         c.refresh_flags();
         c.advance_clocks();
         c.clock_begin();
         LG_HIP(hipGetLastError());
     });
+}
+
+int lg_resync(lg_case* h) {
+    return dispatch_mod(h, [&](auto& c) { (void)c; });
 }
 
 int lg_get_counters(const lg_case* h, int* jt, int* jt_total) {
@@ -228,7 +251,7 @@ int lg_synchronize(lg_case* h) {
 }
 
 int lg_filt_da(lg_case* h, int f, int dfdx, int dfdy) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(f);
         check_field(dfdx);
         check_field(dfdy);
@@ -237,7 +260,7 @@ int lg_filt_da(lg_case* h, int f, int dfdx, int dfdy) {
 }
 
 int lg_ddx(lg_case* h, int f, int dfdx) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(f);
         check_field(dfdx);
         c.ddx(f, dfdx);
@@ -245,7 +268,7 @@ int lg_ddx(lg_case* h, int f, int dfdx) {
 }
 
 int lg_ddy(lg_case* h, int f, int dfdy) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(f);
         check_field(dfdy);
         c.ddy(f, dfdy);
@@ -253,7 +276,7 @@ int lg_ddy(lg_case* h, int f, int dfdy) {
 }
 
 int lg_ddxy(lg_case* h, int f, int dfdx, int dfdy) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(f);
         check_field(dfdx);
         check_field(dfdy);
@@ -262,7 +285,7 @@ int lg_ddxy(lg_case* h, int f, int dfdx, int dfdy) {
 }
 
 int lg_ddz_uv(lg_case* h, int f, int dfdz) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(f);
         check_field(dfdz);
         c.ddz_uv(f, dfdz);
@@ -270,47 +293,47 @@ int lg_ddz_uv(lg_case* h, int f, int dfdz) {
 }
 
 int lg_ddz_w(lg_case* h, int f, int dfdz) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(f);
         check_field(dfdz);
         c.ddz_w(f, dfdz);
     });
 }
 
-int lg_wallstress(lg_case* h) { return dispatch(h, [&](auto& c) { c.wallstress(); }); }
+int lg_wallstress(lg_case* h) { return dispatch_mod(h, [&](auto& c) { c.wallstress(); }); }
 int lg_sgs_stag(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         c.refresh_flags();
         c.sgs_stag();
     });
 }
 int lg_lagrange_sdep(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         c.refresh_flags(true);
         c.lagrange_sdep();
     });
 }
 int lg_interpolag_sdep(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         c.refresh_flags(true);
         c.interpolag_sdep();
     });
 }
-int lg_divstress_uv(lg_case* h) { return dispatch(h, [&](auto& c) { c.divstress_uv(); }); }
-int lg_divstress_w(lg_case* h) { return dispatch(h, [&](auto& c) { c.divstress_w(); }); }
-int lg_convec(lg_case* h) { return dispatch(h, [&](auto& c) { c.convec(); }); }
+int lg_divstress_uv(lg_case* h) { return dispatch_mod(h, [&](auto& c) { c.divstress_uv(); }); }
+int lg_divstress_w(lg_case* h) { return dispatch_mod(h, [&](auto& c) { c.divstress_w(); }); }
+int lg_convec(lg_case* h) { return dispatch_mod(h, [&](auto& c) { c.convec(); }); }
 
 int lg_press_stag_array(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         c.press_stag_array();
         c.check_device_error();
     });
 }
 
-int lg_project(lg_case* h) { return dispatch(h, [&](auto& c) { c.project(); }); }
+int lg_project(lg_case* h) { return dispatch_mod(h, [&](auto& c) { c.project(); }); }
 
 int lg_copy_field(lg_case* h, int src, int dst) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         check_field(src);
         check_field(dst);
         using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
@@ -321,12 +344,12 @@ int lg_copy_field(lg_case* h, int src, int dst) {
 }
 
 int lg_rhs_ab2(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         c.refresh_flags();
         c.rhs_ab2();
     });
 }
-int lg_rhs_gradp(lg_case* h) { return dispatch(h, [&](auto& c) { c.rhs_gradp(); }); }
+int lg_rhs_gradp(lg_case* h) { return dispatch_mod(h, [&](auto& c) { c.rhs_gradp(); }); }
 
 int lg_step(lg_case* h, int nsteps) {
     return dispatch(h, [&](auto& c) {
@@ -369,7 +392,7 @@ int lg_set_clock_member(lg_case* h, int m, double dt, double total_time, int fre
 
 int lg_turbines_init_member(lg_case* h, int m, int nturb, const double* spec8, double alpha1,
                             double alpha2, double filter_cutoff, int adm_correction) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_read(h, [&](auto& c) {
         check_member(c, m);
         REQUIRE(nturb >= 0 && (nturb == 0 || spec8), "bad turbine list");
         REQUIRE(alpha1 > 0 && alpha2 > 0 && filter_cutoff >= 0, "bad turbine filter parameters");
@@ -385,7 +408,7 @@ int lg_turbines_init(lg_case* h, int nturb, const double* spec8, double alpha1, 
 }
 
 int lg_turbines_forcing(lg_case* h) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_mod(h, [&](auto& c) {
         REQUIRE(c.turb, "turbines not initialised");
         c.turbines_forcing();
     });
@@ -444,7 +467,7 @@ int lg_batch_info(const lg_case* h, int* members, int* active) {
 }
 
 int lg_set_active(lg_case* h, int active) {
-    return dispatch(h, [&](auto& c) {
+    return dispatch_read(h, [&](auto& c) {
         REQUIRE(active >= 1 && active <= c.M, "active member count out of range");
         c.Ma = active;
     });
@@ -577,6 +600,8 @@ int lg_diff_l2(lg_case* a, int ma, lg_case* b, int mb, int field, int k0, int k1
             check_member(ca, ma);
             check_member(cb, mb);
             REQUIRE(k0 >= 0 && k1 <= ca.d.nz && k0 <= k1, "level range out of bounds");
+            ca.touch(false);
+            cb.touch(false);
             lg::diff_l2(ca, ma, cb, mb, field, k0, k1, num, den);
         };
         if (a->c64 && b->c64) run(*a->c64, *b->c64);

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -15,6 +16,7 @@
 #include "lagrange.hpp"
 #include "pointwise.hpp"
 #include "pressure.hpp"
+#include "spectral.hpp"
 
 namespace lg {
 
@@ -158,15 +160,32 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     mturb.resize(M);
     mjt.assign(M, 0);
     lm_init.assign(M, 0);
-    hflags.assign(5 * M, 0);
-    dflags_now.assign(5 * M, 0);
-    d_flags = dalloc<int>(5L * M);
+    hflags.assign(6 * M, 0);
+    dflags_now.assign(6 * M, 0);
+    d_flags = dalloc<int>(6L * M);
+    mcold.assign(M, 1);
     cfl_mode = p.use_cfl_dt != 0;
     d_clock = dalloc<double>((long)CK_N * M);
     d_cmax = dalloc<unsigned long long>(3L * M);
     mtime.assign(M, 0.0);
     clk_fresh.assign(M, 1);
     for (int m = 0; m < M; ++m) clock_reset(m);
+    // a CFL step reads the end-of-step velocity at its head, so only fixed-dt
+    // cases keep the spectra between steps; LESGO_AMD_PHYSICAL_STEP=1 in the
+    // environment keeps every case on the all-physical step
+    const char* phys = std::getenv("LESGO_AMD_PHYSICAL_STEP");
+    spec_on = !cfl_mode && !(phys && phys[0] == '1');
+    if (spec_on) {
+        for (int q = 0; q < 3; ++q) {
+            sR[q] = dalloc<cplx<T>>(M * msp);
+            sRf[q] = dalloc<cplx<T>>(M * msp);
+            sD[q] = dalloc<cplx<T>>(M * msp);
+            sF[q] = dalloc<cplx<T>>(M * msp);
+            fprev[q] = dalloc<T>(M * d.field);
+            tprev[q] = dalloc<T>(M * d.field);
+        }
+        sP = dalloc<cplx<T>>(M * msp);
+    }
     px.build(d.nx);
     py.build(d.ny);
     pbx.build(d.nx2);
@@ -224,6 +243,15 @@ Case<T>::~Case() {
     fr(d_flags);
     fr(d_clock);
     fr(d_cmax);
+    for (int q = 0; q < 3; ++q) {
+        fr(sR[q]);
+        fr(sRf[q]);
+        fr(sD[q]);
+        fr(sF[q]);
+        fr(fprev[q]);
+        fr(tprev[q]);
+    }
+    fr(sP);
     fr(tavg);
     fr(tspec);
     fr(G1);
@@ -252,6 +280,7 @@ inline double uniform01(uint64_t seed, uint64_t fid, uint64_t i, uint64_t j, uin
 // u(nz) = u(nz-1), Cs_opt2 = 0.03, everything else 0, level 0 BOGUS.
 template <typename T>
 void Case<T>::seed_member(int m, long seed, double noise) {
+    touch(false);
     std::vector<std::vector<T>> f(4, std::vector<T>((size_t)d.field, (T)0));
     for (int k = 1; k < d.nz; ++k) {
         const double zk = (k - 0.5) * d.dz;
@@ -286,6 +315,7 @@ void Case<T>::seed_member(int m, long seed, double noise) {
     mjt[m] = 0;
     lm_init[m] = 0;
     clock_reset(m);
+    mcold[m] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -372,6 +402,7 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
     b += (15 + 42) * M * chunk * d.plane * T + 21 * M * chunk * d.splane * 2 * T;
     b += 3 * M * sizeof(double) + 1024 * sizeof(double) + 2 * d.splane * T;
     b += M * (CK_N * sizeof(double) + 3 * sizeof(unsigned long long) + 2 * sizeof(int));
+    if (!p.use_cfl_dt) b += 13 * M * msp * 2 * T + 6 * M * d.field * T;   // spectral state
     return b;
 }
 
@@ -550,6 +581,7 @@ static void copy_dev(const T* src, T* dst, long n, hipStream_t s) {
 template <typename T>
 void Case<T>::copy_member(int src, int dst) {
     if (src == dst) return;
+    touch(false);
     for (int id = 0; id < LG_NFIELDS; ++id)
         LG_HIP(hipMemcpyAsync(mem(id, dst), mem(id, src), d.field * sizeof(T),
                               hipMemcpyDeviceToDevice, stream));
@@ -560,6 +592,7 @@ void Case<T>::copy_member(int src, int dst) {
                           CK_N * sizeof(double), hipMemcpyDeviceToDevice, stream));
     mtime[dst] = mtime[src];
     clk_fresh[dst] = clk_fresh[src];
+    copy_spec_member(src, dst);
     if (turb || mturb[dst].n() || mturb[src].n()) {
         pull_turbine_state();
         mturb[dst] = mturb[src];
@@ -569,6 +602,7 @@ void Case<T>::copy_member(int src, int dst) {
 
 template <typename T>
 void Case<T>::retire_member(int slot) {
+    touch(false);
     const int last = Ma - 1;
     const bool disks = mturb[slot].n() > 0 || mturb[last].n() > 0;
     if (disks) pull_turbine_state();
@@ -584,6 +618,7 @@ void Case<T>::retire_member(int slot) {
                               CK_N * sizeof(double), hipMemcpyDeviceToDevice, stream));
         mtime[slot] = mtime[last];
         clk_fresh[slot] = clk_fresh[last];
+        copy_spec_member(last, slot);
     }
     mturb[last] = MemberTurbines{};
     Ma = last;
@@ -612,12 +647,12 @@ void Case<T>::filt_da(int f, int dfdx, int dfdy, cplx<T>* keep) {
 // transforms in one pass pair, the nine inverses (filtered field, d/dx, d/dy
 // of each) in another; the spectra are kept in sU, sV, sW for convec.
 template <typename T>
-void Case<T>::filt_da_uvw() {
+void Case<T>::filt_da_uvw(bool forward) {
     const T sc = (T)(1.0 / (d.nx * d.ny));
     const T* src[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
     const T scs[3] = {sc, sc, sc};
     cplx<T>* S[3] = {sU, sV, sW};
-    fwd2d(3, src, scs, d.plane, 0, d.field, S, d.splane, 0, msp, d.nz + 1);
+    if (forward) fwd2d(3, src, scs, d.plane, 0, d.field, S, d.splane, 0, msp, d.nz + 1);
     const cplx<T>* is[9] = {sU, sU, sU, sV, sV, sV, sW, sW, sW};
     const int ops[9] = {OP_NONE, OP_IKX, OP_IKY, OP_NONE, OP_IKX, OP_IKY,
                         OP_NONE, OP_IKX, OP_IKY};
@@ -899,7 +934,7 @@ void Case<T>::divstress_uvw() {
 // convec.f90
 // ---------------------------------------------------------------------------
 template <typename T>
-void Case<T>::convec(bool reuse_spectra) {
+void Case<T>::convec(bool reuse_spectra, bool spectral_out) {
     const T c = (T)(1.0 / (d.nx * d.ny));
     const int nl = d.nz;
     const cplx<T>* ps[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
@@ -942,6 +977,7 @@ void Case<T>::convec(bool reuse_spectra) {
     const T* cs[3] = {big[6], big[7], big[8]};
     cplx<T>* cd[3] = {spec[0], spec[1], spec[2]};
     fwd_big_unpad(3, cs, cd, 1, nl);
+    if (spectral_out) return;          // the advection spectra stay in spec[0..2]
     const cplx<T>* is[3] = {spec[0], spec[1], spec[2]};
     const int ops[3] = {OP_NONE, OP_NONE, OP_NONE};
     T* dst[3] = {fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ]};
@@ -1067,14 +1103,6 @@ void Case<T>::step_body() {
     project();
 }
 
-// RHS_f = RHS by exchanging the storage: RHS is fully rewritten each step.
-#define LG_SWAP_RHS()                                 \
-    do {                                              \
-        std::swap(fld[LG_RHSX], fld[LG_RHSX_F]);      \
-        std::swap(fld[LG_RHSY], fld[LG_RHSY_F]);      \
-        std::swap(fld[LG_RHSZ], fld[LG_RHSZ_F]);      \
-        swapped = !swapped;                           \
-    } while (0)
 
 // Per-member flags of the step the counters describe: Euler update and
 // Cs_opt2 = 0.03 at a fresh member's first step; dynamic-model mode (0 not
@@ -1104,6 +1132,7 @@ bool Case<T>::refresh_flags(bool force_lag) {
         hflags[m] = eu;
         hflags[M + m] = ci;
         hflags[2 * M + m] = lm;
+        hflags[5 * M + m] = (spec_on && mcold[m]) ? 1 : 0;
         if (cfl_mode) {
             hflags[3 * M + m] = clk_fresh[m] ? 1 : 0;
             hflags[4 * M + m] = (d.prm.initu || j >= d.prm.dyn_init - cs + 1) ? 1 : 0;
@@ -1131,6 +1160,343 @@ void Case<T>::advance_clocks() {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Spectral-resident step (spectral.hpp)
+// ---------------------------------------------------------------------------
+template <typename T>
+static SpecGeo<T> spec_geo(const Dims& d, long msp, int k0, int nlev) {
+    return SpecGeo<T>{d.ny, d.NK, d.nz, k0, nlev, d.splane, msp, (T)(2.0 * PI_D / d.prm.lx),
+                      (T)(2.0 * PI_D / d.prm.ly)};
+}
+
+// divstress_uv + divstress_w from the six stress spectra (levels 1..nz) into
+// sD; no inverse transform.
+template <typename T>
+void Case<T>::divstress_spec() {
+    const T sc = (T)(1.0 / (d.nx * d.ny));
+    const T* src[6] = {fld[LG_TXX], fld[LG_TXY], fld[LG_TYY],
+                       fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ]};
+    const T scs[6] = {sc, sc, sc, sc, sc, sc};
+    cplx<T>* S[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
+    fwd2d(6, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
+    const int n = d.ny * d.NK;
+    hipLaunchKernelGGL(k_spec_divt<T>, dim3((n + BLK - 1) / BLK, d.nz * Ma), dim3(BLK), 0, stream,
+                       (const cplx<T>*)spec[0], (const cplx<T>*)spec[1], (const cplx<T>*)spec[2],
+                       (const cplx<T>*)spec[3], (const cplx<T>*)spec[4], (const cplx<T>*)spec[5],
+                       sD[0], sD[1], sD[2], spec_geo<T>(d, msp, 1, d.nz), (T)(1.0 / d.dz));
+}
+
+// press_stag_array on u*, v*, w* spectra (in sU, sV, sW; the solve scales
+// them by 1/(tadv1 dt)) and the divtz spectra of levels 1 and nz; p_hat
+// into sP.
+template <typename T>
+void Case<T>::press_spec() {
+    TriGeo tg{d.ny, d.NK, d.nz, d.splane, msp, Ma, 2.0 * PI_D / d.prm.lx,
+              2.0 * PI_D / d.prm.ly, d.dz, 1.0 / (d.dz * d.dz), 1.0 / d.dz};
+    tg.hsc = (1.0 / d.tadv1) / d.prm.dt;
+    const int nsys = Ma * d.ny * d.NK;
+    const double tbytes =
+        (double)Ma * d.splane * sizeof(cplx<T>) * (3.0 * d.nz + 1 + 2 + d.nz + 1);
+    const PassChoice ch = choice_for<T>(PK_TRIDAG, d.ny, d.NK);
+    const int tb = ch.S;
+    timer.run(stream, KK_TRIDAG, tbytes, [&] {
+        LG_LAUNCH(k_tridag<T>, dim3((nsys + tb - 1) / tb), dim3(tb), 0, stream,
+                  (const cplx<T>*)sU, (const cplx<T>*)sV, (const cplx<T>*)sW,
+                  (const cplx<T>*)(sD[2] + d.splane), (const cplx<T>*)(sD[2] + d.nz * d.splane),
+                  sP, gam, usc, tg, dev_err);
+    });
+}
+
+// The loop body with the state in spectra.  cold: u, v, w and RHS_f are the
+// physical fields (their spectra are made first); warm: sU, sV, sW and sRf
+// hold them from the previous step.
+template <typename T>
+void Case<T>::step_body_spec(bool cold) {
+    const T sc = (T)(1.0 / (d.nx * d.ny));
+    if (cold) {
+        // spectra of the cold members' physical u, v, w (levels 0..nz) and
+        // RHS_f (x, y on 1..nz-1, z on 1..nz): straight into the state when
+        // every member is cold, else into scratch and then per member
+        const bool all = count_cold() == Ma;
+        const T* us[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
+        const T scs[3] = {sc, sc, sc};
+        cplx<T>* ud[3] = {sU, sV, sW};
+        cplx<T>* rd[3] = {sRf[0], sRf[1], sRf[2]};
+        if (!all) {
+            for (int q = 0; q < 3; ++q) {
+                ud[q] = spec[q];
+                rd[q] = spec[3 + q];
+            }
+        }
+        fwd2d(3, us, scs, d.plane, 0, d.field, ud, d.splane, 0, msp, d.nz + 1);
+        const T* rs[2] = {fld[LG_RHSX_F], fld[LG_RHSY_F]};
+        fwd2d(2, rs, scs, d.plane, 1, d.field, rd, d.splane, 1, msp, d.nz - 1);
+        const T* rz[1] = {fld[LG_RHSZ_F]};
+        fwd2d(1, rz, &sc, d.plane, 1, d.field, rd + 2, d.splane, 1, msp, d.nz);
+        if (!all) {
+            SelArgs<T> sa{};
+            cplx<T>* dsts[6] = {sU, sV, sW, sRf[0], sRf[1], sRf[2]};
+            for (int q = 0; q < 6; ++q) {
+                sa.src[q] = spec[q];
+                sa.dst[q] = dsts[q];
+            }
+            const long n = (long)(d.nz + 1) * d.splane;
+            hipLaunchKernelGGL(k_select_members<T>, dim3((unsigned)((n + BLK - 1) / BLK), Ma, 6),
+                               dim3(BLK), 0, stream, sa, (const int*)(d_flags + 5 * M), msp, n);
+        }
+    }
+    filt_da_uvw(false);
+    ddz_uv(LG_U, LG_DUDZ);
+    ddz_uv(LG_V, LG_DVDZ);
+    ddz_w(LG_W, LG_DWDZ);
+    wallstress();
+    sgs_stag();
+    divstress_spec();
+    convec(true, true);
+    const bool tb = turb && turb->total_nodes > 0 && kf1 >= kf0;
+    if (tb) {
+        turbines_forcing();
+        const T* fs[3] = {fld[LG_FXA], fld[LG_FYA], fld[LG_FZA]};
+        const T scs[3] = {sc, sc, sc};
+        fwd2d(3, fs, scs, d.plane, kf0, d.field, sF, d.splane, kf0, msp, kf1 - kf0 + 1);
+    }
+    const int n = d.ny * d.NK;
+    const dim3 gs((n + BLK - 1) / BLK, d.nz * Ma);
+    const SpecGeo<T> g = spec_geo<T>(d, msp, 1, d.nz);
+    SpecAb2<T> a{spec[0], spec[1], spec[2], sD[0], sD[1], sD[2],
+                 tb ? sF[0] : nullptr, tb ? sF[1] : nullptr, tb ? sF[2] : nullptr,
+                 sR[0], sR[1], sR[2], sRf[0], sRf[1], sRf[2], sU, sV, sW, d_mpf,
+                 (const int*)d_flags, d.prm.use_mean_p_force, kf0, kf1,
+                 (T)d.prm.dt, (T)d.tadv1, (T)d.tadv2};
+    hipLaunchKernelGGL(k_spec_ab2<T>, gs, dim3(BLK), 0, stream, a, g);
+    press_spec();
+    hipLaunchKernelGGL(k_spec_project<T>, gs, dim3(BLK), 0, stream, (const cplx<T>*)sP, sR[0],
+                       sR[1], sR[2], sU, sV, sW, g, (T)d.prm.dt, (T)d.tadv1, (T)d.dz);
+    hipLaunchKernelGGL(k_spec_bc<T>, dim3((n + BLK - 1) / BLK, Ma), dim3(BLK), 0, stream, sU,
+                       sV, sW, g);
+}
+
+// The storage exchange of a step (RHS_f = RHS): physical and spectral RHS,
+// and in spectral mode the disk forcing of the previous step; the staleness
+// of RHS moves to RHS_f.
+template <typename T>
+void Case<T>::swap_rhs() {
+    std::swap(fld[LG_RHSX], fld[LG_RHSX_F]);
+    std::swap(fld[LG_RHSY], fld[LG_RHSY_F]);
+    std::swap(fld[LG_RHSZ], fld[LG_RHSZ_F]);
+    swapped = !swapped;
+    if (!spec_on) return;
+    for (int q = 0; q < 3; ++q) std::swap(sR[q], sRf[q]);
+    std::swap(fld[LG_FXA], fprev[0]);
+    std::swap(fld[LG_FYA], fprev[1]);
+    std::swap(fld[LG_FZA], fprev[2]);
+    std::swap(fld[LG_TXZ], tprev[0]);
+    std::swap(fld[LG_TYZ], tprev[1]);
+    std::swap(fld[LG_TZZ], tprev[2]);
+    const bool rhs_stale = (stale & SG_RHS) != 0;
+    stale = (stale & ~(unsigned)(SG_RHS | SG_RHSF)) | (rhs_stale ? (unsigned)SG_RHSF : 0u);
+    rhsf_input = !rhs_stale;
+}
+
+template <typename T>
+void Case<T>::run_body(bool cold) {
+    if (spec_on) step_body_spec(cold);
+    else step_body();
+}
+
+template <typename T>
+void Case<T>::after_body() {
+    if (spec_on) {
+        // a step that started from physical fields leaves RHS_f as it was,
+        // except for Euler members (RHS_f = RHS)
+        const bool cold = count_cold() > 0;
+        bool any_euler = false;
+        for (int m = 0; m < Ma; ++m) any_euler |= hflags[m] != 0;
+        stale |= SG_UVW | SG_RHS | SG_P | SG_DIVT;
+        if (cold && any_euler) stale |= SG_RHSF;
+        std::fill(mcold.begin(), mcold.end(), 0);
+    }
+    if (st.on) stats_after_step();
+}
+
+// f - (its band-limited part): out = f - F^-1(trunc F(f)) on levels ka..kb.
+template <typename T>
+void Case<T>::high_part(T* const* f, T* const* out, int nf, int ka, int kb) {
+    const int nl = kb - ka + 1;
+    if (nl <= 0) return;
+    const T sc = (T)(1.0 / (d.nx * d.ny));
+    const T scs[3] = {sc, sc, sc};
+    const T* fs[3] = {f[0], f[1], f[2]};
+    cplx<T>* S[3] = {spec[0], spec[1], spec[2]};
+    fwd2d(nf, fs, scs, d.plane, ka, d.field, S, d.splane, ka, msp, nl);
+    const cplx<T>* is[3] = {spec[0], spec[1], spec[2]};
+    const int ops[3] = {OP_NONE, OP_NONE, OP_NONE};
+    cplx<T>* tsp[3] = {spec[6], spec[7], spec[8]};
+    inv2d(nf, is, ops, d.splane, ka, msp, out, d.plane, ka, d.field, nl, tsp);
+    for (int q = 0; q < nf; ++q)
+        hipLaunchKernelGGL(k_sub<T>, grid_ij(d.ld, d.ny, nl, Ma), dim3(BLK), 0, stream, out[q],
+                           (const T*)f[q], (const T*)out[q], GEO(ka, nl));
+}
+
+// The part of a step's RHS that its spectra do not carry: the
+// high-wavenumber (Nyquist) content of the disk forcing and of the vertical
+// stress derivatives, which the reference's physical RHS holds and its next
+// filt_da removes.  g = hp(f) - d/dz hp(t) with hp = x - F^-1(trunc F(x)):
+// x: hp(fx) - (hp(txz)(k+1) - hp(txz)(k))/dz, y alike with tyz, z: hp(fz) -
+// (hp(tzz)(k) - hp(tzz)(k-1))/dz on 2..nz-1; levels 1..nz-1.  f may be null
+// (no disks).  scratch: three fields.
+template <typename T>
+void Case<T>::rhs_high(T* const* f, T* const* t, T* const* g, T* const* scratch) {
+    const int nz = d.nz, n1 = nz - 1;
+    high_part(t, scratch, 3, 1, nz);
+    if (f) high_part(f, g, 3, 1, n1);
+    hipLaunchKernelGGL(k_rhs_high<T>, grid_ij(d.ld, d.ny, n1, Ma), dim3(BLK), 0, stream, g[0],
+                       g[1], g[2], (const T*)scratch[0], (const T*)scratch[1],
+                       (const T*)scratch[2], f != nullptr, GEO(1, n1), (T)(1.0 / d.dz));
+}
+
+// Make the stale physical fields of `groups` from the spectra, with the
+// high-wavenumber remainders (rhs_high) added back where the reference's
+// fields hold them.
+template <typename T>
+void Case<T>::materialize(unsigned groups) {
+    groups &= stale;
+    if (!groups) return;
+    const int nz = d.nz, n1 = nz - 1;
+    const bool tb = turb && turb->total_nodes > 0;
+    // gc: this step's remainder; gq: the previous step's (or the high part of
+    // a physical RHS_f); hs: the stress high parts for divt
+    T* gc[3] = {tmp[0], tmp[1], tmp[2]};
+    T* gq[3] = {tmp[3], tmp[4], tmp[5]};
+    T* hs[3] = {tmp[6], tmp[7], tmp[8]};
+    bool have_gq = false;
+    const unsigned need_g = SG_UVW | SG_RHS | SG_RHSF | SG_DIVT;
+    if (groups & need_g) {
+        T* f[3] = {fld[LG_FXA], fld[LG_FYA], fld[LG_FZA]};
+        T* t[3] = {fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ]};
+        rhs_high(tb ? f : nullptr, t, gc, hs);
+    }
+    if (groups & SG_DIVT) {
+        const cplx<T>* is[3] = {sD[0], sD[1], sD[2]};
+        T* dst[3] = {fld[LG_DIVTX], fld[LG_DIVTY], fld[LG_DIVTZ]};
+        cplx<T>* tsp[3] = {spec[9], spec[10], spec[11]};
+        const int on[3] = {OP_NONE, OP_NONE, OP_NONE};
+        inv2d(2, is, on, d.splane, 1, msp, dst, d.plane, 1, d.field, n1, tsp);
+        inv2d(1, is + 2, on, d.splane, 1, msp, dst + 2, d.plane, 1, d.field, nz, tsp);
+        // divt = F^-1(divt_hat) + (-(gc) without the forcing): recompute the
+        // stress part alone from hs
+        hipLaunchKernelGGL(k_divt_high<T>, grid_ij(d.ld, d.ny, n1, Ma), dim3(BLK), 0, stream,
+                           dst[0], dst[1], dst[2], (const T*)hs[0], (const T*)hs[1],
+                           (const T*)hs[2], GEO(1, n1), (T)(1.0 / d.dz));
+        fill_levels({{LG_DIVTX, 0, BOGUS}, {LG_DIVTX, nz, BOGUS}, {LG_DIVTY, 0, BOGUS},
+                     {LG_DIVTY, nz, BOGUS}, {LG_DIVTZ, 0, BOGUS}});
+    }
+    if (groups & (SG_UVW | SG_RHSF)) {
+        if (rhsf_input) {
+            T* f[3] = {fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F]};
+            high_part(f, gq, 3, 1, n1);
+        } else {
+            T* t[3] = {tprev[0], tprev[1], tprev[2]};
+            rhs_high(tb ? fprev : nullptr, t, gq, hs);
+        }
+        have_gq = true;
+    }
+    const int on[3] = {OP_NONE, OP_NONE, OP_NONE};
+    cplx<T>* tsp[3] = {spec[9], spec[10], spec[11]};
+    const T dt = (T)d.prm.dt, t1 = (T)d.tadv1, t2 = (T)d.tadv2;
+    if (groups & SG_UVW) {
+        const cplx<T>* is[3] = {sU, sV, sW};
+        T* dst[3] = {fld[LG_U], fld[LG_V], fld[LG_W]};
+        inv2d(3, is, on, d.splane, 1, msp, dst, d.plane, 1, d.field, nz, tsp);
+        // u = F^-1(u_hat) + dt (t1 gc + t2 (Euler ? gc : gq))
+        for (int q = 0; q < 3; ++q)
+            hipLaunchKernelGGL(k_add_high<T>, grid_ij(d.ld, d.ny, n1, Ma), dim3(BLK), 0, stream,
+                               fld[LG_U + q], (const T*)gc[q], have_gq ? (const T*)gq[q] : nullptr,
+                               (const int*)d_flags, GEO(1, n1), dt, t1, t2);
+        const dim3 gp((unsigned)((d.plane + BLK - 1) / BLK), Ma);
+        hipLaunchKernelGGL(k_copy_plane<T>, gp, dim3(BLK), 0, stream,
+                           (const T*)lev(LG_U, nz - 1), lev(LG_U, nz), d.field, d.plane);
+        hipLaunchKernelGGL(k_copy_plane<T>, gp, dim3(BLK), 0, stream,
+                           (const T*)lev(LG_V, nz - 1), lev(LG_V, nz), d.field, d.plane);
+        fill_levels({{LG_W, nz, 0.0}, {LG_W, 1, 0.0}, {LG_U, 0, BOGUS}, {LG_V, 0, BOGUS},
+                     {LG_W, 0, BOGUS}});
+    }
+    if (groups & SG_RHS) {
+        const cplx<T>* is[3] = {sR[0], sR[1], sR[2]};
+        T* dst[3] = {fld[LG_RHSX], fld[LG_RHSY], fld[LG_RHSZ]};
+        inv2d(3, is, on, d.splane, 1, msp, dst, d.plane, 1, d.field, nz, tsp);
+        for (int q = 0; q < 3; ++q)
+            hipLaunchKernelGGL(k_add<T>, grid_ij(d.ld, d.ny, n1, Ma), dim3(BLK), 0, stream,
+                               fld[LG_RHSX + q], (const T*)gc[q], GEO(1, n1));
+        fill_levels({{LG_RHSX, 0, BOGUS}, {LG_RHSY, 0, BOGUS}, {LG_RHSZ, 0, BOGUS},
+                     {LG_RHSX, nz, BOGUS}, {LG_RHSY, nz, BOGUS}});
+    }
+    if (groups & SG_RHSF) {
+        // RHS_f = the previous RHS (or the physical input of a step that
+        // started from it), or this step's (before - grad p) for an Euler
+        // member
+        const cplx<T>* is[3] = {sRf[0], sRf[1], sRf[2]};
+        T* dst[3] = {fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F]};
+        inv2d(3, is, on, d.splane, 1, msp, dst, d.plane, 1, d.field, nz, tsp);
+        for (int q = 0; q < 3; ++q)
+            hipLaunchKernelGGL(k_add_high<T>, grid_ij(d.ld, d.ny, n1, Ma), dim3(BLK), 0, stream,
+                               fld[LG_RHSX_F + q], (const T*)gc[q], (const T*)gq[q],
+                               (const int*)d_flags, GEO(1, n1), (T)1, (T)0, (T)1);
+        fill_levels({{LG_RHSX_F, 0, BOGUS}, {LG_RHSY_F, 0, BOGUS}, {LG_RHSZ_F, 0, BOGUS},
+                     {LG_RHSX_F, nz, BOGUS}, {LG_RHSY_F, nz, BOGUS}});
+    }
+    if (groups & SG_P) {
+        const cplx<T>* is[1] = {sP};
+        T* dst[1] = {fld[LG_P]};
+        inv2d(1, is, on, d.splane, 0, msp, dst, d.plane, 0, d.field, nz + 1, tsp);
+        const cplx<T>* is2[2] = {sP, sP};
+        const int ops[2] = {OP_IKX, OP_IKY};
+        T* dst2[2] = {fld[LG_DPDX], fld[LG_DPDY]};
+        inv2d(2, is2, ops, d.splane, 1, msp, dst2, d.plane, 1, d.field, n1, tsp);
+        hipLaunchKernelGGL(k_dpdz<T>, grid_ij(d.nx, d.ny, nz, Ma), dim3(BLK), 0, stream,
+                           (const T*)fld[LG_P], fld[LG_DPDZ], GEO(1, nz), (T)d.dz);
+        fill_levels({{LG_DPDX, nz, BOGUS}, {LG_DPDY, nz, BOGUS}});
+    }
+    stale &= ~groups;
+    LG_HIP(hipGetLastError());
+}
+
+// Before the host reads (modify = false) or changes (true) physical
+// fields: make the stale ones; a change also makes the next step start
+// from the physical fields.
+template <typename T>
+void Case<T>::touch(bool modify) {
+    if (!spec_on) return;
+    materialize(SG_ALL);
+    if (modify) std::fill(mcold.begin(), mcold.end(), 1);
+}
+
+template <typename T>
+int Case<T>::count_cold() const {
+    int n = 0;
+    for (int m = 0; m < Ma; ++m) n += mcold[m] ? 1 : 0;
+    return n;
+}
+
+// Member dst takes member src's spectral state (with its physical fields
+// copied by the caller, the two members step alike).
+template <typename T>
+void Case<T>::copy_spec_member(int src, int dst) {
+    if (!spec_on) return;
+    cplx<T>* bufs[13] = {sU, sV, sW, sR[0], sR[1], sR[2], sRf[0], sRf[1], sRf[2],
+                         sP, sD[0], sD[1], sD[2]};
+    for (cplx<T>* b : bufs)
+        LG_HIP(hipMemcpyAsync(b + (long)dst * msp, b + (long)src * msp, msp * sizeof(cplx<T>),
+                              hipMemcpyDeviceToDevice, stream));
+    for (int q = 0; q < 3; ++q) {
+        LG_HIP(hipMemcpyAsync(fprev[q] + (long)dst * d.field, fprev[q] + (long)src * d.field,
+                              d.field * sizeof(T), hipMemcpyDeviceToDevice, stream));
+        LG_HIP(hipMemcpyAsync(tprev[q] + (long)dst * d.field, tprev[q] + (long)src * d.field,
+                              d.field * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    }
+    mcold[dst] = mcold[src];
+}
+
 template <typename T>
 void Case<T>::step() {
     ++jt;
@@ -1138,9 +1504,10 @@ void Case<T>::step() {
     for (int m = 0; m < Ma; ++m) ++mjt[m];
     refresh_flags();
     advance_clocks();
-    LG_SWAP_RHS();
-    step_body();
-    if (st.on) stats_after_step();
+    const bool cold = spec_on && count_cold() > 0;
+    swap_rhs();
+    run_body(cold);
+    after_body();
 }
 
 template <typename T>
@@ -1155,8 +1522,8 @@ void Case<T>::drop_graphs() {
 // One step replayed from a captured graph.  A step's launches depend on the
 // RHS storage parity and on whether the dynamic model runs, so up to four
 // graphs are captured, each on first use (and again after the member count
-// or the disk set changes).  A member's first step and its first dynamic
-// call run eagerly.
+// or the disk set changes).  A member's first step, its first dynamic call
+// and a step that starts from physical fields run eagerly.
 template <typename T>
 void Case<T>::step_graph() {
     ++jt;
@@ -1164,10 +1531,11 @@ void Case<T>::step_graph() {
     for (int m = 0; m < Ma; ++m) ++mjt[m];
     const bool special = refresh_flags();
     advance_clocks();
-    LG_SWAP_RHS();
-    if (special) {
-        step_body();
-        if (st.on) stats_after_step();
+    const bool cold = spec_on && count_cold() > 0;
+    swap_rhs();
+    if (special || cold) {
+        run_body(cold);
+        after_body();
         return;
     }
     const int key = (swapped ? 1 : 0) + (lag_now ? 2 : 0);
@@ -1179,7 +1547,7 @@ void Case<T>::step_graph() {
         LG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipGraph_t g = nullptr;
         try {
-            step_body();
+            run_body(false);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -1196,7 +1564,7 @@ void Case<T>::step_graph() {
         graph_ma[key] = Ma;
     }
     LG_HIP(hipGraphLaunch(graphs[key], stream));
-    if (st.on) stats_after_step();
+    after_body();
 }
 
 // ---------------------------------------------------------------------------
@@ -1218,6 +1586,7 @@ double Case<T>::rmsdiv(int m) {
 
 template <typename T>
 double Case<T>::energy(int m) {
+    touch(false);
     const int nb = 256;
     hipLaunchKernelGGL(k_energy_part<T>, dim3(nb), dim3(BLK), 0, stream, (const T*)mem(LG_U, m),
                        (const T*)mem(LG_V, m), (const T*)mem(LG_W, m), GEO(1, d.nz), red);
@@ -1231,6 +1600,7 @@ double Case<T>::energy(int m) {
 
 template <typename T>
 double Case<T>::max_cfl(int m) {
+    touch(false);
     const int nb = 256;
     double mx3[3];
     const double h3[3] = {d.dx, d.dy, d.dz};

@@ -103,6 +103,24 @@ struct Case {
     unsigned long long* d_cmax = nullptr;  // 3 per member: max |u|, |v|, |w|
     std::vector<double> mtime;       // total_time per member (fixed dt)
     std::vector<char> clk_fresh;     // the member's clock holds no previous step
+    // spectral-resident stepping (spectral.hpp): with a fixed dt the step
+    // keeps u, v, w, RHS, RHS_f, p and divt as spectra between steps and
+    // makes the physical fields on demand (materialize).  mcold[m]: member
+    // m's next step starts from its physical fields (its spectra do not
+    // describe it); stale: the physical groups not yet made.
+    enum : unsigned { SG_UVW = 1, SG_RHS = 2, SG_RHSF = 4, SG_P = 8, SG_DIVT = 16, SG_ALL = 31 };
+    bool spec_on = false;
+    std::vector<char> mcold;
+    bool rhsf_input = false;         // the physical RHS_f was valid when the last step began
+    unsigned stale = 0;
+    cplx<T>* sR[3] = {};             // RHS spectra (after - grad p)
+    cplx<T>* sRf[3] = {};            // RHS_f spectra
+    cplx<T>* sP = nullptr;           // pressure spectrum, levels 0..nz
+    cplx<T>* sD[3] = {};             // stress divergence spectra
+    cplx<T>* sF[3] = {};             // forcing spectra of the step (disk levels)
+    T* fprev[3] = {};                // the previous step's fxa, fya, fza
+    T* tprev[3] = {};                // the previous step's txz, tyz, tzz
+    int kf0 = 1, kf1 = 0;            // levels where the disks force
     // running time averages (time_average.f90) and streamwise spectra
     struct StatsCfg {
         int nstart = 0, nend = -1, nskip = 1;
@@ -184,9 +202,9 @@ struct Case {
     void divstress_w();
     // the step's fused forms: filt_da of u, v, w in one pass pair; both
     // divstress routines with one forward and one inverse pass
-    void filt_da_uvw();
+    void filt_da_uvw(bool forward = true);
     void divstress_uvw();
-    void convec(bool reuse_spectra = false);
+    void convec(bool reuse_spectra = false, bool spectral_out = false);
     void press_stag_array();
     void project();
     void rhs_ab2();
@@ -199,6 +217,18 @@ struct Case {
     void clock_set(int m, double dt, double total_time, bool fresh);
     void advance_clocks();
     void step_body();
+    void step_body_spec(bool cold);
+    void swap_rhs();
+    int count_cold() const;
+    void copy_spec_member(int src, int dst);
+    void run_body(bool cold);
+    void after_body();
+    void divstress_spec();
+    void press_spec();
+    void materialize(unsigned groups);
+    void touch(bool modify);
+    void high_part(T* const* f, T* const* out, int nf, int ka, int kb);
+    void rhs_high(T* const* f, T* const* t, T* const* g, T* const* scratch);
     void step();
     void step_graph();
     void drop_graphs();

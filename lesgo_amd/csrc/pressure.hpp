@@ -14,6 +14,7 @@ struct TriGeo {
     int M;
     double kxs, kys;     // 2 pi / Lx, 2 pi / Ly
     double dz, const3, const4;
+    double hsc = 1.0;    // factor on Hx, Hy, Hz (1/(tadv1 dt) for velocity spectra)
 };
 
 // One thread per (member, ky, kx < nx/2).  Hx, Hy hold levels 1..nz-1, Hz levels
@@ -45,6 +46,11 @@ __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz
         const cplx<T> v = p[o];
         return make_double2((double)v.x, (double)v.y);
     };
+    const double hs = g.hsc;
+    auto ldh = [&](const cplx<T>* p, long o) -> double2 {
+        const cplx<T> v = p[o];
+        return make_double2((double)v.x * hs, (double)v.y * hs);
+    };
     auto st = [&](long o, double2 v) { ph[o] = cplx<T>{(T)v.x, (T)v.y}; };
     if (jy == g.ny / 2) {
         for (int m = 0; m < n; ++m) st((long)m * S + idx, make_double2(0.0, 0.0));
@@ -58,7 +64,7 @@ __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz
         double2 pk = make_double2(p0.x - g.dz * b.x, p0.y - g.dz * b.y);
         st(S + idx, pk);
         for (int m = 2; m <= g.nz; ++m) {
-            const double2 h = ld(Hz, (long)m * S + idx);
+            const double2 h = ldh(Hz, (long)m * S + idx);
             pk = make_double2(pk.x + h.x * g.dz, pk.y + h.y * g.dz);
             st((long)m * S + idx, pk);
         }
@@ -82,10 +88,10 @@ __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz
             a = a_in;
             b = b_in;
             cc = c_in;
-            const double2 hx = ld(Hx, (long)m * S + idx);
-            const double2 hy = ld(Hy, (long)m * S + idx);
-            const double2 z1 = ld(Hz, (long)(m + 1) * S + idx);
-            const double2 z0 = ld(Hz, (long)m * S + idx);
+            const double2 hx = ldh(Hx, (long)m * S + idx);
+            const double2 hy = ldh(Hy, (long)m * S + idx);
+            const double2 z1 = ldh(Hz, (long)(m + 1) * S + idx);
+            const double2 z0 = ldh(Hz, (long)m * S + idx);
             // i kx Hx + i ky Hy + (Hz(m+1) - Hz(m)) / dz
             const double ahx_r = -hx.y * kx, ahx_i = hx.x * kx;
             const double ahy_r = -hy.y * ky, ahy_i = hy.x * ky;

@@ -167,6 +167,7 @@ struct Server {
         Job& job = jobs[j];
         p.with([&](auto& c) {
             if (job.keep) {
+                c.touch(false);
                 using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
                 std::vector<T> tmp((size_t)c.d.field);
                 for (int id : {LG_U, LG_V, LG_W, LG_P}) {

@@ -145,6 +145,7 @@ void Case<T>::stats_after_step() {
             hipLaunchKernelGGL(k_clock_stats, dim3(1), dim3(64), 0, stream, d_clock, Ma, 1);
         return;
     }
+    materialize(SG_UVW | SG_P);
     const long pm = (long)(d.nz + 1) * d.ny * d.nx;
     TavgArgs<T> a{fld[LG_U],    fld[LG_V],    fld[LG_W],    fld[LG_P],    fld[LG_TXX],
                   fld[LG_TYY],  fld[LG_TZZ],  fld[LG_TXY],  fld[LG_TXZ],  fld[LG_TYZ],

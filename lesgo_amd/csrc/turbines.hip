@@ -270,6 +270,8 @@ void Case<T>::pull_turbine_state() {
 // every step.
 template <typename T>
 void Case<T>::rebuild_turbines() {
+    // the physical fields are made before the forcing arrays are cleared
+    touch(false);
     LG_HIP(hipStreamSynchronize(stream));
     drop_graphs();
     turb.reset();
@@ -279,10 +281,14 @@ void Case<T>::rebuild_turbines() {
     std::vector<double> ind, coef, st;
     ts->first.push_back(0);
     int s = 0;
+    int kmin = d.nz + 1, kmax = 0;
     for (int m = 0; m < M; ++m) {
         const MemberTurbines& mt = mturb[m];
         for (int q = 0; q < mt.n(); ++q, ++s) {
             for (int l = mt.off[q]; l < mt.off[q + 1]; ++l) {
+                const int k = (int)(mt.lin[l] / d.plane);
+                kmin = std::min(kmin, k);
+                kmax = std::max(kmax, k);
                 lin.push_back((long)m * d.field + mt.lin[l]);
                 ind.push_back(mt.ind[l]);
                 tid.push_back(s);
@@ -297,8 +303,18 @@ void Case<T>::rebuild_turbines() {
     }
     for (int id : {LG_FXA, LG_FYA, LG_FZA})
         LG_HIP(hipMemsetAsync(fld[id], 0, M * d.field * sizeof(T), stream));
+    for (int q = 0; q < 3; ++q)
+        if (fprev[q]) LG_HIP(hipMemsetAsync(fprev[q], 0, M * d.field * sizeof(T), stream));
     LG_HIP(hipMemsetAsync(tmp[8], 0, M * d.field * sizeof(T), stream));
     LG_HIP(hipStreamSynchronize(stream));
+    // levels where the forcing can be non-zero: the nodes' levels and, for
+    // fza on the w grid, one above
+    kf0 = 1;
+    kf1 = 0;
+    if (kmax >= kmin) {
+        kf0 = std::max(1, kmin);
+        kf1 = std::min(d.nz, kmax + 1);
+    }
     if (s == 0) return;
     ts->n = s;
     ts->total_nodes = (int)lin.size();

@@ -150,6 +150,10 @@ interface
         import :: c_int, c_ptr
         type(c_ptr), value :: c
     end function
+    integer(c_int) function lg_resync(c) bind(C, name='lg_resync')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+    end function
     integer(c_int) function lg_clock_member(c, m, out) bind(C, name='lg_clock_member')
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: c

@@ -165,6 +165,8 @@ subroutine checkpoint()
 ! io.f90:1173-1267: vel.out, grid.out, total_time.dat (one process)
 integer :: un
 real(rprec) :: cfl_w
+! the written fields are the state from here on, as in a restart from them
+call gpu_check(lg_resync(gpu_case), 'lg_resync')
 call gpu_download(u); call gpu_download(v); call gpu_download(w)
 call gpu_download(RHSx); call gpu_download(RHSy); call gpu_download(RHSz)
 call gpu_download(Cs_opt2); call gpu_download(F_LM); call gpu_download(F_MM)
