@@ -469,11 +469,28 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         const int nj = std::min(MAX_JOBS, njobs - j0);
         ColArgs<T> c{};
         c.plan = py.plan;
+        // an OP_IKX job on the same input as an OP_NONE job rides along with
+        // it: one y transform, two outputs (i kx is constant along y)
+        int nc = 0;
+        bool taken[MAX_JOBS] = {};
         for (int i = 0; i < nj; ++i) {
-            c.src[i] = src[j0 + i];
-            c.src2[i] = src2 ? src2[j0 + i] : nullptr;
-            c.dst[i] = tmpspec[i];
-            c.op[i] = ops[j0 + i];
+            if (taken[i]) continue;
+            c.src[nc] = src[j0 + i];
+            c.src2[nc] = src2 ? src2[j0 + i] : nullptr;
+            c.dst[nc] = tmpspec[i];
+            c.dst2[nc] = nullptr;
+            c.op[nc] = ops[j0 + i];
+            if (ops[j0 + i] == OP_NONE && !c.src2[nc]) {
+                for (int i2 = i + 1; i2 < nj; ++i2) {
+                    if (!taken[i2] && ops[j0 + i2] == OP_IKX && src[j0 + i2] == src[j0 + i] &&
+                        !(src2 && src2[j0 + i2])) {
+                        c.dst2[nc] = tmpspec[i2];
+                        taken[i2] = true;
+                        break;
+                    }
+                }
+            }
+            ++nc;
         }
         c.src_plane = src_plane;
         c.dst_plane = d.splane;
@@ -484,7 +501,7 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         c.map = MAP_ID;
         c.zero_nyq = 0;
         col_common(c, d, G1, G2);
-        launch_cols<T, +1>(timer, c, nj, nlev, Ma, d.ny, stream, nullptr);
+        launch_cols<T, +1>(timer, c, nc, nlev, Ma, d.ny, stream, nullptr);
         std::vector<const cplx<T>*> ts(tmpspec, tmpspec + nj);
         launch_rows_c2r<T>(timer, px.plan, nj, ts.data(), d.splane, 0, tms, dst + j0, dst_plane,
                            d.ld, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr);

@@ -248,7 +248,10 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
     c.nlev = nlev;
     const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
     dim3 grid(nj, (c.ncols + c.TC - 1) / c.TC, nlev * M);
-    const double bytes = (double)nj * nlev * M * c.ncols * sizeof(cplx<T>) * (ny_in + N);
+    int n2out = 0;
+    for (int i = 0; i < nj; ++i) n2out += c.dst2[i] ? 1 : 0;
+    const double bytes =
+        (double)nlev * M * c.ncols * sizeof(cplx<T>) * ((double)nj * (ny_in + N) + n2out * N);
     tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
         if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
             LG_LAUNCH((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
