@@ -208,6 +208,18 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
             g2[(size_t)j * d.NK + i] = (T)b;
         }
     }
+    for (int f = 0; f < 2; ++f) {
+        const std::vector<T>& g = f ? g2 : g1;
+        int nzc = 0, yk = 0;
+        for (int j = 0; j < d.ny; ++j)
+            for (int i = 0; i < d.NK; ++i)
+                if (g[(size_t)j * d.NK + i] != (T)0) {
+                    nzc = std::max(nzc, i + 1);
+                    yk = std::max(yk, ((j < d.ny / 2) ? j : d.ny - j) + 1);
+                }
+        g_nzc[f] = std::max(nzc, 1);
+        g_ykeep[f] = std::max(yk, 1);
+    }
     G1 = dalloc<T>(d.splane);
     G2 = dalloc<T>(d.splane);
     LG_HIP(hipMemcpy(G1, g1.data(), d.splane * sizeof(T), hipMemcpyHostToDevice));
@@ -434,11 +446,11 @@ static void col_common(ColArgs<T>& c, const Dims& d, const T* G1, const T* G2) {
 template <typename T>
 void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
                     long src_ms, cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms,
-                    int nlev, const double* mscale) {
+                    int nlev, const double* mscale, int band_nzc, int band_ykeep) {
     if (nlev <= 0) return;
     launch_rows_r2c<T>(timer, px.plan, njobs, src, scales, src_plane, d.ld, src_lev0, src_ms, dst,
                        dst_plane, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr,
-                       mscale, CK_N);
+                       mscale, CK_N, band_nzc);
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
         ColArgs<T> c{};
@@ -447,6 +459,8 @@ void Case<T>::fwd2d(int njobs, const T* const* src, const T* scales, long src_pl
             c.src[i] = dst[j0 + i];
             c.dst[i] = dst[j0 + i];
             c.op[i] = OP_NONE;
+            c.nzc[i] = band_nzc;
+            c.ykeep[i] = band_ykeep;
         }
         c.src_plane = c.dst_plane = dst_plane;
         c.lev0_src = c.lev0_dst = dst_lev0;
@@ -480,6 +494,12 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
             c.dst[nc] = tmpspec[i];
             c.dst2[nc] = nullptr;
             c.op[nc] = ops[j0 + i];
+            if (ops[j0 + i] == OP_G1 || ops[j0 + i] == OP_G2) {
+                // test-filtered: nothing outside the filter's band
+                const int f = ops[j0 + i] == OP_G2 ? 1 : 0;
+                c.nzc[nc] = g_nzc[f];
+                c.ykeep[nc] = g_ykeep[f];
+            }
             if (ops[j0 + i] == OP_NONE && !c.src2[nc]) {
                 for (int i2 = i + 1; i2 < nj; ++i2) {
                     if (!taken[i2] && ops[j0 + i2] == OP_IKX && src[j0 + i2] == src[j0 + i] &&
@@ -503,8 +523,11 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         col_common(c, d, G1, G2);
         launch_cols<T, +1>(timer, c, nc, nlev, Ma, d.ny, stream, nullptr);
         std::vector<const cplx<T>*> ts(tmpspec, tmpspec + nj);
+        int nce[MAX_JOBS] = {};
+        for (int i = 0; i < nj; ++i)
+            nce[i] = ops[j0 + i] == OP_G1 ? g_nzc[0] : ops[j0 + i] == OP_G2 ? g_nzc[1] : 0;
         launch_rows_c2r<T>(timer, px.plan, nj, ts.data(), d.splane, 0, tms, dst + j0, dst_plane,
-                           d.ld, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr);
+                           d.ld, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr, nce);
     }
 }
 
@@ -844,10 +867,14 @@ void Case<T>::lagrange_sdep() {
             psrc[9 + q] = lag_phys + (9 + q) * Qp;
             pdst[9 + q] = dst[15 + q];
         }
-        fwd2d(15, psrc, nullptr, d.plane, 0, mp, pdst, d.splane, 0, ms, nc);
+        // only the test filters read these spectra: the G1 band (G2's lies
+        // inside it)
+        fwd2d(15, psrc, nullptr, d.plane, 0, mp, pdst, d.splane, 0, ms, nc, nullptr, g_nzc[0],
+              g_ykeep[0]);
         const T* ssrc[6];
         for (int q = 0; q < 6; ++q) ssrc[q] = fld[sids[q]];
-        fwd2d(6, ssrc, nullptr, d.plane, k0, d.field, dst + 9, d.splane, 0, ms, nc);
+        fwd2d(6, ssrc, nullptr, d.plane, k0, d.field, dst + 9, d.splane, 0, ms, nc, nullptr,
+              g_nzc[0], g_ykeep[0]);
         const cplx<T>* isrc[42];
         int ops[42];
         T* idst[42];

@@ -160,6 +160,9 @@ struct Case {
     double* red = nullptr;           // reduction scratch
     T* G1 = nullptr;                 // [ny][NK]
     T* G2 = nullptr;
+    // the filters' bands: columns kx < g_nzc[f] and rows |ky index| <
+    // g_ykeep[f] hold every nonzero entry of G1 (f = 0) and G2 (f = 1)
+    int g_nzc[2] = {0, 0}, g_ykeep[2] = {0, 0};
     PlanTables px, py, pbx, pby;     // nx, ny, nx2, ny2
     std::unique_ptr<TurbineSet> turb;
     // u_d_T time filter (turbines.f90:561-566): T_avg_dim <= 0 is none
@@ -176,9 +179,11 @@ struct Case {
     T* mem(int id, int m) { return fld[id] + (long)m * d.field; }
 
     // 2-D transform helpers over planes [lev0, lev0+nlev) of every member
+    // band (nzc, ykeep; 0: none): the spectra are only read inside the test
+    // filters' band, so only that part is computed and written
     void fwd2d(int njobs, const T* const* src, const T* scales, long src_plane, int src_lev0,
                long src_ms, cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms,
-               int nlev, const double* mscale = nullptr);
+               int nlev, const double* mscale = nullptr, int band_nzc = 0, int band_ykeep = 0);
     void inv2d(int njobs, const cplx<T>* const* src, const int* ops, long src_plane, int src_lev0,
                long src_ms, T* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nlev,
                cplx<T>* const* tmpspec, const cplx<T>* const* src2 = nullptr);

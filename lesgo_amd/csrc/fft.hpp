@@ -193,6 +193,8 @@ struct RowR2CArgs {
     // per-member input scale (replaces scale[] when set): mscale[m * stride]
     const double* mscale;
     int mscale_stride;
+    int nwr;               // columns of dst written (0: ncols; a band-limited consumer
+                           // reads no more; the four-step and generic kernels honour it)
 };
 
 template <typename T>
@@ -221,9 +223,10 @@ __global__ void __launch_bounds__(256) k_rows_r2c(RowR2CArgs<T> a) {
     __syncthreads();
     fft_stages<T, -1>(lds, S, a.stride, a.plan);
     const int nc = a.ncols;
-    for (int idx = threadIdx.x; idx < S * nc; idx += blockDim.x) {
-        const int s = idx / nc;
-        const int k = idx - s * nc;
+    const int nw = a.nwr ? a.nwr : nc;
+    for (int idx = threadIdx.x; idx < S * nw; idx += blockDim.x) {
+        const int s = idx / nw;
+        const int k = idx - s * nw;
         const int j = 2 * (pair0 + s);
         const cplx<T> z = lds[s * a.stride + k];
         const cplx<T> zc = lds[s * a.stride + ((N - k) % N)];
@@ -259,6 +262,8 @@ struct RowC2RArgs {
     int nlev;
     long src_mstride;      // complex elements between members of src
     long dst_mstride;      // elements between members of dst
+    int nce[MAX_JOBS];     // columns of src read per job (0: ncols); the rest are
+                           // taken as zero (band-limited spectra)
 };
 
 template <typename T>
@@ -276,6 +281,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r(RowC2RArgs<T> a) {
     T* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int* perm = a.plan.perm;
     const int nc = a.ncols;
+    const int nce = a.nce[job] ? a.nce[job] : nc;
     const int half = N / 2;
     for (int idx = threadIdx.x; idx < S * N; idx += blockDim.x) {
         const int s = idx / N;
@@ -287,7 +293,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r(RowC2RArgs<T> a) {
         if (k == 0) {
             z = cplx<T>{ra[0].x, rb[0].x};
         } else if (k < half) {
-            if (k < nc) {
+            if (k < nce) {
                 const cplx<T> A = ra[k], B = rb[k];
                 z = cplx<T>{A.x - B.y, A.y + B.x};
             } else {
@@ -297,7 +303,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r(RowC2RArgs<T> a) {
             z = cplx<T>{(T)0, (T)0};
         } else {
             const int kk = N - k;
-            if (kk < nc) {
+            if (kk < nce) {
                 const cplx<T> A = ra[kk], B = rb[kk];
                 // conj(A) + i conj(B)
                 z = cplx<T>{A.x + B.y, B.x - A.y};
@@ -372,7 +378,23 @@ struct ColArgs {
     T kys;             // 2 pi / Ly
     const T* G1;       // [ny][ncols] filter tables
     const T* G2;
+    // band limits of band-limited (test-filtered) spectra, per job, 0: none.
+    // nzc: only columns c < nzc are transformed (tiles past it do nothing).
+    // ykeep: only rows with |ky index| < ykeep (not ny/2) carry data: an
+    // inverse MAP_ID pass reads no others (they are zero), a forward MAP_ID
+    // pass writes no others.
+    int nzc[MAX_JOBS];
+    int ykeep[MAX_JOBS];
 };
+
+// Row jb of a small-grid spectrum inside the band |m| < ykeep (0: all rows).
+__device__ __forceinline__ bool in_yband(int jb, int ny, int ykeep) {
+    if (!ykeep) return true;
+    const int nyh = ny / 2;
+    if (jb == nyh) return false;
+    const int m = jb < nyh ? jb : ny - jb;
+    return m < ykeep;
+}
 
 // i kx v + i ky v2 at small-grid index (c, j) (Nyquist row: both 0)
 template <typename T>
@@ -410,7 +432,10 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
     const int N = a.plan.N;
     const int job = blockIdx.x;
     const int c0 = blockIdx.y * a.TC;
-    const int tc = min(a.TC, a.ncols - c0);
+    const int nzc = a.nzc[job] ? a.nzc[job] : a.ncols;
+    if (c0 >= nzc) return;
+    const int tc = min(a.TC, nzc - c0);
+    const int ykeep = a.ykeep[job];
     const int mem = blockIdx.z / a.nlev;
     const int plane = blockIdx.z - mem * a.nlev;
     const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
@@ -433,7 +458,8 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
             else if (jb > N - nyh) j = jb - (N - ny);
             if (j >= 0) v = apply_op<T>(op, src[(long)j * a.ncols + c0 + c], c0 + c, j, a);
         } else {
-            v = src[(long)jb * a.ncols + c0 + c];
+            v = (DIR > 0 && !in_yband(jb, ny, ykeep)) ? cplx<T>{(T)0, (T)0}
+                                                      : src[(long)jb * a.ncols + c0 + c];
             if (DIR > 0 && a.map == MAP_ID) {
                 if (dst2 && jb == nyh) v = cplx<T>{(T)0, (T)0};
                 if (op == OP_DIVXY) {
@@ -467,7 +493,7 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
                 if (a.zero_nyq && j == nyh) v = cplx<T>{(T)0, (T)0};
                 v = apply_op<T>(op, v, c0 + c, j, a);
             }
-            dst[(long)j * a.ncols + c0 + c] = v;
+            if (DIR > 0 || in_yband(j, ny, ykeep)) dst[(long)j * a.ncols + c0 + c] = v;
             if (DIR > 0 && dst2) {
                 const T kx = a.kxs * (T)(c0 + c);
                 dst2[(long)j * a.ncols + c0 + c] = cplx<T>{-v.y * kx, v.x * kx};

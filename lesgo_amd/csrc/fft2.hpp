@@ -166,9 +166,10 @@ __global__ void __launch_bounds__(256) k_rows_r2c_v2(RowR2CArgs<T> a) {
     }
     __syncthreads();
     const int nc = a.ncols;
-    for (int idx = threadIdx.x; idx < S * nc; idx += blockDim.x) {
-        const int s = idx / nc;
-        const int k = idx - s * nc;
+    const int nw = a.nwr ? a.nwr : nc;
+    for (int idx = threadIdx.x; idx < S * nw; idx += blockDim.x) {
+        const int s = idx / nw;
+        const int k = idx - s * nw;
         const int j = 2 * (pair0 + s);
         const cplx<T>* X = lds + s * SP::YS;
         const cplx<T> z = X[k];
@@ -200,6 +201,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
     const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
     T* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int nc = a.ncols;
+    const int nce = a.nce[job] ? a.nce[job] : nc;
     for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
         const int s = t / N2, n2 = t - s * N2;
         const int j = 2 * (pair0 + s);
@@ -213,13 +215,13 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
             if (k == 0) {
                 z = cplx<T>{gld(ra).x, gld(rb).x};
             } else if (k < half) {
-                if (k < nc) {
+                if (k < nce) {
                     const cplx<T> A = gld(ra + k), B = gld(rb + k);
                     z = cplx<T>{A.x - B.y, A.y + B.x};
                 }
             } else if (k > half) {
                 const int kk = N - k;
-                if (kk < nc) {
+                if (kk < nce) {
                     const cplx<T> A = gld(ra + kk), B = gld(rb + kk);
                     z = cplx<T>{A.x + B.y, B.x - A.y};
                 }
@@ -273,7 +275,10 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
     const int job = blockIdx.x;
     const int TC = a.TC;
     const int c0 = blockIdx.y * TC;
-    const int tc = min(TC, a.ncols - c0);
+    const int nzc = a.nzc[job] ? a.nzc[job] : a.ncols;
+    if (c0 >= nzc) return;
+    const int tc = min(TC, nzc - c0);
+    const int ykeep = a.ykeep[job];
     const int mem = blockIdx.z / a.nlev;
     const int plane = blockIdx.z - mem * a.nlev;
     const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
@@ -299,7 +304,8 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 else if (jb > N - nyh) j = jb - (N - ny);
                 if (j >= 0) v = apply_op<T>(op, gld(src + (long)j * a.ncols + c0 + c), c0 + c, j, a);
             } else {
-                v = gld(src + (long)jb * a.ncols + c0 + c);
+                v = (DIR > 0 && !in_yband(jb, a.ny, ykeep)) ? cplx<T>{(T)0, (T)0}
+                                                              : gld(src + (long)jb * a.ncols + c0 + c);
                 if (DIR > 0 && a.map == MAP_ID) {
                     if (dst2 && jb == nyh) v = cplx<T>{(T)0, (T)0};
                     if (op == OP_DIVXY) {
@@ -343,7 +349,7 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                     if (a.zero_nyq && jb == nyh) v = cplx<T>{(T)0, (T)0};
                     v = apply_op<T>(op, v, c0 + c, jb, a);
                 }
-                gst(dst + (long)jb * a.ncols + c0 + c, v);
+                if (DIR > 0 || in_yband(jb, ny, ykeep)) gst(dst + (long)jb * a.ncols + c0 + c, v);
                 if (DIR > 0 && dst2) {
                     const T kx = a.kxs * (T)(c0 + c);
                     gst(dst2 + (long)jb * a.ncols + c0 + c, cplx<T>{-v.y * kx, v.x * kx});

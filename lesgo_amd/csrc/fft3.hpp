@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v3(RowC2RArgs<T> a) {
     const int pair0 = blockIdx.x * a.S;
     const int S = min(a.S, npairs - pair0);
     const int nc = a.ncols;
+    const int nce = a.nce[job] ? a.nce[job] : nc;
     const int ld = a.dst_ld;
     T* stage = reinterpret_cast<T*>(smem_c3);
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c3 + ((stage_elems(a.S, ld, nc) * sizeof(T) + 15) & ~15L));
@@ -149,13 +150,13 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v3(RowC2RArgs<T> a) {
             if (k == 0) {
                 z = cplx<T>{ra[0].x, rb[0].x};
             } else if (k < half) {
-                if (k < nc) {
+                if (k < nce) {
                     const cplx<T> A = ra[k], B = rb[k];
                     z = cplx<T>{A.x - B.y, A.y + B.x};
                 }
             } else if (k > half) {
                 const int kk = N - k;
-                if (kk < nc) {
+                if (kk < nce) {
                     const cplx<T> A = ra[kk], B = rb[kk];
                     z = cplx<T>{A.x + B.y, B.x - A.y};
                 }

@@ -28,28 +28,36 @@ __device__ __forceinline__ int cell_xy(double& px, double L, double d, int n) {
     return min(max((int)floor(r / d) + 1, 1), n);
 }
 
-// trilinear_interp_w (functions.f90:349-474) for lbc_mom > 0, ubc_mom = 0.
-template <typename T>
-__device__ double trilinear_w(const T* var, const InterpGeo& g, double px, double py, double pz) {
+// trilinear_interp_w (functions.f90:349-474) for lbc_mom > 0, ubc_mom = 0,
+// split into the stencil of a point (cell corners and offsets, found once)
+// and its application to each field.
+struct TriStencil {
+    long o00, o01, o10, o11;   // (j0, i0), (j0, i1), (j1, i0), (j1, i1) at level k0
+    long dk;                   // plane offset from k0 to k1
+    double xdiff, ydiff, zdiff;
+};
+
+__device__ __forceinline__ TriStencil tri_stencil(const InterpGeo& g, double px, double py,
+                                                  double pz) {
     const int i0 = cell_xy(px, g.lx, g.dx, g.nx);
     const int j0 = cell_xy(py, g.ly, g.dy, g.ny);
     const int i1 = (i0 + 1 > g.nx) ? 1 : i0 + 1;
     const int j1 = (j0 + 1 > g.ny) ? 1 : j0 + 1;
-    const double xdiff = px - (double)(i0 - 1) * g.dx;
-    const double ydiff = py - (double)(j0 - 1) * g.dy;
+    TriStencil t;
+    t.xdiff = px - (double)(i0 - 1) * g.dx;
+    t.ydiff = py - (double)(j0 - 1) * g.dy;
     const double z1 = (1 - 0.5) * g.dz;
     const double zw2 = ((2 - 0.5) * g.dz) - g.dz / 2.0;
     int k0, k1;
-    double zdiff;
     if (pz < zw2) {
         if (pz < z1) {
             k0 = 1;
             k1 = 1;
-            zdiff = 0.0;
+            t.zdiff = 0.0;
         } else {
             k0 = 1;
             k1 = 2;
-            zdiff = 2.0 * (pz - z1);
+            t.zdiff = 2.0 * (pz - z1);
         }
     } else {
         const double zw1 = ((1 - 0.5) * g.dz) - g.dz / 2.0;
@@ -58,22 +66,35 @@ __device__ double trilinear_w(const T* var, const InterpGeo& g, double px, doubl
         else if (!(pz == pz)) k0 = 1;
         else k0 = (int)fmin(fmax(floor((pz - zw1) / g.dz) + 1.0, 0.0), (double)g.nz);
         k1 = k0 + 1;
-        zdiff = pz - (((k0 - 0.5) * g.dz) - g.dz / 2.0);
+        t.zdiff = pz - (((k0 - 0.5) * g.dz) - g.dz / 2.0);
     }
     // keep the gather inside the array (the departure point is assumed to
     // stay within one cell, interpolag_Sdep.f90:31-34)
     k0 = min(max(k0, 0), g.nz);
     k1 = min(max(k1, 0), g.nz);
-    auto V = [&](int kk, int jj, int ii) -> double {
-        return (double)var[(long)kk * g.plane + (long)(jj - 1) * g.ld + (ii - 1)];
-    };
-    const double u1 = V(k0, j0, i0) + xdiff * (V(k0, j0, i1) - V(k0, j0, i0)) / g.dx;
-    const double u2 = V(k0, j1, i0) + xdiff * (V(k0, j1, i1) - V(k0, j1, i0)) / g.dx;
-    const double u3 = V(k1, j0, i0) + xdiff * (V(k1, j0, i1) - V(k1, j0, i0)) / g.dx;
-    const double u4 = V(k1, j1, i0) + xdiff * (V(k1, j1, i1) - V(k1, j1, i0)) / g.dx;
-    const double u5 = u1 + ydiff * (u2 - u1) / g.dy;
-    const double u6 = u3 + ydiff * (u4 - u3) / g.dy;
-    return u5 + zdiff * (u6 - u5) / g.dz;
+    const long base = (long)k0 * g.plane;
+    t.o00 = base + (long)(j0 - 1) * g.ld + (i0 - 1);
+    t.o01 = base + (long)(j0 - 1) * g.ld + (i1 - 1);
+    t.o10 = base + (long)(j1 - 1) * g.ld + (i0 - 1);
+    t.o11 = base + (long)(j1 - 1) * g.ld + (i1 - 1);
+    t.dk = (long)(k1 - k0) * g.plane;
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ double tri_apply(const T* var, const TriStencil& t,
+                                            const InterpGeo& g) {
+    const double a00 = (double)var[t.o00], a01 = (double)var[t.o01];
+    const double a10 = (double)var[t.o10], a11 = (double)var[t.o11];
+    const double b00 = (double)var[t.o00 + t.dk], b01 = (double)var[t.o01 + t.dk];
+    const double b10 = (double)var[t.o10 + t.dk], b11 = (double)var[t.o11 + t.dk];
+    const double u1 = a00 + t.xdiff * (a01 - a00) / g.dx;
+    const double u2 = a10 + t.xdiff * (a11 - a10) / g.dx;
+    const double u3 = b00 + t.xdiff * (b01 - b00) / g.dx;
+    const double u4 = b10 + t.xdiff * (b11 - b10) / g.dx;
+    const double u5 = u1 + t.ydiff * (u2 - u1) / g.dy;
+    const double u6 = u3 + t.ydiff * (u4 - u3) / g.dy;
+    return u5 + t.zdiff * (u6 - u5) / g.dz;
 }
 
 // interpolag_Sdep (interpolag_Sdep.f90:73-241): departure points and the
@@ -118,10 +139,11 @@ __global__ void k_interpolag(const T* u, const T* v, const T* w, const T* tLM, c
         py = y - 0.5 * ((double)v[o - P] + (double)v[o]) * g.ldt;
         pz = (((k - 0.5) * g.dz) - g.dz / 2.0) - (double)w[o] * g.ldt;
     }
-    LM[o] = (T)trilinear_w(tLM, g, px, py, pz);
-    MM[o] = (T)trilinear_w(tMM, g, px, py, pz);
-    QN[o] = (T)trilinear_w(tQN, g, px, py, pz);
-    NN[o] = (T)trilinear_w(tNN, g, px, py, pz);
+    const TriStencil t = tri_stencil(g, px, py, pz);
+    LM[o] = (T)tri_apply(tLM, t, g);
+    MM[o] = (T)tri_apply(tMM, t, g);
+    QN[o] = (T)tri_apply(tQN, t, g);
+    NN[o] = (T)tri_apply(tNN, t, g);
 }
 
 // Quantities to be test filtered for levels k = k0..k0+nc-1

@@ -9,10 +9,10 @@ template PassChoice choice_for<float>(int, int, int);
 template void launch_rows_r2c<float>(KernelTimer&, const FftPlan&, int, const float* const*,
                                    const float*, long, int, int, long, cplx<float>* const*, long,
                                    int, long, int, int, int, int, hipStream_t,
-                                   const PassChoice*, const double*, int);
+                                   const PassChoice*, const double*, int, int);
 template void launch_rows_c2r<float>(KernelTimer&, const FftPlan&, int, const cplx<float>* const*,
                                    long, int, long, float* const*, long, int, int, long, int, int,
-                                   int, int, hipStream_t, const PassChoice*);
+                                   int, int, hipStream_t, const PassChoice*, const int*);
 template void launch_cols<float, -1>(KernelTimer&, ColArgs<float>&, int, int, int, int, hipStream_t,
                                    const PassChoice*);
 template void launch_cols<float, +1>(KernelTimer&, ColArgs<float>&, int, int, int, int, hipStream_t,

@@ -9,10 +9,10 @@ template PassChoice choice_for<double>(int, int, int);
 template void launch_rows_r2c<double>(KernelTimer&, const FftPlan&, int, const double* const*,
                                    const double*, long, int, int, long, cplx<double>* const*, long,
                                    int, long, int, int, int, int, hipStream_t,
-                                   const PassChoice*, const double*, int);
+                                   const PassChoice*, const double*, int, int);
 template void launch_rows_c2r<double>(KernelTimer&, const FftPlan&, int, const cplx<double>* const*,
                                    long, int, long, double* const*, long, int, int, long, int, int,
-                                   int, int, hipStream_t, const PassChoice*);
+                                   int, int, hipStream_t, const PassChoice*, const int*);
 template void launch_cols<double, -1>(KernelTimer&, ColArgs<double>&, int, int, int, int, hipStream_t,
                                    const PassChoice*);
 template void launch_cols<double, +1>(KernelTimer&, ColArgs<double>&, int, int, int, int, hipStream_t,

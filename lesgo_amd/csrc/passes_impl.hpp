@@ -161,7 +161,7 @@ void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* con
                      const T* scales, long src_plane, int src_ld, int src_lev0, long src_ms,
                      cplx<T>* const* dst, long dst_plane, int dst_lev0, long dst_ms, int nrows,
                      int ncols, int nlev, int M, hipStream_t s, const PassChoice* force,
-                     const double* mscale, int mscale_stride) {
+                     const double* mscale, int mscale_stride, int nwr) {
     const PassChoice ch = force ? *force : choice_for<T>(PK_R2C, pl.N, nrows);
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
@@ -184,12 +184,13 @@ void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* con
         a.dst_mstride = dst_ms;
         a.mscale = mscale;
         a.mscale_stride = mscale_stride;
+        a.nwr = (nwr > 0 && nwr < ncols) ? nwr : 0;
         a.S = ch.S;
         a.stride = seq_stride(ch, pl.N);
         const size_t lds = row_lds_bytes<T>(ch, pl.N, src_ld, ncols);
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
         const double bytes = (double)nj * nlev * M * nrows *
-                             ((double)pl.N * sizeof(T) + (double)ncols * sizeof(cplx<T>));
+                             ((double)pl.N * sizeof(T) + (double)(a.nwr ? a.nwr : ncols) * sizeof(cplx<T>));
         tm.run(s, KK_ROWS_R2C, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (!(ch.n1 && v2_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)))
@@ -202,15 +203,19 @@ template <typename T>
 void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T>* const* src,
                      long src_plane, int src_lev0, long src_ms, T* const* dst, long dst_plane,
                      int dst_ld, int dst_lev0, long dst_ms, int nrows, int ncols, int nlev, int M,
-                     hipStream_t s, const PassChoice* force) {
+                     hipStream_t s, const PassChoice* force, const int* nce) {
     const PassChoice ch = force ? *force : choice_for<T>(PK_C2R, pl.N, nrows);
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
         RowC2RArgs<T> a{};
         a.plan = pl;
+        double cin = 0.0;      // complex columns read, summed over the jobs
         for (int i = 0; i < nj; ++i) {
             a.src[i] = src[j0 + i];
             a.dst[i] = dst[j0 + i];
+            const int e = nce ? nce[j0 + i] : 0;
+            a.nce[i] = (e > 0 && e < ncols) ? e : 0;
+            cin += a.nce[i] ? a.nce[i] : ncols;
         }
         a.src_plane = src_plane;
         a.dst_plane = dst_plane;
@@ -226,8 +231,8 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
         a.stride = seq_stride(ch, pl.N);
         const size_t lds = row_lds_bytes<T>(ch, pl.N, dst_ld, ncols);
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
-        const double bytes = (double)nj * nlev * M * nrows *
-                             ((double)ncols * sizeof(cplx<T>) + (double)dst_ld * sizeof(T));
+        const double bytes = (double)nlev * M * nrows *
+                             (cin * sizeof(cplx<T>) + (double)nj * dst_ld * sizeof(T));
         tm.run(s, KK_ROWS_C2R, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (!(ch.n1 && v2_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)))
@@ -247,11 +252,21 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
     c.stride = seq_stride(ch, N);
     c.nlev = nlev;
     const size_t lds = (size_t)c.TC * c.stride * sizeof(cplx<T>);
-    dim3 grid(nj, (c.ncols + c.TC - 1) / c.TC, nlev * M);
-    int n2out = 0;
-    for (int i = 0; i < nj; ++i) n2out += c.dst2[i] ? 1 : 0;
-    const double bytes =
-        (double)nlev * M * c.ncols * sizeof(cplx<T>) * ((double)nj * (ny_in + N) + n2out * N);
+    // band-limited jobs: columns < nzc, and (MAP_ID) rows |ky| < ykeep read by
+    // an inverse pass or written by a forward one
+    int cmax = 0;
+    double cols_rw = 0.0;      // sum over jobs of columns x rows moved
+    for (int i = 0; i < nj; ++i) {
+        const int nzc = (c.nzc[i] > 0 && c.nzc[i] < c.ncols) ? c.nzc[i] : c.ncols;
+        c.nzc[i] = nzc < c.ncols ? nzc : 0;
+        cmax = std::max(cmax, nzc);
+        const int band = c.ykeep[i] > 0 ? std::min(2 * c.ykeep[i] - 1, ny_in) : 0;
+        const double rin = (DIR > 0 && band) ? band : ny_in;
+        const double rout = (DIR < 0 && band) ? band : N;
+        cols_rw += (double)nzc * (rin + rout + (c.dst2[i] ? N : 0));
+    }
+    dim3 grid(nj, (cmax + c.TC - 1) / c.TC, nlev * M);
+    const double bytes = (double)nlev * M * sizeof(cplx<T>) * cols_rw;
     tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
         if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
             LG_LAUNCH((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
