@@ -21,10 +21,20 @@ struct TriGeo {
 // 1..nz (all already scaled by 1/(nx ny tadv1 dt)); rb, rt the bottom/top
 // Neumann planes scaled by 1/(nx ny).  Writes p_hat for rows 0..nz and
 // uses gam and usc (FP64, (nz+1) * ny * NK each) as the Thomas scratch.
+// One column per thread leaves few waves per SIMD (ny * NK / 64 in all), so
+// each sweep issues the loads of a block of TRI_BLK levels together and then
+// runs the recurrence through them: one memory round trip per block, not
+// per level.  The arithmetic is the plain per-level Thomas recurrence.
 template <typename T>
-__global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz,
+struct TriBlk {
+    static constexpr int n = sizeof(T) == 4 ? 16 : 8;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz,
                          const cplx<T>* rb, const cplx<T>* rt, cplx<T>* ph, double* gam,
                          double2* usc, TriGeo g, int* err) {
+    constexpr int B = TriBlk<T>::n;
     const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (gidx >= g.M * g.ny * g.NK) return;
     const int mi = gidx / (g.ny * g.NK);
@@ -47,10 +57,7 @@ __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz
         return make_double2((double)v.x, (double)v.y);
     };
     const double hs = g.hsc;
-    auto ldh = [&](const cplx<T>* p, long o) -> double2 {
-        const cplx<T> v = p[o];
-        return make_double2((double)v.x * hs, (double)v.y * hs);
-    };
+    auto sc = [&](cplx<T> v) -> double2 { return make_double2((double)v.x * hs, (double)v.y * hs); };
     auto st = [&](long o, double2 v) { ph[o] = cplx<T>{(T)v.x, (T)v.y}; };
     if (jy == g.ny / 2) {
         for (int m = 0; m < n; ++m) st((long)m * S + idx, make_double2(0.0, 0.0));
@@ -63,10 +70,19 @@ __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz
         const double2 b = ld(rb, idx);
         double2 pk = make_double2(p0.x - g.dz * b.x, p0.y - g.dz * b.y);
         st(S + idx, pk);
-        for (int m = 2; m <= g.nz; ++m) {
-            const double2 h = ldh(Hz, (long)m * S + idx);
-            pk = make_double2(pk.x + h.x * g.dz, pk.y + h.y * g.dz);
-            st((long)m * S + idx, pk);
+        for (int m0 = 2; m0 <= g.nz; m0 += B) {
+            cplx<T> hz[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) hz[i] = Hz[(long)min(m0 + i, g.nz) * S + idx];
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                const int m = m0 + i;
+                if (m <= g.nz) {
+                    const double2 h = sc(hz[i]);
+                    pk = make_double2(pk.x + h.x * g.dz, pk.y + h.y * g.dz);
+                    st((long)m * S + idx, pk);
+                }
+            }
         }
         return;
     }
@@ -81,45 +97,76 @@ __global__ void k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz
     usc[idx] = u;
     double bet = -1.0;
     double cprev = 1.0;
-    for (int m = 1; m <= g.nz; ++m) {
-        double a, b, cc;
-        double2 r;
-        if (m < g.nz) {
-            a = a_in;
-            b = b_in;
-            cc = c_in;
-            const double2 hx = ldh(Hx, (long)m * S + idx);
-            const double2 hy = ldh(Hy, (long)m * S + idx);
-            const double2 z1 = ldh(Hz, (long)(m + 1) * S + idx);
-            const double2 z0 = ldh(Hz, (long)m * S + idx);
-            // i kx Hx + i ky Hy + (Hz(m+1) - Hz(m)) / dz
-            const double ahx_r = -hx.y * kx, ahx_i = hx.x * kx;
-            const double ahy_r = -hy.y * ky, ahy_i = hy.x * ky;
-            r.x = (ahx_r + ahy_r) + (z1.x - z0.x) * g.const4;
-            r.y = (ahx_i + ahy_i) + (z1.y - z0.y) * g.const4;
-        } else {
-            a = -1.0;
-            b = 1.0;
-            cc = 0.0;
-            const double2 t = ld(rt, idx);
-            r = make_double2(-g.dz * t.x, -g.dz * t.y);
+    // Hz(m) of the level being solved (Hz(1) before the first)
+    double2 z0 = sc(Hz[S + idx]);
+    for (int m0 = 1; m0 <= g.nz; m0 += B) {
+        // rows m < nz read Hx(m), Hy(m), Hz(m + 1); clamped indices keep the
+        // block's loads in bounds past its end
+        cplx<T> hx[B], hy[B], hz[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const long o = (long)min(m0 + i, g.nz - 1) * S + idx;
+            hx[i] = Hx[o];
+            hy[i] = Hy[o];
+            hz[i] = Hz[o + S];
         }
-        const double gm = cprev / bet;
-        gam[(long)m * S + idx] = gm;
-        bet = b - a * gm;
-        if (bet == 0.0) atomicExch(err, 1);
-        u = make_double2((r.x - a * u.x) / bet, (r.y - a * u.y) / bet);
-        usc[(long)m * S + idx] = u;
-        cprev = cc;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int m = m0 + i;
+            if (m > g.nz) continue;
+            double a, b, cc;
+            double2 r;
+            if (m < g.nz) {
+                a = a_in;
+                b = b_in;
+                cc = c_in;
+                const double2 vx = sc(hx[i]);
+                const double2 vy = sc(hy[i]);
+                const double2 z1 = sc(hz[i]);
+                // i kx Hx + i ky Hy + (Hz(m+1) - Hz(m)) / dz
+                const double ahx_r = -vx.y * kx, ahx_i = vx.x * kx;
+                const double ahy_r = -vy.y * ky, ahy_i = vy.x * ky;
+                r.x = (ahx_r + ahy_r) + (z1.x - z0.x) * g.const4;
+                r.y = (ahx_i + ahy_i) + (z1.y - z0.y) * g.const4;
+                z0 = z1;
+            } else {
+                a = -1.0;
+                b = 1.0;
+                cc = 0.0;
+                const double2 t = ld(rt, idx);
+                r = make_double2(-g.dz * t.x, -g.dz * t.y);
+            }
+            const double gm = cprev / bet;
+            gam[(long)m * S + idx] = gm;
+            bet = b - a * gm;
+            if (bet == 0.0) atomicExch(err, 1);
+            u = make_double2((r.x - a * u.x) / bet, (r.y - a * u.y) / bet);
+            usc[(long)m * S + idx] = u;
+            cprev = cc;
+        }
     }
     st((long)g.nz * S + idx, u);
     double2 up = u;
-    for (int m = g.nz - 1; m >= 0; --m) {
-        const double gm = gam[(long)(m + 1) * S + idx];
-        double2 um = usc[(long)m * S + idx];
-        um = make_double2(um.x - gm * up.x, um.y - gm * up.y);
-        st((long)m * S + idx, um);
-        up = um;
+    // back substitution, levels nz-1 .. 0 in blocks
+    for (int m0 = g.nz - 1; m0 >= 0; m0 -= B) {
+        double gb[B];
+        double2 ub[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int m = max(m0 - i, 0);
+            gb[i] = gam[(long)(m + 1) * S + idx];
+            ub[i] = usc[(long)m * S + idx];
+        }
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int m = m0 - i;
+            if (m < 0) continue;
+            const double gm = gb[i];
+            double2 um = ub[i];
+            um = make_double2(um.x - gm * up.x, um.y - gm * up.y);
+            st((long)m * S + idx, um);
+            up = um;
+        }
     }
 }
 
