@@ -175,6 +175,8 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     // environment keeps every case on the all-physical step
     const char* phys = std::getenv("LESGO_AMD_PHYSICAL_STEP");
     spec_on = !cfl_mode && !(phys && phys[0] == '1');
+    const char* lr = std::getenv("LESGO_AMD_LAG_ROWS");
+    lag_fused = !(lr && lr[0] == '0') && lag_rows_ok<T>(d.nx, d.ny);
     if (spec_on) {
         for (int q = 0; q < 3; ++q) {
             sR[q] = dalloc<cplx<T>>(M * msp);
@@ -875,22 +877,72 @@ void Case<T>::lagrange_sdep() {
         for (int q = 0; q < 6; ++q) ssrc[q] = fld[sids[q]];
         fwd2d(6, ssrc, nullptr, d.plane, k0, d.field, dst + 9, d.splane, 0, ms, nc, nullptr,
               g_nzc[0], g_ykeep[0]);
-        const cplx<T>* isrc[42];
-        int ops[42];
-        T* idst[42];
-        for (int q = 0; q < 21; ++q) {
-            isrc[q] = dst[q];
-            ops[q] = OP_G1;
-            idst[q] = lag_filt + q * Qp;
-            isrc[21 + q] = dst[q];
-            ops[21 + q] = OP_G2;
-            idst[21 + q] = lag_filt + (21 + q) * Qp;
+        double* contr = nullptr;
+        if (lag_fused) {
+            // per filter: the 21 quantities' column pass into lag_filt
+            // (quantity q at q * Qs complex), then k_lag_rows forms L_ij M_ij
+            // and M_ij M_ij (Q_ij N_ij and N_ij N_ij) in four FP64 fields Qp
+            // apart after them
+            cplx<T>* colb = reinterpret_cast<cplx<T>*>(lag_filt);
+            contr = reinterpret_cast<double*>(lag_filt + 21 * Qp);
+            for (int f = 0; f < 2; ++f) {
+                for (int q0 = 0; q0 < 21; q0 += MAX_JOBS) {
+                    const int nj = std::min(MAX_JOBS, 21 - q0);
+                    ColArgs<T> cc{};
+                    cc.plan = py.plan;
+                    for (int i = 0; i < nj; ++i) {
+                        cc.src[i] = dst[q0 + i];
+                        cc.dst[i] = colb + (q0 + i) * Qs;
+                        cc.op[i] = f ? OP_G2 : OP_G1;
+                        cc.nzc[i] = g_nzc[f];
+                        cc.ykeep[i] = g_ykeep[f];
+                    }
+                    cc.src_plane = cc.dst_plane = d.splane;
+                    cc.lev0_src = cc.lev0_dst = 0;
+                    cc.src_mstride = cc.dst_mstride = ms;
+                    cc.map = MAP_ID;
+                    cc.zero_nyq = 0;
+                    col_common(cc, d, G1, G2);
+                    launch_cols<T, +1>(timer, cc, nj, nc, Ma, d.ny, stream, nullptr);
+                }
+                LagRowArgs<T> la{};
+                la.plan = px.plan;
+                la.src = colb;
+                la.qstride = Qs;
+                la.plane = d.splane;
+                la.mstride = ms;
+                la.nrows = d.ny;
+                la.ncols = d.NK;
+                la.nce = g_nzc[f];
+                la.nlev = nc;
+                la.out0 = contr + (2 * f) * Qp;
+                la.out1 = contr + (2 * f + 1) * Qp;
+                la.oplane = d.plane;
+                la.omstride = mp;
+                la.ld = d.ld;
+                la.cn = up.cnst;
+                la.tf2 = f ? up.tf2_2 : up.tf1_2;
+                if (!launch_lag_rows<T>(timer, la, Ma, stream))
+                    throw std::runtime_error("k_lag_rows not launchable for this grid");
+            }
+        } else {
+            const cplx<T>* isrc[42];
+            int ops[42];
+            T* idst[42];
+            for (int q = 0; q < 21; ++q) {
+                isrc[q] = dst[q];
+                ops[q] = OP_G1;
+                idst[q] = lag_filt + q * Qp;
+                isrc[21 + q] = dst[q];
+                ops[21 + q] = OP_G2;
+                idst[21 + q] = lag_filt + (21 + q) * Qp;
+            }
+            inv2d(42, isrc, ops, d.splane, 0, ms, idst, d.plane, 0, mp, nc, spec);
         }
-        inv2d(42, isrc, ops, d.splane, 0, ms, idst, d.plane, 0, mp, nc, spec);
         hipLaunchKernelGGL(k_lag_update<T>, grid_ij(d.ld, d.ny, nc, Ma), dim3(BLK), 0, stream,
                            (const T*)lag_filt, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN],
                            fld[LG_F_NN], fld[LG_CS_OPT2], fld[LG_BETA], fld[LG_TN_ALL], g, k0, nc,
-                           Ma, up, (const int*)(d_flags + 2 * M));
+                           Ma, up, (const int*)(d_flags + 2 * M), (const double*)contr);
     }
     if (cfl_mode)
         hipLaunchKernelGGL(k_clock_lag_reset, dim3(1), dim3(64), 0, stream, d_clock,

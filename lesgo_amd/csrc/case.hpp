@@ -110,6 +110,10 @@ struct Case {
     // describe it); stale: the physical groups not yet made.
     enum : unsigned { SG_UVW = 1, SG_RHS = 2, SG_RHSF = 4, SG_P = 8, SG_DIVT = 16, SG_ALL = 31 };
     bool spec_on = false;
+    // the Lagrangian model's inverse row pass forms the tensor contractions
+    // itself (k_lag_rows) where the grid allows it; LESGO_AMD_LAG_ROWS=0
+    // keeps the 42 filtered fields
+    bool lag_fused = true;
     std::vector<char> mcold;
     bool rhsf_input = false;         // the physical RHS_f was valid when the last step began
     unsigned stale = 0;

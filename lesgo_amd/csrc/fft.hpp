@@ -172,6 +172,26 @@ __device__ void fft_stages(cplx<T>* lds, int S, int stride, const FftPlan& pl) {
 //   A[k] = (Z[k] + conj Z[N-k]) / 2,  B[k] = (Z[k] - conj Z[N-k]) / (2i).
 // Output: complex rows of width `ncols` (kx < ncols <= N/2).
 // --------------------------------------------------------------------------
+// The Lagrangian model's inverse row pass with its contractions
+// (lag_rows.hpp)
+template <typename T>
+struct LagRowArgs {
+    FftPlan plan;              // length nx
+    const cplx<T>* src;        // column-pass output of quantity q at src + q * qstride:
+                               // [member][level][ky][kx < ncols]
+    long qstride;              // complex elements between quantities
+    long plane;                // complex elements per level (ny * ncols)
+    long mstride;              // complex elements per member
+    int nrows, ncols, nce;     // ny, nx / 2, columns inside the filter's band
+    int nlev;                  // levels; grid.y = nlev * members
+    double* out0;              // L_ij M_ij (or Q_ij N_ij): [member][level][j][ld]
+    double* out1;              // M_ij M_ij (or N_ij N_ij)
+    long oplane, omstride;     // elements per level / member of out0, out1
+    int ld;
+    double cn;                 // 2 delta^2
+    double tf2;                // (filter width / delta)^2
+};
+
 template <typename T>
 struct RowR2CArgs {
     FftPlan plan;

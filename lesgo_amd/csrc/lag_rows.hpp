@@ -1,0 +1,119 @@
+// The Lagrangian model's inverse row pass with its tensor contractions
+// (lagrange_Sdep.f90:137-167, 203-259): for one row pair (j, j+1) of one
+// level, the 21 quantities filtered at one width (the column pass's output
+// of ubar, vbar, wbar, the six products, S_ij and |S| S_ij) are transformed
+// back along x in LDS and contracted at once into L_ij M_ij and M_ij M_ij
+// (2 delta) or Q_ij N_ij and N_ij N_ij (4 delta), stored in FP64.  The 21
+// filtered fields never reach HBM.
+//
+// The transforms are the paired c2r of k_rows_c2r_v2 (same four-step
+// arithmetic); the contractions are k_lag_update's.  LDS: 21 slots of
+// Split<N1, N2>::YS complex elements; a slot holds a row pair's
+// intermediate, then its physical values (row j in .x, row j + 1 in .y).
+#pragma once
+
+#include "fft2.hpp"
+#include "pointwise.hpp"
+
+namespace lg {
+
+constexpr int LAGROW_BLK = 384;
+constexpr int LAGROW_Q = 21;
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(LAGROW_BLK) k_lag_rows(LagRowArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    constexpr int half = N / 2;
+    constexpr int YS = SP::YS;
+    static_assert(LAGROW_Q * N1 <= LAGROW_BLK, "step 2 in one round");
+    extern __shared__ __align__(16) unsigned char smem_lr[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_lr);
+    const auto* tw = plan_tw<T>(a.plan);
+    const int mem = blockIdx.y / a.nlev;
+    const int c = blockIdx.y - mem * a.nlev;
+    const int j = 2 * blockIdx.x;
+    const int nc = a.ncols, nce = a.nce;
+    const int tid = threadIdx.x;
+    const cplx<T>* base = a.src + (long)mem * a.mstride + (long)c * a.plane + (long)j * nc;
+    // step 1: N1-point DFTs of the Hermitian-extended packed rows
+    for (int t = tid; t < LAGROW_Q * N2; t += blockDim.x) {
+        const int q = t / N2, n2 = t - q * N2;
+        const cplx<T>* ra = base + (long)q * a.qstride;
+        const cplx<T>* rb = ra + nc;
+        cplx<T> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int k = n2 + N2 * n1;
+            cplx<T> z{(T)0, (T)0};
+            if (k == 0) {
+                z = cplx<T>{gld(ra).x, gld(rb).x};
+            } else if (k < half) {
+                if (k < nce) {
+                    const cplx<T> A = gld(ra + k), B = gld(rb + k);
+                    z = cplx<T>{A.x - B.y, A.y + B.x};
+                }
+            } else if (k > half) {
+                const int kk = N - k;
+                if (kk < nce) {
+                    const cplx<T> A = gld(ra + kk), B = gld(rb + kk);
+                    z = cplx<T>{A.x + B.y, B.x - A.y};
+                }
+            }
+            x[n1] = z;
+        }
+        dft_reg<T, N1, +1>(x);
+        cplx<T>* Y = lds + q * YS;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+            Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, +1>(tw, n2 * k1));
+    }
+    __syncthreads();
+    // step 2 (one round): N2-point DFTs; every LDS read is done before the
+    // barrier, then the physical values overwrite the slot in natural order
+    {
+        const int q = tid / N1, k1 = tid - q * N1;
+        const bool act = q < LAGROW_Q;
+        cplx<T> y[N2];
+        if (act) {
+            const cplx<T>* Y = lds + q * YS + k1 * SP::YR;
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+            dft_reg<T, N2, +1>(y);
+        }
+        __syncthreads();
+        if (act) {
+            cplx<T>* X = lds + q * YS;
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) X[k1 + N1 * k2] = y[k2];
+        }
+        __syncthreads();
+    }
+    // the contractions at each point of the two rows (k_lag_update's
+    // formulas, FP64 from the filtered values)
+    const double cn = a.cn, tf2 = a.tf2;
+    for (int t = tid; t < 2 * N; t += blockDim.x) {
+        const int r = t / N, n = t - r * N;
+        auto V = [&](int q) -> double {
+            const cplx<T> v = lds[q * YS + n];
+            return (double)(r ? v.y : v.x);
+        };
+        const double ub = V(0), vb = V(1), wb = V(2);
+        const double L11 = V(3) - ub * ub, L12 = V(4) - ub * vb, L13 = V(5) - ub * wb;
+        const double L22 = V(6) - vb * vb, L23 = V(7) - vb * wb, L33 = V(8) - wb * wb;
+        const double Sn = smag_norm(V(9), V(10), V(11), V(12), V(13), V(14));
+        const double M11 = cn * (V(15) - tf2 * Sn * V(9));
+        const double M12 = cn * (V(16) - tf2 * Sn * V(10));
+        const double M13 = cn * (V(17) - tf2 * Sn * V(11));
+        const double M22 = cn * (V(18) - tf2 * Sn * V(12));
+        const double M23 = cn * (V(19) - tf2 * Sn * V(13));
+        const double M33 = cn * (V(20) - tf2 * Sn * V(14));
+        const double LM = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
+        const double MM = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
+        const long o = (long)mem * a.omstride + (long)c * a.oplane + (long)(j + r) * a.ld + n;
+        a.out0[o] = LM;
+        a.out1[o] = MM;
+    }
+}
+
+}  // namespace lg

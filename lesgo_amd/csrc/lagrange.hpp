@@ -209,7 +209,7 @@ struct LagUpd {
 template <typename T>
 __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, T* Cs,
                              T* Beta, T* Tn_all, Geo g, int k0, int nc, int M, LagUpd p,
-                             const int* mode) {
+                             const int* mode, const double* contr) {
     int j, i;
     if (!ij_of(g.ld, g.ny, j, i)) return;
     const int m = blockIdx.y / nc;
@@ -227,37 +227,48 @@ __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, 
     }
     const long Q = (long)M * nc * P;
     const long oc = ((long)m * nc + c) * P + (long)j * g.ld + i;
-    auto B = [&](int q) -> double { return (double)filt[oc + (long)q * Q]; };
-    auto H = [&](int q) -> double { return (double)filt[oc + (long)(21 + q) * Q]; };
-    // L_ij and Q_ij (lagrange_Sdep.f90:137-167)
-    const double ub = B(0), vb = B(1), wb = B(2);
-    const double uh = H(0), vh = H(1), wh = H(2);
-    const double L11 = B(3) - ub * ub, L12 = B(4) - ub * vb, L13 = B(5) - ub * wb;
-    const double L22 = B(6) - vb * vb, L23 = B(7) - vb * wb, L33 = B(8) - wb * wb;
-    const double Q11 = H(3) - uh * uh, Q12 = H(4) - uh * vh, Q13 = H(5) - uh * wh;
-    const double Q22 = H(6) - vh * vh, Q23 = H(7) - vh * wh, Q33 = H(8) - wh * wh;
-    // |S_bar|, |S_hat| (lagrange_Sdep.f90:203-209)
-    const double S_bar = smag_norm(B(9), B(10), B(11), B(12), B(13), B(14));
-    const double S_hat = smag_norm(H(9), H(10), H(11), H(12), H(13), H(14));
-    const double cn = p.cnst;
-    // M_ij, N_ij (lagrange_Sdep.f90:241-253)
-    const double M11 = cn * (B(15) - p.tf1_2 * S_bar * B(9));
-    const double M12 = cn * (B(16) - p.tf1_2 * S_bar * B(10));
-    const double M13 = cn * (B(17) - p.tf1_2 * S_bar * B(11));
-    const double M22 = cn * (B(18) - p.tf1_2 * S_bar * B(12));
-    const double M23 = cn * (B(19) - p.tf1_2 * S_bar * B(13));
-    const double M33 = cn * (B(20) - p.tf1_2 * S_bar * B(14));
-    const double N11 = cn * (H(15) - p.tf2_2 * S_hat * H(9));
-    const double N12 = cn * (H(16) - p.tf2_2 * S_hat * H(10));
-    const double N13 = cn * (H(17) - p.tf2_2 * S_hat * H(11));
-    const double N22 = cn * (H(18) - p.tf2_2 * S_hat * H(12));
-    const double N23 = cn * (H(19) - p.tf2_2 * S_hat * H(13));
-    const double N33 = cn * (H(20) - p.tf2_2 * S_hat * H(14));
-    // contractions (lagrange_Sdep.f90:256-259)
-    const double LM = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
-    const double MM = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
-    const double QN = Q11 * N11 + Q22 * N22 + Q33 * N33 + 2.0 * (Q12 * N12 + Q13 * N13 + Q23 * N23);
-    const double NN = N11 * N11 + N22 * N22 + N33 * N33 + 2.0 * (N12 * N12 + N13 * N13 + N23 * N23);
+    // L_ij M_ij, M_ij M_ij, Q_ij N_ij, N_ij N_ij: from the row pass that
+    // formed them (contr: four FP64 fields Q apart) or from the 42 filtered
+    // fields
+    double LM, MM, QN, NN;
+    if (contr) {
+        LM = contr[oc];
+        MM = contr[oc + Q];
+        QN = contr[oc + 2 * Q];
+        NN = contr[oc + 3 * Q];
+    } else {
+        auto B = [&](int q) -> double { return (double)filt[oc + (long)q * Q]; };
+        auto H = [&](int q) -> double { return (double)filt[oc + (long)(21 + q) * Q]; };
+        // L_ij and Q_ij (lagrange_Sdep.f90:137-167)
+        const double ub = B(0), vb = B(1), wb = B(2);
+        const double uh = H(0), vh = H(1), wh = H(2);
+        const double L11 = B(3) - ub * ub, L12 = B(4) - ub * vb, L13 = B(5) - ub * wb;
+        const double L22 = B(6) - vb * vb, L23 = B(7) - vb * wb, L33 = B(8) - wb * wb;
+        const double Q11 = H(3) - uh * uh, Q12 = H(4) - uh * vh, Q13 = H(5) - uh * wh;
+        const double Q22 = H(6) - vh * vh, Q23 = H(7) - vh * wh, Q33 = H(8) - wh * wh;
+        // |S_bar|, |S_hat| (lagrange_Sdep.f90:203-209)
+        const double S_bar = smag_norm(B(9), B(10), B(11), B(12), B(13), B(14));
+        const double S_hat = smag_norm(H(9), H(10), H(11), H(12), H(13), H(14));
+        const double cn = p.cnst;
+        // M_ij, N_ij (lagrange_Sdep.f90:241-253)
+        const double M11 = cn * (B(15) - p.tf1_2 * S_bar * B(9));
+        const double M12 = cn * (B(16) - p.tf1_2 * S_bar * B(10));
+        const double M13 = cn * (B(17) - p.tf1_2 * S_bar * B(11));
+        const double M22 = cn * (B(18) - p.tf1_2 * S_bar * B(12));
+        const double M23 = cn * (B(19) - p.tf1_2 * S_bar * B(13));
+        const double M33 = cn * (B(20) - p.tf1_2 * S_bar * B(14));
+        const double N11 = cn * (H(15) - p.tf2_2 * S_hat * H(9));
+        const double N12 = cn * (H(16) - p.tf2_2 * S_hat * H(10));
+        const double N13 = cn * (H(17) - p.tf2_2 * S_hat * H(11));
+        const double N22 = cn * (H(18) - p.tf2_2 * S_hat * H(12));
+        const double N23 = cn * (H(19) - p.tf2_2 * S_hat * H(13));
+        const double N33 = cn * (H(20) - p.tf2_2 * S_hat * H(14));
+        // contractions (lagrange_Sdep.f90:256-259)
+        LM = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
+        MM = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
+        QN = Q11 * N11 + Q22 * N22 + Q33 * N33 + 2.0 * (Q12 * N12 + Q13 * N13 + Q23 * N23);
+        NN = N11 * N11 + N22 * N22 + N33 * N33 + 2.0 * (N12 * N12 + N13 * N13 + N23 * N23);
+    }
     const double zero = p.zero;
     // 2-delta running averages (lagrange_Sdep.f90:297-315)
     double fLM = F_LM[o], fMM = F_MM[o];

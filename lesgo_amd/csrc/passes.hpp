@@ -98,5 +98,11 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
 template <typename T, int DIR>
 void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny_in,
                  hipStream_t s, const PassChoice* force);
+// The Lagrangian model's inverse row pass with its tensor contractions
+// (levels a.nlev of M members).  False: not launchable for this nx.
+template <typename T>
+bool launch_lag_rows(KernelTimer& tm, const LagRowArgs<T>& a, int M, hipStream_t s);
+template <typename T>
+bool lag_rows_ok(int N, int nrows);
 
 }  // namespace lg

@@ -17,5 +17,7 @@ template void launch_cols<float, -1>(KernelTimer&, ColArgs<float>&, int, int, in
                                    const PassChoice*);
 template void launch_cols<float, +1>(KernelTimer&, ColArgs<float>&, int, int, int, int, hipStream_t,
                                    const PassChoice*);
+template bool launch_lag_rows<float>(KernelTimer&, const LagRowArgs<float>&, int, hipStream_t);
+template bool lag_rows_ok<float>(int, int);
 
 }  // namespace lg

@@ -19,5 +19,7 @@ template void launch_cols<double, +1>(KernelTimer&, ColArgs<double>&, int, int, 
                                    const PassChoice*);
 
 std::vector<std::pair<int, int>> pass_splits(int N) { return splits_for(N); }
+template bool launch_lag_rows<double>(KernelTimer&, const LagRowArgs<double>&, int, hipStream_t);
+template bool lag_rows_ok<double>(int, int);
 
 }  // namespace lg

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "fft3.hpp"
+#include "lag_rows.hpp"
 #include "passes.hpp"
 
 namespace lg {
@@ -271,6 +272,44 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
         if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
             LG_LAUNCH((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
     });
+}
+
+// The Lagrangian model's row pass with contractions (lag_rows.hpp), with
+// the default split of nx.  Returns false, launching nothing, when nx has
+// no split that fits the kernel's block.
+#define LG_LAGROW_SPLITS(X) \
+    X(4, 4) X(4, 6) X(4, 8) X(6, 8) X(8, 8) X(8, 9) X(8, 12) X(8, 16) X(12, 12) X(12, 16) X(16, 16)
+
+template <typename T>
+bool lag_rows_ok(int N, int nrows) {
+    int n1, n2;
+    if (!split_of(N, n1, n2) || LAGROW_Q * n1 > LAGROW_BLK || (nrows & 1)) return false;
+    bool known = false;
+#define LG_CASE(A, B) known |= (n1 == A && n2 == B);
+    LG_LAGROW_SPLITS(LG_CASE)
+#undef LG_CASE
+    return known && (size_t)LAGROW_Q * (n1 * (n2 + 1) + 1) * sizeof(cplx<T>) <= LDS_MAX;
+}
+
+template <typename T>
+bool launch_lag_rows(KernelTimer& tm, const LagRowArgs<T>& a, int M, hipStream_t s) {
+    const int N = a.plan.N;
+    if (!lag_rows_ok<T>(N, a.nrows)) return false;
+    int n1, n2;
+    split_of(N, n1, n2);
+    const size_t lds = (size_t)LAGROW_Q * (n1 * (n2 + 1) + 1) * sizeof(cplx<T>);
+    const dim3 grid(a.nrows / 2, a.nlev * M);
+    // algorithmic bytes: the 21 band-limited half spectra read, two FP64
+    // fields written
+    const double bytes = (double)M * a.nlev * a.nrows *
+                         (LAGROW_Q * (double)a.nce * sizeof(cplx<T>) + 2.0 * N * sizeof(double));
+    tm.run(s, KK_ROWS_C2R, bytes, [&] {
+#define LG_CASE(A, B)                                                                \
+    if (n1 == A && n2 == B) LG_LAUNCH((k_lag_rows<T, A, B>), grid, dim3(LAGROW_BLK), lds, s, a);
+        LG_LAGROW_SPLITS(LG_CASE)
+#undef LG_CASE
+    });
+    return true;
 }
 
 }  // namespace lg

@@ -1,0 +1,79 @@
+"""The Lagrangian model's fused inverse row pass (k_lag_rows: the 21
+filtered quantities of each width transformed back in LDS and contracted
+into L_ij M_ij, M_ij M_ij, Q_ij N_ij, N_ij N_ij at once) against the 42
+filtered fields of the separate passes (LESGO_AMD_LAG_ROWS=0): the same
+Cs_opt2, F_* and flow, bit for bit, through a fresh start's first dynamic
+call and a later dynamic step, alone and in a batch (the transforms are the
+same four-step arithmetic and the contractions the same FP64 expressions of
+the same filtered values); a grid whose nx has no fused kernel takes the
+separate passes either way."""
+import os
+
+import numpy as np
+import pytest
+
+from gpu_util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "w", "p", "Cs_opt2", "F_LM", "F_MM", "F_QN", "F_NN")
+
+
+def _case(fused, nx, ny, nz, precision, members):
+    import lesgo_amd
+    if not fused:
+        os.environ["LESGO_AMD_LAG_ROWS"] = "0"
+    try:
+        cfg = lesgo_amd.Config(nx, ny, nz, initu=False, dyn_init=6, precision=precision)
+        return lesgo_amd.Case(cfg, members=members)
+    finally:
+        os.environ.pop("LESGO_AMD_LAG_ROWS", None)
+
+
+def _run_pair(nx, ny, nz, precision, tols, members=1):
+    """tols: step -> largest allowed relative L2 gap of every field (0: the
+    same bits)."""
+    cases = [_case(False, nx, ny, nz, precision, members),
+             _case(True, nx, ny, nz, precision, members)]
+    K = np.s_[1:nz]
+    try:
+        for c in cases:
+            for m in range(members):
+                c.seed_state(3 + m, member=m) if members > 1 else c.seed_state(3)
+        for chunk in (10, 5):
+            for c in cases:
+                c.step(chunk)
+            step = cases[0].counters()[1]
+            worst = 0.0
+            for m in range(members):
+                for n in FIELDS:
+                    a = cases[0].download(n, member=m)[K][:, :, :nx]
+                    b = cases[1].download(n, member=m)[K][:, :, :nx]
+                    e = rel_l2(b, a)
+                    worst = max(worst, e)
+                    assert e <= tols[step], (n, m, step, e)
+            print(f"  {nx}x{ny}x{nz} {precision} x{members} step {step}: worst {worst:.2e}",
+                  flush=True)
+    finally:
+        for c in cases:
+            c.close()
+
+
+EXACT = {10: 0.0, 15: 0.0}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_lag_rows_equal_separate_passes(precision):
+    # 48 x 32 x 16: nx = 48 (split 6 x 8); dynamic calls at steps 10 and 15
+    _run_pair(48, 32, 16, precision, EXACT)
+
+
+def test_fused_lag_rows_batch_and_256_rows():
+    # the bench's x length (nx = 256, split 16 x 16), two members (FP64: this
+    # short 8-level domain does not stay stable in mixed precision)
+    _run_pair(256, 32, 8, "fp64", EXACT, members=2)
+
+
+def test_grid_without_fused_lag_rows_takes_separate_passes():
+    # nx = 36 has no four-step split: both cases run the separate passes
+    _run_pair(36, 32, 12, "fp64", EXACT)
