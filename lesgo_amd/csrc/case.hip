@@ -794,7 +794,9 @@ void Case<T>::calc_sij() {
 
 template <typename T>
 void Case<T>::sgs_stag() {
-    calc_sij();
+    // without a dynamic-model update between them, S_ij and the stresses
+    // are formed in one pass (k_sij_tau); Cs_opt2 is set first either way
+    if (lag_now) calc_sij();
     // Cs_opt2 = 0.03 at a fresh member's first step (sgs_stag_util.f90:187-189)
     bool any_cs = false;
     for (int m = 0; m < Ma; ++m) any_cs |= hflags[M + m] != 0;
@@ -807,8 +809,16 @@ void Case<T>::sgs_stag() {
     TauArgs<T> a{fld[LG_S11], fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33],
                  fld[LG_CS_OPT2], fld[LG_NU_T], fld[LG_TXX], fld[LG_TXY], fld[LG_TYY],
                  fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ], (T)(d.delta * d.delta), (T)d.prm.nu};
-    hipLaunchKernelGGL(k_sgs_tau<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
-                       GEO(1, d.nz));
+    if (lag_now) {
+        hipLaunchKernelGGL(k_sgs_tau<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
+                           GEO(1, d.nz));
+    } else {
+        SijArgs<T> sa{fld[LG_DUDX], fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDY],
+                      fld[LG_DVDZ], fld[LG_DWDX], fld[LG_DWDY], fld[LG_DWDZ], fld[LG_S11],
+                      fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33]};
+        hipLaunchKernelGGL(k_sij_tau<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+                           sa, a, GEO(1, d.nz));
+    }
     fill_levels({{LG_TXX, 0, BOGUS}, {LG_TXY, 0, BOGUS}, {LG_TXZ, 0, BOGUS}, {LG_TYY, 0, BOGUS},
                  {LG_TYZ, 0, BOGUS}, {LG_TZZ, 0, BOGUS}, {LG_TXX, d.nz, BOGUS},
                  {LG_TXY, d.nz, BOGUS}, {LG_TYY, d.nz, BOGUS}, {LG_TZZ, d.nz, BOGUS}});

@@ -123,6 +123,48 @@ struct SijArgs {
     T *S11, *S12, *S13, *S22, *S23, *S33;
 };
 
+// S_ij at point o of level k (calc_Sij, sgs_stag_util.f90:436-515 with the
+// w-node interpolations of the one-rank case)
+template <typename T>
+struct Sij6 {
+    T s11, s12, s13, s22, s23, s33;
+};
+
+template <typename T>
+__device__ __forceinline__ Sij6<T> sij_at(const SijArgs<T>& a, long o, long P, int k, int nz) {
+    const T h = (T)0.5;
+    Sij6<T> r;
+    if (k == 1) {
+        r.s11 = a.dudx[o];
+        r.s12 = h * (a.dudy[o] + a.dvdx[o]);
+        const T wx = h * (a.dwdx[o] + a.dwdx[o + P]);
+        r.s13 = h * (a.dudz[o] + wx);
+        r.s22 = a.dvdy[o];
+        const T wy = h * (a.dwdy[o] + a.dwdy[o + P]);
+        r.s23 = h * (a.dvdz[o] + wy);
+        r.s33 = a.dwdz[o];
+    } else if (k == nz) {
+        const long m = o - P;
+        r.s11 = a.dudx[m];
+        r.s12 = h * (a.dudy[m] + a.dvdx[m]);
+        r.s13 = h * (a.dudz[o] + a.dwdx[o]);
+        r.s22 = a.dvdy[m];
+        r.s23 = h * (a.dvdz[o] + a.dwdy[o]);
+        r.s33 = h * (a.dwdz[m] + (T)0);
+    } else {
+        const long m = o - P;
+        r.s11 = h * (a.dudx[o] + a.dudx[m]);
+        const T uy = a.dudy[o] + a.dudy[m];
+        const T vx = a.dvdx[o] + a.dvdx[m];
+        r.s12 = (T)0.25 * (uy + vx);
+        r.s13 = h * (a.dudz[o] + a.dwdx[o]);
+        r.s22 = h * (a.dvdy[o] + a.dvdy[m]);
+        r.s23 = h * (a.dvdz[o] + a.dwdy[o]);
+        r.s33 = h * (a.dwdz[o] + a.dwdz[m]);
+    }
+    return r;
+}
+
 template <typename T>
 __global__ void k_calc_sij(SijArgs<T> a, Geo g) {
     int m, k, j, i;
@@ -130,35 +172,13 @@ __global__ void k_calc_sij(SijArgs<T> a, Geo g) {
     if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
     const long P = g.plane;
     const long o = mb + (long)k * P + (long)j * g.ld + i;
-    const T h = (T)0.5;
-    if (k == 1) {
-        a.S11[o] = a.dudx[o];
-        a.S12[o] = h * (a.dudy[o] + a.dvdx[o]);
-        const T wx = h * (a.dwdx[o] + a.dwdx[o + P]);
-        a.S13[o] = h * (a.dudz[o] + wx);
-        a.S22[o] = a.dvdy[o];
-        const T wy = h * (a.dwdy[o] + a.dwdy[o + P]);
-        a.S23[o] = h * (a.dvdz[o] + wy);
-        a.S33[o] = a.dwdz[o];
-    } else if (k == g.nz) {
-        const long m = o - P;
-        a.S11[o] = a.dudx[m];
-        a.S12[o] = h * (a.dudy[m] + a.dvdx[m]);
-        a.S13[o] = h * (a.dudz[o] + a.dwdx[o]);
-        a.S22[o] = a.dvdy[m];
-        a.S23[o] = h * (a.dvdz[o] + a.dwdy[o]);
-        a.S33[o] = h * (a.dwdz[m] + (T)0);
-    } else {
-        const long m = o - P;
-        a.S11[o] = h * (a.dudx[o] + a.dudx[m]);
-        const T uy = a.dudy[o] + a.dudy[m];
-        const T vx = a.dvdx[o] + a.dvdx[m];
-        a.S12[o] = (T)0.25 * (uy + vx);
-        a.S13[o] = h * (a.dudz[o] + a.dwdx[o]);
-        a.S22[o] = h * (a.dvdy[o] + a.dvdy[m]);
-        a.S23[o] = h * (a.dvdz[o] + a.dwdy[o]);
-        a.S33[o] = h * (a.dwdz[o] + a.dwdz[m]);
-    }
+    const Sij6<T> r = sij_at(a, o, P, k, g.nz);
+    a.S11[o] = r.s11;
+    a.S12[o] = r.s12;
+    a.S13[o] = r.s13;
+    a.S22[o] = r.s22;
+    a.S23[o] = r.s23;
+    a.S33[o] = r.s33;
 }
 
 template <typename T>
@@ -220,6 +240,61 @@ __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
         a.tzz[o] = (c + c3) * (a.S33[o] + a.S33[q]);
         a.txz[o] = (c2 + c4) * a.S13[o];
         a.tyz[o] = (c2 + c4) * a.S23[o];
+    }
+}
+
+// calc_Sij and the stresses in one pass, for steps without a dynamic-model
+// update in between: S_ij at k (written, as k_calc_sij does) and at k + 1
+// are formed from the derivatives in registers, then Nu_t and tau_ij as in
+// k_sgs_tau (same expressions, so the same values).
+template <typename T>
+__global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g) {
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long P = g.plane;
+    const long o = mb + (long)k * P + (long)j * g.ld + i;
+    const Sij6<T> s0 = sij_at(sa, o, P, k, g.nz);
+    sa.S11[o] = s0.s11;
+    sa.S12[o] = s0.s12;
+    sa.S13[o] = s0.s13;
+    sa.S22[o] = s0.s22;
+    sa.S23[o] = s0.s23;
+    sa.S33[o] = s0.s33;
+    const T nk = smag_norm(s0.s11, s0.s12, s0.s13, s0.s22, s0.s23, s0.s33) * a.Cs[o] * a.l2;
+    a.Nu_t[o] = nk;
+    const T nu = a.nu;
+    if (k == 1) {
+        const T c = (T)-2 * (nk + nu);
+        a.txx[o] = c * s0.s11;
+        a.txy[o] = c * s0.s12;
+        a.tyy[o] = c * s0.s22;
+        a.tzz[o] = c * s0.s33;
+    } else if (k <= g.nz - 1) {
+        const long q = o + P;
+        const Sij6<T> s1 = sij_at(sa, q, P, k + 1, g.nz);
+        const T nk1 = smag_norm(s1.s11, s1.s12, s1.s13, s1.s22, s1.s23, s1.s33) * a.Cs[q] * a.l2;
+        if (k == g.nz - 1) {
+            const T c = (T)0.5 * (nk + nk1) + nu;
+            const T c2 = (T)2 * (nk + nu);
+            a.txx[o] = -c * (s0.s11 + s1.s11);
+            a.txy[o] = -c * (s0.s12 + s1.s12);
+            a.tyy[o] = -c * (s0.s22 + s1.s22);
+            a.tzz[o] = -c * (s0.s33 + s1.s33);
+            a.txz[o] = -c2 * s0.s13;
+            a.tyz[o] = -c2 * s0.s23;
+        } else {
+            const T c = (T)-0.5 * (nk + nk1);
+            const T c2 = (T)-2 * nk;
+            const T c3 = (T)-2 * nu * (T)0.5;
+            const T c4 = (T)-2 * nu;
+            a.txx[o] = (c + c3) * (s0.s11 + s1.s11);
+            a.txy[o] = (c + c3) * (s0.s12 + s1.s12);
+            a.tyy[o] = (c + c3) * (s0.s22 + s1.s22);
+            a.tzz[o] = (c + c3) * (s0.s33 + s1.s33);
+            a.txz[o] = (c2 + c4) * s0.s13;
+            a.tyz[o] = (c2 + c4) * s0.s23;
+        }
     }
 }
 
