@@ -487,15 +487,22 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         c.plan = py.plan;
         // an OP_IKX job on the same input as an OP_NONE job rides along with
         // it: one y transform, two outputs (i kx is constant along y)
+        // (the row pass forms i kx f from the y transform of f: the column
+        // job zeroes its input's ky = ny/2 row, as the i kx job does)
         int nc = 0;
         bool taken[MAX_JOBS] = {};
+        T* rdst[MAX_JOBS] = {};
+        T* rdst2[MAX_JOBS] = {};
+        int nce[MAX_JOBS] = {};
         for (int i = 0; i < nj; ++i) {
             if (taken[i]) continue;
             c.src[nc] = src[j0 + i];
             c.src2[nc] = src2 ? src2[j0 + i] : nullptr;
-            c.dst[nc] = tmpspec[i];
+            c.dst[nc] = tmpspec[nc];
             c.dst2[nc] = nullptr;
             c.op[nc] = ops[j0 + i];
+            rdst[nc] = dst[j0 + i];
+            nce[nc] = ops[j0 + i] == OP_G1 ? g_nzc[0] : ops[j0 + i] == OP_G2 ? g_nzc[1] : 0;
             if (ops[j0 + i] == OP_G1 || ops[j0 + i] == OP_G2) {
                 // test-filtered: nothing outside the filter's band
                 const int f = ops[j0 + i] == OP_G2 ? 1 : 0;
@@ -506,7 +513,8 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
                 for (int i2 = i + 1; i2 < nj; ++i2) {
                     if (!taken[i2] && ops[j0 + i2] == OP_IKX && src[j0 + i2] == src[j0 + i] &&
                         !(src2 && src2[j0 + i2])) {
-                        c.dst2[nc] = tmpspec[i2];
+                        c.nyq0[nc] = 1;
+                        rdst2[nc] = dst[j0 + i2];
                         taken[i2] = true;
                         break;
                     }
@@ -524,12 +532,10 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
         c.zero_nyq = 0;
         col_common(c, d, G1, G2);
         launch_cols<T, +1>(timer, c, nc, nlev, Ma, d.ny, stream, nullptr);
-        std::vector<const cplx<T>*> ts(tmpspec, tmpspec + nj);
-        int nce[MAX_JOBS] = {};
-        for (int i = 0; i < nj; ++i)
-            nce[i] = ops[j0 + i] == OP_G1 ? g_nzc[0] : ops[j0 + i] == OP_G2 ? g_nzc[1] : 0;
-        launch_rows_c2r<T>(timer, px.plan, nj, ts.data(), d.splane, 0, tms, dst + j0, dst_plane,
-                           d.ld, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr, nce);
+        std::vector<const cplx<T>*> ts(tmpspec, tmpspec + nc);
+        launch_rows_c2r<T>(timer, px.plan, nc, ts.data(), d.splane, 0, tms, rdst, dst_plane,
+                           d.ld, dst_lev0, dst_ms, d.ny, d.NK, nlev, Ma, stream, nullptr, nce,
+                           rdst2, (T)(2.0 * PI_D / d.prm.lx));
     }
 }
 

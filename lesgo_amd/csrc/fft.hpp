@@ -18,6 +18,8 @@
 // test_filtermodule.f90:74-75) and is never stored.
 #pragma once
 
+#include <utility>
+
 #include "common.hpp"
 
 namespace lg {
@@ -284,6 +286,9 @@ struct RowC2RArgs {
     long dst_mstride;      // elements between members of dst
     int nce[MAX_JOBS];     // columns of src read per job (0: ncols); the rest are
                            // taken as zero (band-limited spectra)
+    T* dst2[MAX_JOBS];     // when set, the job also writes the x derivative: the
+                           // transform of i kx times its input (a second pass)
+    T kxs;                 // 2 pi / Lx
 };
 
 template <typename T>
@@ -303,51 +308,71 @@ __global__ void __launch_bounds__(256) k_rows_c2r(RowC2RArgs<T> a) {
     const int nc = a.ncols;
     const int nce = a.nce[job] ? a.nce[job] : nc;
     const int half = N / 2;
-    for (int idx = threadIdx.x; idx < S * N; idx += blockDim.x) {
-        const int s = idx / N;
-        const int k = idx - s * N;
-        const int j = 2 * (pair0 + s);
-        const cplx<T>* ra = src + (long)j * nc;
-        const cplx<T>* rb = src + (long)(j + 1) * nc;
-        cplx<T> z;
-        if (k == 0) {
-            z = cplx<T>{ra[0].x, rb[0].x};
-        } else if (k < half) {
-            if (k < nce) {
-                const cplx<T> A = ra[k], B = rb[k];
-                z = cplx<T>{A.x - B.y, A.y + B.x};
-            } else {
+    T* const dst2 = a.dst2[job] ? a.dst2[job] + mem * a.dst_mstride +
+                                      (long)(a.lev0_dst + plane) * a.dst_plane
+                                : nullptr;
+    for (int pass = 0; pass < (dst2 ? 2 : 1); ++pass) {
+        T* const out = pass ? dst2 : dst;
+        for (int idx = threadIdx.x; idx < S * N; idx += blockDim.x) {
+            const int s = idx / N;
+            const int k = idx - s * N;
+            const int j = 2 * (pair0 + s);
+            const cplx<T>* ra = src + (long)j * nc;
+            const cplx<T>* rb = src + (long)(j + 1) * nc;
+            // pass 1: the half spectra times i kx of their column
+            auto col = [&](int q) {
+                cplx<T> A = ra[q], B = rb[q];
+                if (pass) {
+                    const T kx = a.kxs * (T)q;
+                    A = cplx<T>{-A.y * kx, A.x * kx};
+                    B = cplx<T>{-B.y * kx, B.x * kx};
+                }
+                return std::make_pair(A, B);
+            };
+            cplx<T> z;
+            if (k == 0) {
+                const auto AB = col(0);
+                z = cplx<T>{AB.first.x, AB.second.x};
+            } else if (k < half) {
+                if (k < nce) {
+                    const auto AB = col(k);
+                    const cplx<T> A = AB.first, B = AB.second;
+                    z = cplx<T>{A.x - B.y, A.y + B.x};
+                } else {
+                    z = cplx<T>{(T)0, (T)0};
+                }
+            } else if (k == half) {
                 z = cplx<T>{(T)0, (T)0};
-            }
-        } else if (k == half) {
-            z = cplx<T>{(T)0, (T)0};
-        } else {
-            const int kk = N - k;
-            if (kk < nce) {
-                const cplx<T> A = ra[kk], B = rb[kk];
-                // conj(A) + i conj(B)
-                z = cplx<T>{A.x + B.y, B.x - A.y};
             } else {
-                z = cplx<T>{(T)0, (T)0};
+                const int kk = N - k;
+                if (kk < nce) {
+                    const auto AB = col(kk);
+                    const cplx<T> A = AB.first, B = AB.second;
+                    // conj(A) + i conj(B)
+                    z = cplx<T>{A.x + B.y, B.x - A.y};
+                } else {
+                    z = cplx<T>{(T)0, (T)0};
+                }
             }
+            lds[s * a.stride + perm[k]] = z;
         }
-        lds[s * a.stride + perm[k]] = z;
-    }
-    __syncthreads();
-    fft_stages<T, +1>(lds, S, a.stride, a.plan);
-    const int ld = a.dst_ld;
-    for (int idx = threadIdx.x; idx < S * ld; idx += blockDim.x) {
-        const int s = idx / ld;
-        const int n = idx - s * ld;
-        const int j = 2 * (pair0 + s);
-        T va = (T)0, vb = (T)0;
-        if (n < N) {
-            const cplx<T> z = lds[s * a.stride + n];
-            va = z.x;
-            vb = z.y;
+        __syncthreads();
+        fft_stages<T, +1>(lds, S, a.stride, a.plan);
+        const int ld = a.dst_ld;
+        for (int idx = threadIdx.x; idx < S * ld; idx += blockDim.x) {
+            const int s = idx / ld;
+            const int n = idx - s * ld;
+            const int j = 2 * (pair0 + s);
+            T va = (T)0, vb = (T)0;
+            if (n < N) {
+                const cplx<T> z = lds[s * a.stride + n];
+                va = z.x;
+                vb = z.y;
+            }
+            out[(long)j * ld + n] = va;
+            out[(long)(j + 1) * ld + n] = vb;
         }
-        dst[(long)j * ld + n] = va;
-        dst[(long)(j + 1) * ld + n] = vb;
+        __syncthreads();
     }
 }
 
@@ -405,6 +430,9 @@ struct ColArgs {
     // pass writes no others.
     int nzc[MAX_JOBS];
     int ykeep[MAX_JOBS];
+    // inverse MAP_ID jobs: the input's ky = ny/2 row is taken as zero (as
+    // filt_da's is) when set
+    int nyq0[MAX_JOBS];
 };
 
 // Row jb of a small-grid spectrum inside the band |m| < ykeep (0: all rows).
@@ -481,7 +509,7 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
             v = (DIR > 0 && !in_yband(jb, ny, ykeep)) ? cplx<T>{(T)0, (T)0}
                                                       : src[(long)jb * a.ncols + c0 + c];
             if (DIR > 0 && a.map == MAP_ID) {
-                if (dst2 && jb == nyh) v = cplx<T>{(T)0, (T)0};
+                if ((dst2 || a.nyq0[job]) && jb == nyh) v = cplx<T>{(T)0, (T)0};
                 if (op == OP_DIVXY) {
                     const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
                     v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);

@@ -202,63 +202,78 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v2(RowC2RArgs<T> a) {
     T* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
     const int nc = a.ncols;
     const int nce = a.nce[job] ? a.nce[job] : nc;
-    for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
-        const int s = t / N2, n2 = t - s * N2;
-        const int j = 2 * (pair0 + s);
-        const cplx<T>* ra = src + (long)j * nc;
-        const cplx<T>* rb = ra + nc;
-        cplx<T> x[N1];
-#pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) {
-            const int k = n2 + N2 * n1;
-            cplx<T> z{(T)0, (T)0};
-            if (k == 0) {
-                z = cplx<T>{gld(ra).x, gld(rb).x};
-            } else if (k < half) {
-                if (k < nce) {
-                    const cplx<T> A = gld(ra + k), B = gld(rb + k);
-                    z = cplx<T>{A.x - B.y, A.y + B.x};
+    T* const dst2 = a.dst2[job] ? a.dst2[job] + mem * a.dst_mstride +
+                                      (long)(a.lev0_dst + plane) * a.dst_plane
+                                : nullptr;
+    // pass 1 (x derivative): the half spectra times i kx of their column
+    auto ikx2 = [&](cplx<T>& A, cplx<T>& B, int col) {
+        const T kx = a.kxs * (T)col;
+        A = cplx<T>{-A.y * kx, A.x * kx};
+        B = cplx<T>{-B.y * kx, B.x * kx};
+    };
+    for (int pass = 0; pass < (dst2 ? 2 : 1); ++pass) {
+        T* const out = pass ? dst2 : dst;
+        for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
+            const int s = t / N2, n2 = t - s * N2;
+            const int j = 2 * (pair0 + s);
+            const cplx<T>* ra = src + (long)j * nc;
+            const cplx<T>* rb = ra + nc;
+            cplx<T> x[N1];
+    #pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int k = n2 + N2 * n1;
+                cplx<T> z{(T)0, (T)0};
+                if (k == 0) {
+                    z = pass ? cplx<T>{(T)0, (T)0} : cplx<T>{gld(ra).x, gld(rb).x};
+                } else if (k < half) {
+                    if (k < nce) {
+                        cplx<T> A = gld(ra + k), B = gld(rb + k);
+                        if (pass) ikx2(A, B, k);
+                        z = cplx<T>{A.x - B.y, A.y + B.x};
+                    }
+                } else if (k > half) {
+                    const int kk = N - k;
+                    if (kk < nce) {
+                        cplx<T> A = gld(ra + kk), B = gld(rb + kk);
+                        if (pass) ikx2(A, B, kk);
+                        z = cplx<T>{A.x + B.y, B.x - A.y};
+                    }
                 }
-            } else if (k > half) {
-                const int kk = N - k;
-                if (kk < nce) {
-                    const cplx<T> A = gld(ra + kk), B = gld(rb + kk);
-                    z = cplx<T>{A.x + B.y, B.x - A.y};
-                }
+                x[n1] = z;
             }
-            x[n1] = z;
+            dft_reg<T, N1, +1>(x);
+            cplx<T>* Y = lds + s * SP::YS;
+    #pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1)
+                Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, +1>(tw, n2 * k1));
         }
-        dft_reg<T, N1, +1>(x);
-        cplx<T>* Y = lds + s * SP::YS;
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1)
-            Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, +1>(tw, n2 * k1));
-    }
-    __syncthreads();
-    const int ld = a.dst_ld;
-    for (int t = threadIdx.x; t < S * N1; t += blockDim.x) {
-        const int s = t / N1, k1 = t - s * N1;
-        const int j = 2 * (pair0 + s);
-        const cplx<T>* Y = lds + s * SP::YS + k1 * SP::YR;
-        cplx<T> y[N2];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
-        dft_reg<T, N2, +1>(y);
-        T* oa = dst + (long)j * ld;
-        T* ob = oa + ld;
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-            gst(oa + k1 + N1 * k2, y[k2].x);
-            gst(ob + k1 + N1 * k2, y[k2].y);
+        __syncthreads();
+        const int ld = a.dst_ld;
+        for (int t = threadIdx.x; t < S * N1; t += blockDim.x) {
+            const int s = t / N1, k1 = t - s * N1;
+            const int j = 2 * (pair0 + s);
+            const cplx<T>* Y = lds + s * SP::YS + k1 * SP::YR;
+            cplx<T> y[N2];
+    #pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+            dft_reg<T, N2, +1>(y);
+            T* oa = out + (long)j * ld;
+            T* ob = oa + ld;
+    #pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                gst(oa + k1 + N1 * k2, y[k2].x);
+                gst(ob + k1 + N1 * k2, y[k2].y);
+            }
         }
-    }
-    // padding columns N..ld-1
-    const int npad = ld - N;
-    for (int idx = threadIdx.x; idx < S * npad; idx += blockDim.x) {
-        const int s = idx / npad, q = idx - s * npad;
-        const int j = 2 * (pair0 + s);
-        dst[(long)j * ld + N + q] = (T)0;
-        dst[(long)(j + 1) * ld + N + q] = (T)0;
+        // padding columns N..ld-1
+        const int npad = ld - N;
+        for (int idx = threadIdx.x; idx < S * npad; idx += blockDim.x) {
+            const int s = idx / npad, q = idx - s * npad;
+            const int j = 2 * (pair0 + s);
+            out[(long)j * ld + N + q] = (T)0;
+            out[(long)(j + 1) * ld + N + q] = (T)0;
+        }
+        __syncthreads();
     }
 }
 
@@ -307,7 +322,7 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 v = (DIR > 0 && !in_yband(jb, a.ny, ykeep)) ? cplx<T>{(T)0, (T)0}
                                                               : gld(src + (long)jb * a.ncols + c0 + c);
                 if (DIR > 0 && a.map == MAP_ID) {
-                    if (dst2 && jb == nyh) v = cplx<T>{(T)0, (T)0};
+                    if ((dst2 || a.nyq0[job]) && jb == nyh) v = cplx<T>{(T)0, (T)0};
                     if (op == OP_DIVXY) {
                         const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
                         v = divxy<T>(v, gld(s2 + (long)jb * a.ncols + c0 + c), c0 + c, jb, a);

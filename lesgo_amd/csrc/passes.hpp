@@ -81,7 +81,9 @@ std::vector<std::pair<int, int>> pass_splits(int N);
 // Row pass, real to complex (paired rows) / complex to real.  force, when
 // given, overrides the table (the tuner's trial launches).  Band-limited
 // spectra: r2c writes only columns < nwr (0: all), c2r reads only columns
-// < nce[job] (null or 0: all) and takes the rest as zero.
+// < nce[job] (null or 0: all) and takes the rest as zero.  c2r jobs with
+// dst2[job] set also write the x derivative (i kx times the input, kxs =
+// 2 pi / Lx) there.
 template <typename T>
 void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* const* src,
                      const T* scales, long src_plane, int src_ld, int src_lev0, long src_ms,
@@ -92,7 +94,8 @@ template <typename T>
 void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T>* const* src,
                      long src_plane, int src_lev0, long src_ms, T* const* dst, long dst_plane,
                      int dst_ld, int dst_lev0, long dst_ms, int nrows, int ncols, int nlev, int M,
-                     hipStream_t s, const PassChoice* force, const int* nce = nullptr);
+                     hipStream_t s, const PassChoice* force, const int* nce = nullptr,
+                     T* const* dst2 = nullptr, T kxs = T(0));
 // Column pass of nj jobs over nlev * M planes (c.src/dst/op/plan filled;
 // sets c.TC, c.stride, c.nlev).
 template <typename T, int DIR>

@@ -12,7 +12,8 @@ template void launch_rows_r2c<float>(KernelTimer&, const FftPlan&, int, const fl
                                    const PassChoice*, const double*, int, int);
 template void launch_rows_c2r<float>(KernelTimer&, const FftPlan&, int, const cplx<float>* const*,
                                    long, int, long, float* const*, long, int, int, long, int, int,
-                                   int, int, hipStream_t, const PassChoice*, const int*);
+                                   int, int, hipStream_t, const PassChoice*, const int*,
+                                   float* const*, float);
 template void launch_cols<float, -1>(KernelTimer&, ColArgs<float>&, int, int, int, int, hipStream_t,
                                    const PassChoice*);
 template void launch_cols<float, +1>(KernelTimer&, ColArgs<float>&, int, int, int, int, hipStream_t,

@@ -12,7 +12,8 @@ template void launch_rows_r2c<double>(KernelTimer&, const FftPlan&, int, const d
                                    const PassChoice*, const double*, int, int);
 template void launch_rows_c2r<double>(KernelTimer&, const FftPlan&, int, const cplx<double>* const*,
                                    long, int, long, double* const*, long, int, int, long, int, int,
-                                   int, int, hipStream_t, const PassChoice*, const int*);
+                                   int, int, hipStream_t, const PassChoice*, const int*,
+                                   double* const*, double);
 template void launch_cols<double, -1>(KernelTimer&, ColArgs<double>&, int, int, int, int, hipStream_t,
                                    const PassChoice*);
 template void launch_cols<double, +1>(KernelTimer&, ColArgs<double>&, int, int, int, int, hipStream_t,

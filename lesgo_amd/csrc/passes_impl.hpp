@@ -204,20 +204,30 @@ template <typename T>
 void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T>* const* src,
                      long src_plane, int src_lev0, long src_ms, T* const* dst, long dst_plane,
                      int dst_ld, int dst_lev0, long dst_ms, int nrows, int ncols, int nlev, int M,
-                     hipStream_t s, const PassChoice* force, const int* nce) {
-    const PassChoice ch = force ? *force : choice_for<T>(PK_C2R, pl.N, nrows);
+                     hipStream_t s, const PassChoice* force, const int* nce, T* const* dst2,
+                     T kxs) {
+    PassChoice ch = force ? *force : choice_for<T>(PK_C2R, pl.N, nrows);
+    bool dual = false;
+    for (int i = 0; dst2 && i < njobs; ++i) dual |= dst2[i] != nullptr;
+    // the LDS-staged kernel writes its output over its input: a job with a
+    // second output takes the four-step kernel of the same split
+    if (dual && ch.v == 1) ch.v = 0;
     for (int j0 = 0; j0 < njobs; j0 += MAX_JOBS) {
         const int nj = std::min(MAX_JOBS, njobs - j0);
         RowC2RArgs<T> a{};
         a.plan = pl;
         double cin = 0.0;      // complex columns read, summed over the jobs
+        int nout = 0;          // real rows written per row of a job, summed
         for (int i = 0; i < nj; ++i) {
             a.src[i] = src[j0 + i];
             a.dst[i] = dst[j0 + i];
+            a.dst2[i] = dst2 ? dst2[j0 + i] : nullptr;
             const int e = nce ? nce[j0 + i] : 0;
             a.nce[i] = (e > 0 && e < ncols) ? e : 0;
             cin += a.nce[i] ? a.nce[i] : ncols;
+            nout += a.dst2[i] ? 2 : 1;
         }
+        a.kxs = kxs;
         a.src_plane = src_plane;
         a.dst_plane = dst_plane;
         a.dst_ld = dst_ld;
@@ -233,7 +243,7 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
         const size_t lds = row_lds_bytes<T>(ch, pl.N, dst_ld, ncols);
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
         const double bytes = (double)nlev * M * nrows *
-                             (cin * sizeof(cplx<T>) + (double)nj * dst_ld * sizeof(T));
+                             (cin * sizeof(cplx<T>) + (double)nout * dst_ld * sizeof(T));
         tm.run(s, KK_ROWS_C2R, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (!(ch.n1 && v2_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)))
