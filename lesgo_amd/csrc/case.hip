@@ -865,34 +865,82 @@ void Case<T>::lagrange_sdep() {
     const int sids[6] = {LG_S11, LG_S12, LG_S13, LG_S22, LG_S23, LG_S33};
     for (int k0 = 1; k0 <= d.nz; k0 += lag_chunk) {
         const int nc = std::min(lag_chunk, d.nz - k0 + 1);
-        hipLaunchKernelGGL(k_lag_prep<T>, grid_ij(d.nx, d.ny, nc, Ma), dim3(BLK), 0, stream,
-                           fld[LG_U], fld[LG_V], fld[LG_W], fld[LG_S11], fld[LG_S12],
-                           fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33], lag_phys, g, k0,
-                           nc, Ma);
         // slot q of a chunk buffer: Ma members of nc planes each
         const long Qp = (long)Ma * nc * d.plane, mp = (long)nc * d.plane;
         const long Qs = (long)Ma * nc * d.splane, ms = (long)nc * d.splane;
         // spectral slots 0-8 prep, 9-14 S_ij, 15-20 |S| S_ij
         cplx<T>* dst[21];
         for (int q = 0; q < 21; ++q) dst[q] = lag_spec + q * Qs;
-        const T* psrc[15];
-        cplx<T>* pdst[15];
-        for (int q = 0; q < 9; ++q) {
-            psrc[q] = lag_phys + q * Qp;
-            pdst[q] = dst[q];
+        if (lag_fused) {
+            // the 21 quantities formed and transformed along x in one pass
+            // (k_lag_prep_rows), then the column pass over the G_test band
+            LagPrepArgs<T> pa{};
+            pa.plan = px.plan;
+            pa.u = fld[LG_U];
+            pa.v = fld[LG_V];
+            pa.w = fld[LG_W];
+            pa.S11 = fld[LG_S11];
+            pa.S12 = fld[LG_S12];
+            pa.S13 = fld[LG_S13];
+            pa.S22 = fld[LG_S22];
+            pa.S23 = fld[LG_S23];
+            pa.S33 = fld[LG_S33];
+            pa.fplane = d.plane;
+            pa.fmstride = d.field;
+            pa.ld = d.ld;
+            pa.nz = d.nz;
+            pa.k0 = k0;
+            pa.nlev = nc;
+            pa.out = lag_spec;
+            pa.qstride = Qs;
+            pa.plane = d.splane;
+            pa.mstride = ms;
+            pa.ncols = d.NK;
+            pa.nwr = g_nzc[0];
+            if (!launch_lag_prep_rows<T>(timer, pa, Ma, stream))
+                throw std::runtime_error("k_lag_prep_rows not launchable for this grid");
+            for (int q0 = 0; q0 < 21; q0 += MAX_JOBS) {
+                const int nj = std::min(MAX_JOBS, 21 - q0);
+                ColArgs<T> cc{};
+                cc.plan = py.plan;
+                for (int i = 0; i < nj; ++i) {
+                    cc.src[i] = cc.dst[i] = dst[q0 + i];
+                    cc.op[i] = OP_NONE;
+                    cc.nzc[i] = g_nzc[0];
+                    cc.ykeep[i] = g_ykeep[0];
+                }
+                cc.src_plane = cc.dst_plane = d.splane;
+                cc.lev0_src = cc.lev0_dst = 0;
+                cc.src_mstride = cc.dst_mstride = ms;
+                cc.map = MAP_ID;
+                cc.zero_nyq = 1;
+                col_common(cc, d, G1, G2);
+                launch_cols<T, -1>(timer, cc, nj, nc, Ma, d.ny, stream, nullptr);
+            }
+        } else {
+            hipLaunchKernelGGL(k_lag_prep<T>, grid_ij(d.nx, d.ny, nc, Ma), dim3(BLK), 0, stream,
+                               fld[LG_U], fld[LG_V], fld[LG_W], fld[LG_S11], fld[LG_S12],
+                               fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33], lag_phys, g,
+                               k0, nc, Ma);
+            const T* psrc[15];
+            cplx<T>* pdst[15];
+            for (int q = 0; q < 9; ++q) {
+                psrc[q] = lag_phys + q * Qp;
+                pdst[q] = dst[q];
+            }
+            for (int q = 0; q < 6; ++q) {
+                psrc[9 + q] = lag_phys + (9 + q) * Qp;
+                pdst[9 + q] = dst[15 + q];
+            }
+            // only the test filters read these spectra: the G1 band (G2's lies
+            // inside it)
+            fwd2d(15, psrc, nullptr, d.plane, 0, mp, pdst, d.splane, 0, ms, nc, nullptr, g_nzc[0],
+                  g_ykeep[0]);
+            const T* ssrc[6];
+            for (int q = 0; q < 6; ++q) ssrc[q] = fld[sids[q]];
+            fwd2d(6, ssrc, nullptr, d.plane, k0, d.field, dst + 9, d.splane, 0, ms, nc, nullptr,
+                  g_nzc[0], g_ykeep[0]);
         }
-        for (int q = 0; q < 6; ++q) {
-            psrc[9 + q] = lag_phys + (9 + q) * Qp;
-            pdst[9 + q] = dst[15 + q];
-        }
-        // only the test filters read these spectra: the G1 band (G2's lies
-        // inside it)
-        fwd2d(15, psrc, nullptr, d.plane, 0, mp, pdst, d.splane, 0, ms, nc, nullptr, g_nzc[0],
-              g_ykeep[0]);
-        const T* ssrc[6];
-        for (int q = 0; q < 6; ++q) ssrc[q] = fld[sids[q]];
-        fwd2d(6, ssrc, nullptr, d.plane, k0, d.field, dst + 9, d.splane, 0, ms, nc, nullptr,
-              g_nzc[0], g_ykeep[0]);
         double* contr = nullptr;
         if (lag_fused) {
             // per filter: the 21 quantities' column pass into lag_filt

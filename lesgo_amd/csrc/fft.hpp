@@ -194,6 +194,20 @@ struct LagRowArgs {
     double tf2;                // (filter width / delta)^2
 };
 
+// The Lagrangian model's forward row pass forming its quantities
+// (lag_rows.hpp)
+template <typename T>
+struct LagPrepArgs {
+    FftPlan plan;              // length nx
+    const T *u, *v, *w;        // physical fields [member][0..nz][ny][ld]
+    const T *S11, *S12, *S13, *S22, *S23, *S33;
+    long fplane, fmstride;     // elements per level / member of those fields
+    int ld, nz, k0, nlev;      // levels k0 .. k0 + nlev - 1; grid.y = nlev * members
+    cplx<T>* out;              // quantity q at out + q * qstride: [member][level][j][kx < ncols]
+    long qstride, plane, mstride;   // complex elements
+    int ncols, nwr;            // nx / 2, columns written (the filters' band)
+};
+
 template <typename T>
 struct RowR2CArgs {
     FftPlan plan;

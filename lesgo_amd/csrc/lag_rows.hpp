@@ -1,4 +1,7 @@
-// The Lagrangian model's inverse row pass with its tensor contractions
+// The Lagrangian model's row passes: the inverse pass with its tensor
+// contractions, and the forward pass that forms the quantities to filter.
+//
+// Inverse: the Lagrangian model's inverse row pass with its tensor contractions
 // (lagrange_Sdep.f90:137-167, 203-259): for one row pair (j, j+1) of one
 // level, the 21 quantities filtered at one width (the column pass's output
 // of ubar, vbar, wbar, the six products, S_ij and |S| S_ij) are transformed
@@ -113,6 +116,116 @@ __global__ void __launch_bounds__(LAGROW_BLK) k_lag_rows(LagRowArgs<T> a) {
         const long o = (long)mem * a.omstride + (long)c * a.oplane + (long)(j + r) * a.ld + n;
         a.out0[o] = LM;
         a.out1[o] = MM;
+    }
+}
+
+// Forward: the 21 quantities the test filters act on (lagrange_Sdep.f90:
+// 93-151, 211-240; k_lag_prep's formulas) formed for one row pair of one
+// level from u, v, w and S_ij, and transformed along x in LDS (the paired
+// r2c of k_rows_r2c_v2), half spectra written for kx < nwr only (the
+// filters' band).  Quantities: 0-2 ubar, vbar, wbar; 3-8 the products 11,
+// 12, 13, 22, 23, 33; 9-14 S_ij; 15-20 |S| S_ij.
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(LAGROW_BLK) k_lag_prep_rows(LagPrepArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    constexpr int YS = SP::YS;
+    static_assert(LAGROW_Q * N1 <= LAGROW_BLK && LAGROW_Q * N2 <= LAGROW_BLK,
+                  "each step in one round");
+    extern __shared__ __align__(16) unsigned char smem_lp[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_lp);
+    const auto* tw = plan_tw<T>(a.plan);
+    const int mem = blockIdx.y / a.nlev;
+    const int c = blockIdx.y - mem * a.nlev;
+    const int k = a.k0 + c;
+    const int j = 2 * blockIdx.x;
+    const int tid = threadIdx.x;
+    const long P = a.fplane;
+    // the quantities at every point of the two rows, into slot q (row j in
+    // .x, row j + 1 in .y), natural order
+    for (int t = tid; t < 2 * N; t += blockDim.x) {
+        const int r = t / N, i = t - r * N;
+        const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + r) * a.ld + i;
+        T ub, vb, wb;
+        if (k == 1) {
+            ub = a.u[o];
+            vb = a.v[o];
+            wb = (T)0.25 * a.w[o + P];
+        } else if (k == a.nz) {
+            ub = a.u[o - P];
+            vb = a.v[o - P];
+            wb = (T)0;
+        } else {
+            ub = (T)0.5 * (a.u[o] + a.u[o - P]);
+            vb = (T)0.5 * (a.v[o] + a.v[o - P]);
+            wb = a.w[o];
+        }
+        const T s11 = a.S11[o], s12 = a.S12[o], s13 = a.S13[o], s22 = a.S22[o], s23 = a.S23[o],
+                s33 = a.S33[o];
+        const T S = smag_norm(s11, s12, s13, s22, s23, s33);
+        const T q[LAGROW_Q] = {ub,      vb,      wb,      ub * ub, ub * vb, ub * wb, vb * vb,
+                               vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23,
+                               s33,     S * s11, S * s12, S * s13, S * s22, S * s23, S * s33};
+#pragma unroll
+        for (int qq = 0; qq < LAGROW_Q; ++qq) {
+            T* e = reinterpret_cast<T*>(lds + qq * YS + i) + r;
+            *e = q[qq];
+        }
+    }
+    __syncthreads();
+    // forward transforms: step 1 from natural order into the four-step
+    // layout of a second copy would need twice the LDS, so each task reads
+    // its N1 inputs into registers first and the slot is rewritten after a
+    // barrier
+    {
+        cplx<T> x[N1];
+        const int q = tid / N2, n2 = tid - (tid / N2) * N2;
+        const bool act = q < LAGROW_Q;
+        if (act) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) x[n1] = lds[q * YS + n2 + N2 * n1];
+            dft_reg<T, N1, -1>(x);
+        }
+        __syncthreads();
+        if (act) {
+            cplx<T>* Y = lds + q * YS;
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1)
+                Y[k1 * SP::YR + n2] = (k1 == 0 || n2 == 0) ? x[k1] : cmul(x[k1], wbig<T, -1>(tw, n2 * k1));
+        }
+        __syncthreads();
+    }
+    {
+        const int q = tid / N1, k1 = tid - (tid / N1) * N1;
+        const bool act = q < LAGROW_Q;
+        cplx<T> y[N2];
+        if (act) {
+            const cplx<T>* Y = lds + q * YS + k1 * SP::YR;
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+            dft_reg<T, N2, -1>(y);
+        }
+        __syncthreads();
+        if (act) {
+            cplx<T>* X = lds + q * YS;
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) X[k1 + N1 * k2] = y[k2];
+        }
+        __syncthreads();
+    }
+    // split each packed transform into the two rows' half spectra, kx < nwr
+    const int nc = a.ncols, nw = a.nwr;
+    for (int t = tid; t < LAGROW_Q * nw; t += blockDim.x) {
+        const int q = t / nw, kx = t - q * nw;
+        const cplx<T>* X = lds + q * YS;
+        const cplx<T> z = X[kx];
+        const cplx<T> zc = X[(N - kx) % N];
+        const cplx<T> A{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y - zc.y)};
+        const cplx<T> Bm{(T)0.5 * (z.x - zc.x), (T)0.5 * (z.y + zc.y)};
+        cplx<T>* o = a.out + (long)q * a.qstride + (long)mem * a.mstride + (long)c * a.plane +
+                     (long)j * nc;
+        gst(o + kx, A);
+        gst(o + nc + kx, cplx<T>{Bm.y, -Bm.x});
     }
 }
 

@@ -107,5 +107,8 @@ template <typename T>
 bool launch_lag_rows(KernelTimer& tm, const LagRowArgs<T>& a, int M, hipStream_t s);
 template <typename T>
 bool lag_rows_ok(int N, int nrows);
+// The Lagrangian model's forward row pass forming its 21 quantities.
+template <typename T>
+bool launch_lag_prep_rows(KernelTimer& tm, const LagPrepArgs<T>& a, int M, hipStream_t s);
 
 }  // namespace lg

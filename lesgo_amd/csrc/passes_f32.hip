@@ -20,5 +20,6 @@ template void launch_cols<float, +1>(KernelTimer&, ColArgs<float>&, int, int, in
                                    const PassChoice*);
 template bool launch_lag_rows<float>(KernelTimer&, const LagRowArgs<float>&, int, hipStream_t);
 template bool lag_rows_ok<float>(int, int);
+template bool launch_lag_prep_rows<float>(KernelTimer&, const LagPrepArgs<float>&, int, hipStream_t);
 
 }  // namespace lg

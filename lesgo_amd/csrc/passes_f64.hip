@@ -22,5 +22,6 @@ template void launch_cols<double, +1>(KernelTimer&, ColArgs<double>&, int, int, 
 std::vector<std::pair<int, int>> pass_splits(int N) { return splits_for(N); }
 template bool launch_lag_rows<double>(KernelTimer&, const LagRowArgs<double>&, int, hipStream_t);
 template bool lag_rows_ok<double>(int, int);
+template bool launch_lag_prep_rows<double>(KernelTimer&, const LagPrepArgs<double>&, int, hipStream_t);
 
 }  // namespace lg

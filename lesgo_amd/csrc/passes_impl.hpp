@@ -322,4 +322,28 @@ bool launch_lag_rows(KernelTimer& tm, const LagRowArgs<T>& a, int M, hipStream_t
     return true;
 }
 
+// The Lagrangian model's forward row pass forming its quantities
+// (lag_rows.hpp); launchable where lag_rows_ok is.
+template <typename T>
+bool launch_lag_prep_rows(KernelTimer& tm, const LagPrepArgs<T>& a, int M, hipStream_t s) {
+    const int N = a.plan.N;
+    int n1, n2;
+    if (!lag_rows_ok<T>(N, 2) || !split_of(N, n1, n2) || LAGROW_Q * n2 > LAGROW_BLK ||
+        a.nwr < 1 || a.nwr > a.ncols)
+        return false;
+    const size_t lds = (size_t)LAGROW_Q * (n1 * (n2 + 1) + 1) * sizeof(cplx<T>);
+    const dim3 grid(a.fplane / a.ld / 2, a.nlev * M);
+    // algorithmic bytes: u, v, w and the six S_ij read once, the 21 half
+    // spectra written over the band
+    const double rows = (double)M * a.nlev * (a.fplane / a.ld);
+    const double bytes = rows * (9.0 * N * sizeof(T) + LAGROW_Q * (double)a.nwr * sizeof(cplx<T>));
+    tm.run(s, KK_ROWS_R2C, bytes, [&] {
+#define LG_CASE(A, B)                                                                \
+    if (n1 == A && n2 == B) LG_LAUNCH((k_lag_prep_rows<T, A, B>), grid, dim3(LAGROW_BLK), lds, s, a);
+        LG_LAGROW_SPLITS(LG_CASE)
+#undef LG_CASE
+    });
+    return true;
+}
+
 }  // namespace lg

@@ -1,12 +1,15 @@
-"""The Lagrangian model's fused inverse row pass (k_lag_rows: the 21
-filtered quantities of each width transformed back in LDS and contracted
-into L_ij M_ij, M_ij M_ij, Q_ij N_ij, N_ij N_ij at once) against the 42
-filtered fields of the separate passes (LESGO_AMD_LAG_ROWS=0): the same
-Cs_opt2, F_* and flow, bit for bit, through a fresh start's first dynamic
-call and a later dynamic step, alone and in a batch (the transforms are the
-same four-step arithmetic and the contractions the same FP64 expressions of
-the same filtered values); a grid whose nx has no fused kernel takes the
-separate passes either way."""
+"""The Lagrangian model's fused row passes against the separate passes
+(LESGO_AMD_LAG_ROWS=0): k_lag_prep_rows forms the 21 quantities to filter
+and transforms them along x in one pass, and k_lag_rows transforms each
+filter width's 21 filtered quantities back in LDS and contracts them into
+L_ij M_ij, M_ij M_ij, Q_ij N_ij, N_ij N_ij at once, so neither the
+quantities nor the 42 filtered fields reach HBM.  The same Cs_opt2, F_* and
+flow through a fresh start's first dynamic call and a later dynamic step,
+alone and in a batch, to rounding: the same expressions and four-step
+transforms, but the compiler contracts |S| into FMAs differently in the
+fused kernel (gap after the first dynamic call: 1e-16 in u in FP64,
+2e-5 in Cs_opt2 in mixed).  A grid whose
+nx has no fused kernel takes the separate passes either way, bit for bit."""
 import os
 
 import numpy as np
@@ -60,18 +63,22 @@ def _run_pair(nx, ny, nz, precision, tols, members=1):
 
 
 EXACT = {10: 0.0, 15: 0.0}
+# Cs_opt2 is the most sensitive: L_ij is a difference of nearly equal
+# filtered products, so float rounding of the inputs shows ~100-fold
+# (mixed: 1.9e-5 at the first dynamic call)
+ROUNDING = {"fp64": {10: 1e-13, 15: 1e-12}, "mixed": {10: 1e-4, 15: 1e-3}}
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_fused_lag_rows_equal_separate_passes(precision):
     # 48 x 32 x 16: nx = 48 (split 6 x 8); dynamic calls at steps 10 and 15
-    _run_pair(48, 32, 16, precision, EXACT)
+    _run_pair(48, 32, 16, precision, ROUNDING[precision])
 
 
 def test_fused_lag_rows_batch_and_256_rows():
     # the bench's x length (nx = 256, split 16 x 16), two members (FP64: this
     # short 8-level domain does not stay stable in mixed precision)
-    _run_pair(256, 32, 8, "fp64", EXACT, members=2)
+    _run_pair(256, 32, 8, "fp64", ROUNDING["fp64"], members=2)
 
 
 def test_grid_without_fused_lag_rows_takes_separate_passes():
