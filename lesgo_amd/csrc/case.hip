@@ -129,6 +129,20 @@ static T* dalloc(long n) {
     return p;
 }
 
+// The Lagrangian model's fused row passes (k_lag_prep_rows, k_lag_rows):
+// where nx has a kernel for them and LESGO_AMD_LAG_ROWS is not 0.  They
+// need no physical quantity fields and 21 complex + 4 FP64 filtered slots
+// instead of 42 real ones.
+static bool lag_fused_for(int nx, int ny, int tsize) {
+    const char* lr = std::getenv("LESGO_AMD_LAG_ROWS");
+    if (lr && lr[0] == '0') return false;
+    return tsize == 4 ? lag_rows_ok<float>(nx, ny) : lag_rows_ok<double>(nx, ny);
+}
+
+// real slots of the filtered buffer: 42 fields, or 21 complex spectra (each
+// within one field) and four FP64 contraction fields
+static long lag_filt_slots(bool fused, int tsize) { return fused ? 21 + 4 * (8 / tsize) : 42; }
+
 template <typename T>
 Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     msp = (long)(d.nz + 1) * d.splane;
@@ -147,9 +161,10 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     usc = dalloc<double2>(M * msp);
     dev_err = dalloc<int>(1);
     lag_chunk = std::min(d.nz, 64);
-    lag_phys = dalloc<T>(15L * M * lag_chunk * d.plane);
+    lag_fused = lag_fused_for(d.nx, d.ny, (int)sizeof(T));
+    if (!lag_fused) lag_phys = dalloc<T>(15L * M * lag_chunk * d.plane);
     lag_spec = dalloc<cplx<T>>(21L * M * lag_chunk * d.splane);
-    lag_filt = dalloc<T>(42L * M * lag_chunk * d.plane);
+    lag_filt = dalloc<T>(lag_filt_slots(lag_fused, (int)sizeof(T)) * M * lag_chunk * d.plane);
     Ftmp = dalloc<T>(4L * M * d.field);
     red = dalloc<double>(1024);
     d_denom = dalloc<double>(M);
@@ -175,8 +190,6 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     // environment keeps every case on the all-physical step
     const char* phys = std::getenv("LESGO_AMD_PHYSICAL_STEP");
     spec_on = !cfl_mode && !(phys && phys[0] == '1');
-    const char* lr = std::getenv("LESGO_AMD_LAG_ROWS");
-    lag_fused = !(lr && lr[0] == '0') && lag_rows_ok<T>(d.nx, d.ny);
     if (spec_on) {
         for (int q = 0; q < 3; ++q) {
             sR[q] = dalloc<cplx<T>>(M * msp);
@@ -413,7 +426,9 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
     b += (size_t)(12 + 3) * M * msp * 2 * T;                     // spec, sU/sV/sW
     b += 9 * M * mbp * T + 6 * M * mbsp * 2 * T;                 // big, bigspec
     b += M * msp * (sizeof(double) + sizeof(double2));           // gam, usc
-    b += (15 + 42) * M * chunk * d.plane * T + 21 * M * chunk * d.splane * 2 * T;
+    const bool lf = lag_fused_for(d.nx, d.ny, tsize);
+    b += ((lf ? 0 : 15) + lag_filt_slots(lf, tsize)) * M * chunk * d.plane * T +
+         21 * M * chunk * d.splane * 2 * T;
     b += 3 * M * sizeof(double) + 1024 * sizeof(double) + 2 * d.splane * T;
     b += M * (CK_N * sizeof(double) + 3 * sizeof(unsigned long long) + 2 * sizeof(int));
     if (!p.use_cfl_dt) b += 13 * M * msp * 2 * T + 6 * M * d.field * T;   // spectral state
