@@ -837,7 +837,8 @@ void Case<T>::sgs_stag() {
         SijArgs<T> sa{fld[LG_DUDX], fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDY],
                       fld[LG_DVDZ], fld[LG_DWDX], fld[LG_DWDY], fld[LG_DWDZ], fld[LG_S11],
                       fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33]};
-        hipLaunchKernelGGL(k_sij_tau<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
+        const int nch = (d.nz + SIJ_L - 1) / SIJ_L;
+        hipLaunchKernelGGL(k_sij_tau<T>, grid_ij(d.nx, d.ny, nch, Ma), dim3(BLK), 0, stream,
                            sa, a, GEO(1, d.nz));
     }
     fill_levels({{LG_TXX, 0, BOGUS}, {LG_TXY, 0, BOGUS}, {LG_TXZ, 0, BOGUS}, {LG_TYY, 0, BOGUS},

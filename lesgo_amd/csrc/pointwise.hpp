@@ -132,6 +132,10 @@ struct Sij6 {
 
 template <typename T>
 __device__ __forceinline__ Sij6<T> sij_at(const SijArgs<T>& a, long o, long P, int k, int nz) {
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
     const T h = (T)0.5;
     Sij6<T> r;
     if (k == 1) {
@@ -167,6 +171,10 @@ __device__ __forceinline__ Sij6<T> sij_at(const SijArgs<T>& a, long o, long P, i
 
 template <typename T>
 __global__ void k_calc_sij(SijArgs<T> a, Geo g) {
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
     int m, k, j, i;
     long mb;
     if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
@@ -183,6 +191,10 @@ __global__ void k_calc_sij(SijArgs<T> a, Geo g) {
 
 template <typename T>
 __device__ __forceinline__ T smag_norm(T s11, T s12, T s13, T s22, T s23, T s33) {
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
     return sqrt((T)2 * (s11 * s11 + s22 * s22 + s33 * s33 +
                         (T)2 * (s12 * s12 + s13 * s13 + s23 * s23)));
 }
@@ -197,11 +209,19 @@ struct TauArgs {
 
 template <typename T>
 __device__ __forceinline__ T nut_at(const TauArgs<T>& a, long o) {
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
     return smag_norm(a.S11[o], a.S12[o], a.S13[o], a.S22[o], a.S23[o], a.S33[o]) * a.Cs[o] * a.l2;
 }
 
 template <typename T>
 __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
     int m, k, j, i;
     long mb;
     if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
@@ -243,57 +263,139 @@ __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
     }
 }
 
+// The nine derivatives at one point of one level.
+template <typename T>
+struct Der9 {
+    T ux, uy, uz, vx, vy, vz, wx, wy, wz;
+};
+
+template <typename T>
+__device__ __forceinline__ Der9<T> der_at(const SijArgs<T>& a, long o) {
+    return Der9<T>{a.dudx[o], a.dudy[o], a.dudz[o], a.dvdx[o], a.dvdy[o],
+                   a.dvdz[o], a.dwdx[o], a.dwdy[o], a.dwdz[o]};
+}
+
+// sij_at's formulas on derivatives held in registers: dm at k - 1, d at k,
+// dp at k + 1 (read for k = 1 only).
+template <typename T>
+__device__ __forceinline__ Sij6<T> sij_of(const Der9<T>& dm, const Der9<T>& d,
+                                          const Der9<T>& dp, int k, int nz) {
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
+    const T h = (T)0.5;
+    Sij6<T> r;
+    if (k == 1) {
+        r.s11 = d.ux;
+        r.s12 = h * (d.uy + d.vx);
+        const T wx = h * (d.wx + dp.wx);
+        r.s13 = h * (d.uz + wx);
+        r.s22 = d.vy;
+        const T wy = h * (d.wy + dp.wy);
+        r.s23 = h * (d.vz + wy);
+        r.s33 = d.wz;
+    } else if (k == nz) {
+        r.s11 = dm.ux;
+        r.s12 = h * (dm.uy + dm.vx);
+        r.s13 = h * (d.uz + d.wx);
+        r.s22 = dm.vy;
+        r.s23 = h * (d.vz + d.wy);
+        r.s33 = h * (dm.wz + (T)0);
+    } else {
+        r.s11 = h * (d.ux + dm.ux);
+        const T uy = d.uy + dm.uy;
+        const T vx = d.vx + dm.vx;
+        r.s12 = (T)0.25 * (uy + vx);
+        r.s13 = h * (d.uz + d.wx);
+        r.s22 = h * (d.vy + dm.vy);
+        r.s23 = h * (d.vz + d.wy);
+        r.s33 = h * (d.wz + dm.wz);
+    }
+    return r;
+}
+
 // calc_Sij and the stresses in one pass, for steps without a dynamic-model
-// update in between: S_ij at k (written, as k_calc_sij does) and at k + 1
-// are formed from the derivatives in registers, then Nu_t and tau_ij as in
-// k_sgs_tau (same expressions, so the same values).
+// update in between.  A thread walks SIJ_L consecutive levels of one (i, j)
+// and keeps the level below's derivatives and S_ij in registers, so each
+// level's nine derivatives and Cs_opt2 are read once; it writes S_ij (as
+// k_calc_sij does), Nu_t and tau_ij (as k_sgs_tau does, same expressions).
+// Grid: x over (i, j), y over chunks of SIJ_L levels times members.
+constexpr int SIJ_L = 8;
+
 template <typename T>
 __global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g) {
-    int m, k, j, i;
-    long mb;
-    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
+    // same roundings, so members batched with others take the same bits
+    // whichever pass a step uses
+#pragma clang fp contract(off)
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int nch = (g.nz + SIJ_L - 1) / SIJ_L;
+    const int m = blockIdx.y / nch;
+    const int kb = 1 + (int)(blockIdx.y - m * nch) * SIJ_L;
+    const int ke = min(kb + SIJ_L - 1, g.nz);
     const long P = g.plane;
-    const long o = mb + (long)k * P + (long)j * g.ld + i;
-    const Sij6<T> s0 = sij_at(sa, o, P, k, g.nz);
-    sa.S11[o] = s0.s11;
-    sa.S12[o] = s0.s12;
-    sa.S13[o] = s0.s13;
-    sa.S22[o] = s0.s22;
-    sa.S23[o] = s0.s23;
-    sa.S33[o] = s0.s33;
-    const T nk = smag_norm(s0.s11, s0.s12, s0.s13, s0.s22, s0.s23, s0.s33) * a.Cs[o] * a.l2;
-    a.Nu_t[o] = nk;
+    const long base = (long)m * g.mstride + (long)j * g.ld + i;
+    const int nz = g.nz;
     const T nu = a.nu;
-    if (k == 1) {
-        const T c = (T)-2 * (nk + nu);
-        a.txx[o] = c * s0.s11;
-        a.txy[o] = c * s0.s12;
-        a.tyy[o] = c * s0.s22;
-        a.tzz[o] = c * s0.s33;
-    } else if (k <= g.nz - 1) {
-        const long q = o + P;
-        const Sij6<T> s1 = sij_at(sa, q, P, k + 1, g.nz);
-        const T nk1 = smag_norm(s1.s11, s1.s12, s1.s13, s1.s22, s1.s23, s1.s33) * a.Cs[q] * a.l2;
-        if (k == g.nz - 1) {
-            const T c = (T)0.5 * (nk + nk1) + nu;
-            const T c2 = (T)2 * (nk + nu);
-            a.txx[o] = -c * (s0.s11 + s1.s11);
-            a.txy[o] = -c * (s0.s12 + s1.s12);
-            a.tyy[o] = -c * (s0.s22 + s1.s22);
-            a.tzz[o] = -c * (s0.s33 + s1.s33);
-            a.txz[o] = -c2 * s0.s13;
-            a.tyz[o] = -c2 * s0.s23;
-        } else {
-            const T c = (T)-0.5 * (nk + nk1);
-            const T c2 = (T)-2 * nk;
-            const T c3 = (T)-2 * nu * (T)0.5;
-            const T c4 = (T)-2 * nu;
-            a.txx[o] = (c + c3) * (s0.s11 + s1.s11);
-            a.txy[o] = (c + c3) * (s0.s12 + s1.s12);
-            a.tyy[o] = (c + c3) * (s0.s22 + s1.s22);
-            a.tzz[o] = (c + c3) * (s0.s33 + s1.s33);
-            a.txz[o] = (c2 + c4) * s0.s13;
-            a.tyz[o] = (c2 + c4) * s0.s23;
+    // derivatives at kb - 1 (when kb >= 2), kb and kb + 1
+    Der9<T> dm{}, d = der_at(sa, base + (long)kb * P), dp{};
+    if (kb >= 2) dm = der_at(sa, base + (long)(kb - 1) * P);
+    if (kb < nz) dp = der_at(sa, base + (long)(kb + 1) * P);
+    Sij6<T> s0 = sij_of(dm, d, dp, kb, nz);
+    T nk = smag_norm(s0.s11, s0.s12, s0.s13, s0.s22, s0.s23, s0.s33) * a.Cs[base + (long)kb * P] * a.l2;
+#pragma unroll
+    for (int q = 0; q < SIJ_L; ++q) {
+        const int k = kb + q;
+        if (k <= ke) {
+            const long o = base + (long)k * P;
+            sa.S11[o] = s0.s11;
+            sa.S12[o] = s0.s12;
+            sa.S13[o] = s0.s13;
+            sa.S22[o] = s0.s22;
+            sa.S23[o] = s0.s23;
+            sa.S33[o] = s0.s33;
+            a.Nu_t[o] = nk;
+            if (k == 1) {
+                const T c = (T)-2 * (nk + nu);
+                a.txx[o] = c * s0.s11;
+                a.txy[o] = c * s0.s12;
+                a.tyy[o] = c * s0.s22;
+                a.tzz[o] = c * s0.s33;
+            }
+            if (k <= nz - 1) {
+                // S_ij and Nu_t at k + 1 from the derivatives at k and k + 1
+                // (k + 2 is not read: k + 1 >= 2)
+                const Der9<T> dnext = (k == kb) ? dp : der_at(sa, o + P);
+                const Sij6<T> s1 = sij_of(d, dnext, dnext, k + 1, nz);
+                const T nk1 = smag_norm(s1.s11, s1.s12, s1.s13, s1.s22, s1.s23, s1.s33) *
+                              a.Cs[o + P] * a.l2;
+                if (k == nz - 1 && k != 1) {
+                    const T c = (T)0.5 * (nk + nk1) + nu;
+                    const T c2 = (T)2 * (nk + nu);
+                    a.txx[o] = -c * (s0.s11 + s1.s11);
+                    a.txy[o] = -c * (s0.s12 + s1.s12);
+                    a.tyy[o] = -c * (s0.s22 + s1.s22);
+                    a.tzz[o] = -c * (s0.s33 + s1.s33);
+                    a.txz[o] = -c2 * s0.s13;
+                    a.tyz[o] = -c2 * s0.s23;
+                } else if (k != 1) {
+                    const T c = (T)-0.5 * (nk + nk1);
+                    const T c2 = (T)-2 * nk;
+                    const T c3 = (T)-2 * nu * (T)0.5;
+                    const T c4 = (T)-2 * nu;
+                    a.txx[o] = (c + c3) * (s0.s11 + s1.s11);
+                    a.txy[o] = (c + c3) * (s0.s12 + s1.s12);
+                    a.tyy[o] = (c + c3) * (s0.s22 + s1.s22);
+                    a.tzz[o] = (c + c3) * (s0.s33 + s1.s33);
+                    a.txz[o] = (c2 + c4) * s0.s13;
+                    a.tyz[o] = (c2 + c4) * s0.s23;
+                }
+                d = dnext;
+                s0 = s1;
+                nk = nk1;
+            }
         }
     }
 }
