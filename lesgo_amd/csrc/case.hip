@@ -837,9 +837,15 @@ void Case<T>::sgs_stag() {
         SijArgs<T> sa{fld[LG_DUDX], fld[LG_DUDY], fld[LG_DUDZ], fld[LG_DVDX], fld[LG_DVDY],
                       fld[LG_DVDZ], fld[LG_DWDX], fld[LG_DWDY], fld[LG_DWDZ], fld[LG_S11],
                       fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23], fld[LG_S33]};
+        // a spectral-resident case leaves S_ij and Nu_t to be made on demand
         const int nch = (d.nz + SIJ_L - 1) / SIJ_L;
         hipLaunchKernelGGL(k_sij_tau<T>, grid_ij(d.nx, d.ny, nch, Ma), dim3(BLK), 0, stream,
-                           sa, a, GEO(1, d.nz));
+                           sa, a, GEO(1, d.nz), spec_on ? 0 : 1);
+    }
+    if (spec_on) {
+        // (after_body does the same for whole steps, replayed graphs included)
+        if (lag_now) stale &= ~(unsigned)SG_SIJ;
+        else stale |= SG_SIJ;
     }
     fill_levels({{LG_TXX, 0, BOGUS}, {LG_TXY, 0, BOGUS}, {LG_TXZ, 0, BOGUS}, {LG_TYY, 0, BOGUS},
                  {LG_TYZ, 0, BOGUS}, {LG_TZZ, 0, BOGUS}, {LG_TXX, d.nz, BOGUS},
@@ -1490,6 +1496,10 @@ void Case<T>::after_body() {
         for (int m = 0; m < Ma; ++m) any_euler |= hflags[m] != 0;
         stale |= SG_UVW | SG_RHS | SG_P | SG_DIVT;
         if (cold && any_euler) stale |= SG_RHSF;
+        // S_ij and Nu_t are stored only by a step with a dynamic-model update
+        // (set here too: a replayed graph runs no host code of sgs_stag)
+        if (lag_now) stale &= ~(unsigned)SG_SIJ;
+        else stale |= SG_SIJ;
         std::fill(mcold.begin(), mcold.end(), 0);
     }
     if (st.on) stats_after_step();
@@ -1632,6 +1642,14 @@ void Case<T>::materialize(unsigned groups) {
         hipLaunchKernelGGL(k_dpdz<T>, grid_ij(d.nx, d.ny, nz, Ma), dim3(BLK), 0, stream,
                            (const T*)fld[LG_P], fld[LG_DPDZ], GEO(1, nz), (T)d.dz);
         fill_levels({{LG_DPDX, nz, BOGUS}, {LG_DPDY, nz, BOGUS}});
+    }
+    if (groups & SG_SIJ) {
+        calc_sij();
+        TauArgs<T> a{fld[LG_S11], fld[LG_S12], fld[LG_S13], fld[LG_S22], fld[LG_S23],
+                     fld[LG_S33], fld[LG_CS_OPT2], fld[LG_NU_T], nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, (T)(d.delta * d.delta), (T)d.prm.nu};
+        hipLaunchKernelGGL(k_nut<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream, a,
+                           GEO(1, d.nz));
     }
     stale &= ~groups;
     LG_HIP(hipGetLastError());

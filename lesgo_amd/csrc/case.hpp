@@ -108,7 +108,12 @@ struct Case {
     // makes the physical fields on demand (materialize).  mcold[m]: member
     // m's next step starts from its physical fields (its spectra do not
     // describe it); stale: the physical groups not yet made.
-    enum : unsigned { SG_UVW = 1, SG_RHS = 2, SG_RHSF = 4, SG_P = 8, SG_DIVT = 16, SG_ALL = 31 };
+    // SG_SIJ: S_ij and Nu_t, which a step without a dynamic-model update
+    // does not store (only the host reads them then; made from the
+    // derivative fields, which stay valid until the next step)
+    enum : unsigned {
+        SG_UVW = 1, SG_RHS = 2, SG_RHSF = 4, SG_P = 8, SG_DIVT = 16, SG_SIJ = 32, SG_ALL = 63
+    };
     bool spec_on = false;
     // the Lagrangian model's inverse row pass forms the tensor contractions
     // itself (k_lag_rows) where the grid allows it; LESGO_AMD_LAG_ROWS=0

@@ -263,6 +263,17 @@ __global__ void k_sgs_tau(TauArgs<T> a, Geo g) {
     }
 }
 
+// Nu_t alone (sgs_stag_util.f90:226-235), from stored S_ij: nut_at, as in
+// k_sgs_tau.
+template <typename T>
+__global__ void k_nut(TauArgs<T> a, Geo g) {
+    int m, k, j, i;
+    long mb;
+    if (!pt_of(g, g.nx, m, k, j, i, mb)) return;
+    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
+    a.Nu_t[o] = nut_at(a, o);
+}
+
 // The nine derivatives at one point of one level.
 template <typename T>
 struct Der9 {
@@ -319,12 +330,14 @@ __device__ __forceinline__ Sij6<T> sij_of(const Der9<T>& dm, const Der9<T>& d,
 // update in between.  A thread walks SIJ_L consecutive levels of one (i, j)
 // and keeps the level below's derivatives and S_ij in registers, so each
 // level's nine derivatives and Cs_opt2 are read once; it writes S_ij (as
-// k_calc_sij does), Nu_t and tau_ij (as k_sgs_tau does, same expressions).
-// Grid: x over (i, j), y over chunks of SIJ_L levels times members.
+// k_calc_sij does), Nu_t and tau_ij (as k_sgs_tau does, same expressions);
+// with store_s = 0 only tau_ij (S_ij and Nu_t are made later by k_calc_sij
+// and k_nut, to the same bits).  Grid: x over (i, j), y over chunks of
+// SIJ_L levels times members.
 constexpr int SIJ_L = 8;
 
 template <typename T>
-__global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g) {
+__global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g, int store_s) {
     // no FMA contraction: every pass computes S_ij, Nu_t and tau_ij with the
     // same roundings, so members batched with others take the same bits
     // whichever pass a step uses
@@ -350,13 +363,15 @@ __global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g) {
         const int k = kb + q;
         if (k <= ke) {
             const long o = base + (long)k * P;
-            sa.S11[o] = s0.s11;
-            sa.S12[o] = s0.s12;
-            sa.S13[o] = s0.s13;
-            sa.S22[o] = s0.s22;
-            sa.S23[o] = s0.s23;
-            sa.S33[o] = s0.s33;
-            a.Nu_t[o] = nk;
+            if (store_s) {
+                sa.S11[o] = s0.s11;
+                sa.S12[o] = s0.s12;
+                sa.S13[o] = s0.s13;
+                sa.S22[o] = s0.s22;
+                sa.S23[o] = s0.s23;
+                sa.S33[o] = s0.s33;
+                a.Nu_t[o] = nk;
+            }
             if (k == 1) {
                 const T c = (T)-2 * (nk + nu);
                 a.txx[o] = c * s0.s11;

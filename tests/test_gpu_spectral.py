@@ -1,7 +1,9 @@
 """The spectral-resident step against the all-physical one
 (LESGO_AMD_PHYSICAL_STEP=1): the same flow, and after materialisation the
 same physical fields the reference holds after a step (u, v, w with the
-disks' high-wavenumber forcing, RHS, RHS_f, p, dpdx, dpdy, dpdz, divt), on a
+disks' high-wavenumber forcing, RHS, RHS_f, p, dpdx, dpdy, dpdz, divt, and
+S_ij and Nu_t, which a step without a dynamic-model update leaves to be
+made on demand), on a
 fresh start with disks through the Euler step, the Smagorinsky phase and
 dynamic steps; and members leaving and joining a batch keep their spectra."""
 import os
@@ -15,7 +17,8 @@ from gpu_util import rel_l2
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("u", "v", "w", "p", "RHSx", "RHSy", "RHSz", "RHSx_f", "RHSy_f", "RHSz_f", "dpdx",
-          "dpdy", "dpdz", "divtx", "divty", "divtz", "fxa", "Cs_opt2", "F_LM")
+          "dpdy", "dpdz", "divtx", "divty", "divtz", "fxa", "Cs_opt2", "F_LM", "S11", "S13",
+          "S23", "S33", "Nu_t", "txx", "txz")
 
 
 def _case(physical, precision="fp64", members=1):
