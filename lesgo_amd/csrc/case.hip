@@ -514,7 +514,6 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
             c.src[nc] = src[j0 + i];
             c.src2[nc] = src2 ? src2[j0 + i] : nullptr;
             c.dst[nc] = tmpspec[nc];
-            c.dst2[nc] = nullptr;
             c.op[nc] = ops[j0 + i];
             rdst[nc] = dst[j0 + i];
             nce[nc] = ops[j0 + i] == OP_G1 ? g_nzc[0] : ops[j0 + i] == OP_G2 ? g_nzc[1] : 0;

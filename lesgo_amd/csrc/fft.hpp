@@ -415,10 +415,6 @@ struct ColArgs {
     const cplx<T>* src[MAX_JOBS];
     const cplx<T>* src2[MAX_JOBS];     // second input of OP_DIVXY
     cplx<T>* dst[MAX_JOBS];
-    // inverse MAP_ID jobs with op OP_NONE: when set, the job also writes
-    // i kx times its output here (i kx commutes with the y transform), and
-    // its input's ky = ny/2 row is taken as zero, as filt_da's is
-    cplx<T>* dst2[MAX_JOBS];
     int op[MAX_JOBS];
     long src_plane;
     long dst_plane;
@@ -502,9 +498,6 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
     const int plane = blockIdx.z - mem * a.nlev;
     const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
     cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
-    cplx<T>* dst2 = a.dst2[job] ? a.dst2[job] + mem * a.dst_mstride +
-                                      (long)(a.lev0_dst + plane) * a.dst_plane
-                                : nullptr;
     const int op = a.op[job];
     const int* perm = a.plan.perm;
     const int ny = a.ny;
@@ -523,7 +516,7 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
             v = (DIR > 0 && !in_yband(jb, ny, ykeep)) ? cplx<T>{(T)0, (T)0}
                                                       : src[(long)jb * a.ncols + c0 + c];
             if (DIR > 0 && a.map == MAP_ID) {
-                if ((dst2 || a.nyq0[job]) && jb == nyh) v = cplx<T>{(T)0, (T)0};
+                if (a.nyq0[job] && jb == nyh) v = cplx<T>{(T)0, (T)0};
                 if (op == OP_DIVXY) {
                     const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
                     v = divxy<T>(v, s2[(long)jb * a.ncols + c0 + c], c0 + c, jb, a);
@@ -556,10 +549,6 @@ __global__ void __launch_bounds__(256) k_cols(ColArgs<T> a) {
                 v = apply_op<T>(op, v, c0 + c, j, a);
             }
             if (DIR > 0 || in_yband(j, ny, ykeep)) dst[(long)j * a.ncols + c0 + c] = v;
-            if (DIR > 0 && dst2) {
-                const T kx = a.kxs * (T)(c0 + c);
-                dst2[(long)j * a.ncols + c0 + c] = cplx<T>{-v.y * kx, v.x * kx};
-            }
         }
     }
 }

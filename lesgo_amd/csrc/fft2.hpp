@@ -298,9 +298,6 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
     const int plane = blockIdx.z - mem * a.nlev;
     const cplx<T>* src = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
     cplx<T>* dst = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
-    cplx<T>* dst2 = a.dst2[job] ? a.dst2[job] + mem * a.dst_mstride +
-                                      (long)(a.lev0_dst + plane) * a.dst_plane
-                                : nullptr;
     const int op = a.op[job];
     const int ny = a.ny;
     const int nyh = ny / 2;
@@ -322,7 +319,7 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                 v = (DIR > 0 && !in_yband(jb, a.ny, ykeep)) ? cplx<T>{(T)0, (T)0}
                                                               : gld(src + (long)jb * a.ncols + c0 + c);
                 if (DIR > 0 && a.map == MAP_ID) {
-                    if ((dst2 || a.nyq0[job]) && jb == nyh) v = cplx<T>{(T)0, (T)0};
+                    if (a.nyq0[job] && jb == nyh) v = cplx<T>{(T)0, (T)0};
                     if (op == OP_DIVXY) {
                         const cplx<T>* s2 = a.src2[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
                         v = divxy<T>(v, gld(s2 + (long)jb * a.ncols + c0 + c), c0 + c, jb, a);
@@ -365,10 +362,6 @@ __global__ void __launch_bounds__(256) k_cols_v2(ColArgs<T> a) {
                     v = apply_op<T>(op, v, c0 + c, jb, a);
                 }
                 if (DIR > 0 || in_yband(jb, ny, ykeep)) gst(dst + (long)jb * a.ncols + c0 + c, v);
-                if (DIR > 0 && dst2) {
-                    const T kx = a.kxs * (T)(c0 + c);
-                    gst(dst2 + (long)jb * a.ncols + c0 + c, cplx<T>{-v.y * kx, v.x * kx});
-                }
             }
         }
     }

@@ -274,7 +274,7 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
         const int band = c.ykeep[i] > 0 ? std::min(2 * c.ykeep[i] - 1, ny_in) : 0;
         const double rin = (DIR > 0 && band) ? band : ny_in;
         const double rout = (DIR < 0 && band) ? band : N;
-        cols_rw += (double)nzc * (rin + rout + (c.dst2[i] ? N : 0));
+        cols_rw += (double)nzc * (rin + rout);
     }
     dim3 grid(nj, (cmax + c.TC - 1) / c.TC, nlev * M);
     const double bytes = (double)nlev * M * sizeof(cplx<T>) * cols_rw;
