@@ -764,6 +764,20 @@ void Case<T>::ddz_w(int f, int dfdz) {
     fill_levels({{dfdz, 0, BOGUS}, {dfdz, d.nz, BOGUS}});
 }
 
+template <typename T>
+void Case<T>::ddz_uvw() {
+    if (d.nz >= 2) {
+        const int nch = (d.nz - 1 + DDZ_L - 1) / DDZ_L;
+        hipLaunchKernelGGL(k_ddz_uvw<T>, grid_ij(d.nx, d.ny, nch, Ma), dim3(BLK), 0, stream,
+                           (const T*)fld[LG_U], (const T*)fld[LG_V], (const T*)fld[LG_W],
+                           fld[LG_DUDZ], fld[LG_DVDZ], fld[LG_DWDZ], GEO(1, nch),
+                           (T)(1.0 / d.dz));
+    }
+    fill_levels({{LG_DUDZ, 0, BOGUS}, {LG_DUDZ, 1, BOGUS}, {LG_DUDZ, d.nz, BOGUS},
+                 {LG_DVDZ, 0, BOGUS}, {LG_DVDZ, 1, BOGUS}, {LG_DVDZ, d.nz, BOGUS},
+                 {LG_DWDZ, 0, BOGUS}, {LG_DWDZ, d.nz, BOGUS}});
+}
+
 // One plane per member (member stride = field) filtered by op.
 template <typename T>
 void Case<T>::test_filter_plane(const T* in, T* out, int op) {
@@ -1152,8 +1166,9 @@ void Case<T>::convec(bool reuse_spectra, bool spectral_out) {
     }
     pad_inv_big(6, ps, 1, nl);
     ProdArgs<T> pa{big[0], big[1], big[2], big[3], big[4], big[5], big[6], big[7], big[8],
-                   (T)(1.0 / (d.nx2 * d.ny2))};
-    hipLaunchKernelGGL(k_products<T>, grid_ij(d.nx2, d.ny2, nl, Ma), dim3(BLK), 0, stream, pa,
+                   1.0 / (d.nx2 * d.ny2)};
+    const int pch = (d.nz + PROD_L - 1) / PROD_L;
+    hipLaunchKernelGGL(k_products<T>, grid_ij(d.nx2, d.ny2, pch, Ma), dim3(BLK), 0, stream, pa,
                        d.nx2, d.ny2, d.ld_big, d.nz, mbp);
     const T* cs[3] = {big[6], big[7], big[8]};
     cplx<T>* cd[3] = {spec[0], spec[1], spec[2]};
@@ -1270,9 +1285,7 @@ template <typename T>
 void Case<T>::step_body() {
     clock_begin();
     filt_da_uvw();
-    ddz_uv(LG_U, LG_DUDZ);
-    ddz_uv(LG_V, LG_DVDZ);
-    ddz_w(LG_W, LG_DWDZ);
+    ddz_uvw();
     wallstress();
     sgs_stag();
     divstress_uvw();
@@ -1427,9 +1440,7 @@ void Case<T>::step_body_spec(bool cold) {
         }
     }
     filt_da_uvw(false);
-    ddz_uv(LG_U, LG_DUDZ);
-    ddz_uv(LG_V, LG_DVDZ);
-    ddz_w(LG_W, LG_DWDZ);
+    ddz_uvw();
     wallstress();
     sgs_stag();
     divstress_spec();

@@ -206,6 +206,7 @@ struct Case {
     void ddxy(int f, int dfdx, int dfdy);
     void ddz_uv(int f, int dfdz);
     void ddz_w(int f, int dfdz);
+    void ddz_uvw();        // ddz_uv(u), ddz_uv(v), ddz_w(w): one pass, the same values
     void test_filter_plane(const T* in, T* out, int op);
     void wallstress();
     void calc_sij();

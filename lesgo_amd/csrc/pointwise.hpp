@@ -92,6 +92,44 @@ __global__ void k_ddz_w(const T* f, T* out, Geo g, T rdz) {
     out[o] = rdz * (f[o + g.plane] - f[o]);
 }
 
+// ddz_uv of u and v and ddz_w of w in one pass (derivatives.f90:214-311),
+// the same expressions as k_ddz_uv / k_ddz_w: each thread walks DDZ_L levels
+// of one (i, j) with the level below in registers, so every plane of u, v, w
+// is read once.  Written: dudz, dvdz on levels 2..nz-1, dwdz on 1..nz-1; the
+// other levels the two kernels compute are overwritten afterwards (BOGUS at
+// 0, 1, nz of dudz / dvdz and 0, nz of dwdz) and are set by the caller.
+constexpr int DDZ_L = 8;
+template <typename T>
+__global__ void k_ddz_uvw(const T* u, const T* v, const T* w, T* dudz, T* dvdz, T* dwdz, Geo g,
+                          T rdz) {
+    int j, i;
+    if (!ij_of(g.nx, g.ny, j, i)) return;
+    const int nch = (g.nz - 1 + DDZ_L - 1) / DDZ_L;
+    const int m = blockIdx.y / nch;
+    const int kb = 1 + (int)(blockIdx.y - m * nch) * DDZ_L;
+    const int ke = min(kb + DDZ_L - 1, g.nz - 1);
+    const long P = g.plane;
+    const long base = (long)m * g.mstride + (long)j * g.ld + i;
+    T ub = u[base + (long)(kb - 1) * P], vb = v[base + (long)(kb - 1) * P];
+    T wk = w[base + (long)kb * P];
+#pragma unroll
+    for (int q = 0; q < DDZ_L; ++q) {
+        const int k = kb + q;
+        if (k <= ke) {
+            const long o = base + (long)k * P;
+            const T uk = u[o], vk = v[o], wp = w[o + P];
+            if (k >= 2) {
+                dudz[o] = rdz * (uk - ub);
+                dvdz[o] = rdz * (vk - vb);
+            }
+            dwdz[o] = rdz * (wp - wk);
+            ub = uk;
+            vb = vk;
+            wk = wp;
+        }
+    }
+}
+
 // ws_equilibrium_lbc (wallstress.f90:171-219) on the k=1 plane; u1f, v1f
 // are the test-filtered k=1 planes (member stride fmstride), the other
 // pointers field bases; denom[m] = ln(0.5 dz / z0_m).
@@ -534,40 +572,74 @@ template <typename T>
 struct ProdArgs {
     const T *ub, *vb, *wb, *o1, *o2, *o3;
     T *cx, *cy, *cz;
-    T c;
+    double c;
 };
 
+// Levels 1..nz; each thread walks PROD_L levels of one (i, j) with ub, vb of
+// the level below and wb, o1, o2 of the level above in registers, so every
+// plane is read once.  The products are formed in FP64 from the stored
+// values and rounded once to T: their sums cancel where the flow has sharp
+// peaks, and float arithmetic there would add errors the velocity field
+// itself does not carry.  No FMA contraction: the rounding does not depend
+// on the compiler's fusion choices.
+constexpr int PROD_L = 8;
 template <typename T>
 __global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz, long bmstride) {
+#pragma clang fp contract(off)
     int j, i;
     if (!ij_of(nx2, ny2, j, i)) return;
-    const int m = blockIdx.y / nz;
-    const int k = blockIdx.y - m * nz + 1;
+    const int nch = (nz + PROD_L - 1) / PROD_L;
+    const int m = blockIdx.y / nch;
+    const int kb = 1 + (int)(blockIdx.y - m * nch) * PROD_L;
+    const int ke = min(kb + PROD_L - 1, nz);
     const long P = (long)ny2 * ldb;
-    const long o = (long)m * bmstride + (long)k * P + (long)j * ldb + i;
-    const T h = (T)0.5;
-    const T c = a.c;
-    if (k <= nz - 1) {
-        T x, y;
-        if (k == 1) {
-            x = c * (a.vb[o] * (-a.o3[o]) + h * a.wb[o + P] * a.o2[o + P]);
-            y = c * (a.ub[o] * a.o3[o] + h * a.wb[o + P] * (-a.o1[o + P]));
-        } else if (k == nz - 1) {
-            x = c * (a.vb[o] * (-a.o3[o]) + h * a.wb[o] * a.o2[o]);
-            y = c * (a.ub[o] * a.o3[o] + h * a.wb[o] * (-a.o1[o]));
-        } else {
-            x = c * (a.vb[o] * (-a.o3[o]) + h * (a.wb[o + P] * a.o2[o + P] + a.wb[o] * a.o2[o]));
-            y = c * (a.ub[o] * a.o3[o] +
-                     h * (a.wb[o + P] * (-a.o1[o + P]) + a.wb[o] * (-a.o1[o])));
+    const long base = (long)m * bmstride + (long)j * ldb + i;
+    const double h = 0.5;
+    const double c = a.c;
+    // ub, vb at k - 1; wb, o1, o2 at k
+    double ubm = 0.0, vbm = 0.0;
+    if (kb >= 2) {
+        ubm = a.ub[base + (long)(kb - 1) * P];
+        vbm = a.vb[base + (long)(kb - 1) * P];
+    }
+    double wb0 = a.wb[base + (long)kb * P], o10 = a.o1[base + (long)kb * P],
+           o20 = a.o2[base + (long)kb * P];
+#pragma unroll
+    for (int q = 0; q < PROD_L; ++q) {
+        const int k = kb + q;
+        if (k <= ke) {
+            const long o = base + (long)k * P;
+            const double ub0 = a.ub[o], vb0 = a.vb[o];
+            double wb1 = 0.0, o11 = 0.0, o21 = 0.0;
+            if (k <= nz - 1) {
+                wb1 = a.wb[o + P];
+                o11 = a.o1[o + P];
+                o21 = a.o2[o + P];
+                const double o30 = a.o3[o];
+                double x, y;
+                if (k == 1) {
+                    x = c * (vb0 * (-o30) + h * wb1 * o21);
+                    y = c * (ub0 * o30 + h * wb1 * (-o11));
+                } else if (k == nz - 1) {
+                    x = c * (vb0 * (-o30) + h * wb0 * o20);
+                    y = c * (ub0 * o30 + h * wb0 * (-o10));
+                } else {
+                    x = c * (vb0 * (-o30) + h * (wb1 * o21 + wb0 * o20));
+                    y = c * (ub0 * o30 + h * (wb1 * (-o11) + wb0 * (-o10)));
+                }
+                a.cx[o] = (T)x;
+                a.cy[o] = (T)y;
+            }
+            double z = 0.0;
+            if (k >= 2 && k <= nz - 1) z = (c * h) * ((ub0 + ubm) * (-o20) + (vb0 + vbm) * o10);
+            a.cz[o] = (T)z;
+            ubm = ub0;
+            vbm = vb0;
+            wb0 = wb1;
+            o10 = o11;
+            o20 = o21;
         }
-        a.cx[o] = x;
-        a.cy[o] = y;
     }
-    T z = (T)0;
-    if (k >= 2 && k <= nz - 1) {
-        z = (c * h) * ((a.ub[o] + a.ub[o - P]) * (-a.o2[o]) + (a.vb[o] + a.vb[o - P]) * a.o1[o]);
-    }
-    a.cz[o] = z;
 }
 
 // RHS assembly, forcing and AB2 (main.f90:211-214, 228-232, 264-266, 287-296);

@@ -139,11 +139,18 @@ def test_cfl_statistics_weights():
 
 def test_cfl_mixed_precision_dt_close():
     """Mixed precision takes nearly the FP64 dt sequence.  dt follows the
-    velocity maximum at one point: the two runs agree to ~1e-7 until the
-    flow forms a sharp peak (step 10 here, dt falls fourfold), after which
-    that one point differs by ~1e-3 (measured 0.8-1.3e-3 over steps 10-40)
-    while the field as a whole differs by ~5e-5.  Bounds: dt within 3e-3,
-    u within 1e-4 relative L2, over 40 steps."""
+    velocity maximum at one point: the two runs agree to ~3e-7 until the flow
+    forms a sharp peak (step 9 here; dt then falls fourfold).  The roundings
+    made at the peak are amplified about a hundredfold, after which that one
+    point differs by ~2e-3 while the field as a whole differs by ~1e-4.  How
+    large depends on the rounding pattern, not on the size of each rounding:
+    three FMA-fusion patterns of the float 3/2-grid products gave 1.3e-3,
+    3.2e-3 and 5.1e-3 in dt (u 0.5e-4 to 2.2e-4); the products formed in FP64
+    give 2.4e-3 (u 1.1e-4).  An FP64 run started from float-rounded fields
+    stays within 4e-5 in dt and 2e-6 in u, so the difference is rounding made
+    every step, amplified at the peak.  Bounds over 40 steps: dt within 1e-6
+    while dt stays within 5 % of its first value, within 1e-2 after; u within
+    5e-4 relative L2."""
     p = o.Params(32, 32, 16, use_cfl_dt=True, cfl=0.05)
     s = o.initial_condition(p, 1)
     K = np.s_[1:p.nz, :, :p.nx]
@@ -152,16 +159,26 @@ def test_cfl_mixed_precision_dt_close():
             for n, arr in s.f.items():
                 c.upload(n, arr)
             c.set_counters(0, 0)
-        worst = worst_u = 0.0
+        worst_pre = worst = worst_u = 0.0
+        dt1 = None
+        npre = 0
         for _ in range(40):
             a.step(1)
             b.step(1)
             da, db = a.clock()["dt"], b.clock()["dt"]
-            worst = max(worst, abs(da - db) / da)
+            dt1 = dt1 or da
+            e = abs(da - db) / da
+            if abs(da - dt1) <= 0.05 * dt1:
+                worst_pre = max(worst_pre, e)
+                npre += 1
+            worst = max(worst, e)
             worst_u = max(worst_u, rel_l2(b.download("u")[K], a.download("u")[K]))
-        print(f"  max relative dt difference {worst:.2e}, u {worst_u:.2e}", flush=True)
-        assert worst <= 3e-3
-        assert worst_u <= 1e-4
+        print(f"  max relative dt difference {worst_pre:.2e} over {npre} steps before the peak,"
+              f" {worst:.2e} over all; u {worst_u:.2e}", flush=True)
+        assert npre >= 5
+        assert worst_pre <= 1e-6
+        assert worst <= 1e-2
+        assert worst_u <= 5e-4
 
 
 def test_fortran_driver_cfl_subsystem_path(tmp_path):
