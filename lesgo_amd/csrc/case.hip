@@ -768,7 +768,7 @@ template <typename T>
 void Case<T>::ddz_uvw() {
     if (d.nz >= 2) {
         const int nch = (d.nz - 1 + DDZ_L - 1) / DDZ_L;
-        hipLaunchKernelGGL(k_ddz_uvw<T>, grid_ij(d.nx, d.ny, nch, Ma), dim3(BLK), 0, stream,
+        hipLaunchKernelGGL(k_ddz_uvw<T>, grid_ij((d.nx + 1) / 2, d.ny, nch, Ma), dim3(BLK), 0, stream,
                            (const T*)fld[LG_U], (const T*)fld[LG_V], (const T*)fld[LG_W],
                            fld[LG_DUDZ], fld[LG_DVDZ], fld[LG_DWDZ], GEO(1, nch),
                            (T)(1.0 / d.dz));
@@ -1168,7 +1168,7 @@ void Case<T>::convec(bool reuse_spectra, bool spectral_out) {
     ProdArgs<T> pa{big[0], big[1], big[2], big[3], big[4], big[5], big[6], big[7], big[8],
                    1.0 / (d.nx2 * d.ny2)};
     const int pch = (d.nz + PROD_L - 1) / PROD_L;
-    hipLaunchKernelGGL(k_products<T>, grid_ij(d.nx2, d.ny2, pch, Ma), dim3(BLK), 0, stream, pa,
+    hipLaunchKernelGGL(k_products<T>, grid_ij((d.nx2 + 1) / 2, d.ny2, pch, Ma), dim3(BLK), 0, stream, pa,
                        d.nx2, d.ny2, d.ld_big, d.nz, mbp);
     const T* cs[3] = {big[6], big[7], big[8]};
     cplx<T>* cd[3] = {spec[0], spec[1], spec[2]};

@@ -92,40 +92,74 @@ __global__ void k_ddz_w(const T* f, T* out, Geo g, T rdz) {
     out[o] = rdz * (f[o + g.plane] - f[o]);
 }
 
+// Two consecutive values of a row (i even; rows and fields hold an even
+// number of values, so the pair is aligned to its size).
+template <typename T> struct Pair2;
+template <> struct Pair2<float> { using type = float2; };
+template <> struct Pair2<double> { using type = double2; };
+
+template <typename T>
+__device__ __forceinline__ void ld_pair(const T* p, T& a, T& b) {
+    const auto v = *reinterpret_cast<const typename Pair2<T>::type*>(p);
+    a = v.x;
+    b = v.y;
+}
+
+// both values, or the first alone when the second is past the row's end
+template <typename T>
+__device__ __forceinline__ void st_pair(T* p, T a, T b, bool both) {
+    if (both) {
+        typename Pair2<T>::type v;
+        v.x = a;
+        v.y = b;
+        *reinterpret_cast<typename Pair2<T>::type*>(p) = v;
+    } else {
+        p[0] = a;
+    }
+}
+
 // ddz_uv of u and v and ddz_w of w in one pass (derivatives.f90:214-311),
-// the same expressions as k_ddz_uv / k_ddz_w: each thread walks DDZ_L levels
-// of one (i, j) with the level below in registers, so every plane of u, v, w
-// is read once.  Written: dudz, dvdz on levels 2..nz-1, dwdz on 1..nz-1; the
-// other levels the two kernels compute are overwritten afterwards (BOGUS at
-// 0, 1, nz of dudz / dvdz and 0, nz of dwdz) and are set by the caller.
+// the same expressions as k_ddz_uv / k_ddz_w: each thread takes two
+// consecutive points (i, i + 1) of a row and DDZ_L levels of them into
+// registers, so every plane of u, v, w is read once, in pairs.  Written:
+// dudz, dvdz on levels 2..nz-1, dwdz on 1..nz-1; the other levels the two
+// kernels compute are overwritten afterwards (BOGUS at 0, 1, nz of dudz /
+// dvdz and 0, nz of dwdz) and are set by the caller.
 constexpr int DDZ_L = 8;
 template <typename T>
 __global__ void k_ddz_uvw(const T* u, const T* v, const T* w, T* dudz, T* dvdz, T* dwdz, Geo g,
                           T rdz) {
-    int j, i;
-    if (!ij_of(g.nx, g.ny, j, i)) return;
+    int j, ip;
+    if (!ij_of((g.nx + 1) / 2, g.ny, j, ip)) return;
+    const int i = 2 * ip;
+    const bool both = i + 1 < g.nx;
     const int nch = (g.nz - 1 + DDZ_L - 1) / DDZ_L;
     const int m = blockIdx.y / nch;
     const int kb = 1 + (int)(blockIdx.y - m * nch) * DDZ_L;
     const int ke = min(kb + DDZ_L - 1, g.nz - 1);
     const long P = g.plane;
     const long base = (long)m * g.mstride + (long)j * g.ld + i;
-    T ub = u[base + (long)(kb - 1) * P], vb = v[base + (long)(kb - 1) * P];
-    T wk = w[base + (long)kb * P];
+    // every level of the chunk is loaded before the first store (level
+    // indices clamped into the chunk, so no load depends on a branch)
+    T U[2][DDZ_L + 1], V[2][DDZ_L + 1], W[2][DDZ_L + 1];
+#pragma unroll
+    for (int q = 0; q <= DDZ_L; ++q) {
+        const long ouv = base + (long)min(kb - 1 + q, ke) * P;
+        const long ow = base + (long)min(kb + q, ke + 1) * P;
+        ld_pair(u + ouv, U[0][q], U[1][q]);
+        ld_pair(v + ouv, V[0][q], V[1][q]);
+        ld_pair(w + ow, W[0][q], W[1][q]);
+    }
 #pragma unroll
     for (int q = 0; q < DDZ_L; ++q) {
         const int k = kb + q;
         if (k <= ke) {
             const long o = base + (long)k * P;
-            const T uk = u[o], vk = v[o], wp = w[o + P];
             if (k >= 2) {
-                dudz[o] = rdz * (uk - ub);
-                dvdz[o] = rdz * (vk - vb);
+                st_pair(dudz + o, rdz * (U[0][q + 1] - U[0][q]), rdz * (U[1][q + 1] - U[1][q]), both);
+                st_pair(dvdz + o, rdz * (V[0][q + 1] - V[0][q]), rdz * (V[1][q + 1] - V[1][q]), both);
             }
-            dwdz[o] = rdz * (wp - wk);
-            ub = uk;
-            vb = vk;
-            wk = wp;
+            st_pair(dwdz + o, rdz * (W[0][q + 1] - W[0][q]), rdz * (W[1][q + 1] - W[1][q]), both);
         }
     }
 }
@@ -396,11 +430,21 @@ __global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g, int store_s) {
     if (kb < nz) dp = der_at(sa, base + (long)(kb + 1) * P);
     Sij6<T> s0 = sij_of(dm, d, dp, kb, nz);
     T nk = smag_norm(s0.s11, s0.s12, s0.s13, s0.s22, s0.s23, s0.s33) * a.Cs[base + (long)kb * P] * a.l2;
+    // level k + 1's derivatives and Cs_opt2 are in registers when level k is
+    // worked on; level k + 2's are loaded before level k's stores
+    Der9<T> d1 = dp;
+    T cs1 = kb < nz ? a.Cs[base + (long)(kb + 1) * P] : (T)0;
 #pragma unroll
     for (int q = 0; q < SIJ_L; ++q) {
         const int k = kb + q;
         if (k <= ke) {
             const long o = base + (long)k * P;
+            Der9<T> d2{};
+            T cs2 = (T)0;
+            if (k + 2 <= min(ke + 1, nz)) {
+                d2 = der_at(sa, o + 2 * P);
+                cs2 = a.Cs[o + 2 * P];
+            }
             if (store_s) {
                 sa.S11[o] = s0.s11;
                 sa.S12[o] = s0.s12;
@@ -420,10 +464,10 @@ __global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g, int store_s) {
             if (k <= nz - 1) {
                 // S_ij and Nu_t at k + 1 from the derivatives at k and k + 1
                 // (k + 2 is not read: k + 1 >= 2)
-                const Der9<T> dnext = (k == kb) ? dp : der_at(sa, o + P);
+                const Der9<T> dnext = d1;
                 const Sij6<T> s1 = sij_of(d, dnext, dnext, k + 1, nz);
                 const T nk1 = smag_norm(s1.s11, s1.s12, s1.s13, s1.s22, s1.s23, s1.s33) *
-                              a.Cs[o + P] * a.l2;
+                              cs1 * a.l2;
                 if (k == nz - 1 && k != 1) {
                     const T c = (T)0.5 * (nk + nk1) + nu;
                     const T c2 = (T)2 * (nk + nu);
@@ -449,6 +493,8 @@ __global__ void k_sij_tau(SijArgs<T> sa, TauArgs<T> a, Geo g, int store_s) {
                 s0 = s1;
                 nk = nk1;
             }
+            d1 = d2;
+            cs1 = cs2;
         }
     }
 }
@@ -575,19 +621,22 @@ struct ProdArgs {
     double c;
 };
 
-// Levels 1..nz; each thread walks PROD_L levels of one (i, j) with ub, vb of
-// the level below and wb, o1, o2 of the level above in registers, so every
-// plane is read once.  The products are formed in FP64 from the stored
-// values and rounded once to T: their sums cancel where the flow has sharp
-// peaks, and float arithmetic there would add errors the velocity field
-// itself does not carry.  No FMA contraction: the rounding does not depend
-// on the compiler's fusion choices.
+// Levels 1..nz; each thread takes two consecutive points (i, i + 1) of one
+// row and PROD_L levels of them (and the levels either side) into
+// registers, so every plane is read once, in pairs.  The products are
+// formed in FP64 from the stored values and rounded once to T: their sums
+// cancel where the flow has sharp peaks, and float arithmetic there would
+// add errors the velocity field itself does not carry.  No FMA contraction:
+// the rounding does not depend on the compiler's fusion choices.
 constexpr int PROD_L = 8;
 template <typename T>
-__global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz, long bmstride) {
+__global__ void __launch_bounds__(256) k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz, long bmstride) {
 #pragma clang fp contract(off)
-    int j, i;
-    if (!ij_of(nx2, ny2, j, i)) return;
+    int j, ip;
+    const int nxp = (nx2 + 1) / 2;
+    if (!ij_of(nxp, ny2, j, ip)) return;
+    const int i = 2 * ip;
+    const bool both = i + 1 < nx2;
     const int nch = (nz + PROD_L - 1) / PROD_L;
     const int m = blockIdx.y / nch;
     const int kb = 1 + (int)(blockIdx.y - m * nch) * PROD_L;
@@ -596,48 +645,79 @@ __global__ void k_products(ProdArgs<T> a, int nx2, int ny2, int ldb, int nz, lon
     const long base = (long)m * bmstride + (long)j * ldb + i;
     const double h = 0.5;
     const double c = a.c;
-    // ub, vb at k - 1; wb, o1, o2 at k
-    double ubm = 0.0, vbm = 0.0;
-    if (kb >= 2) {
-        ubm = a.ub[base + (long)(kb - 1) * P];
-        vbm = a.vb[base + (long)(kb - 1) * P];
+    // every level of the chunk is loaded before the first store (level
+    // indices clamped into range, so no load depends on a branch): ub, vb,
+    // o3 at kb + q (q < PROD_L) and kb - 1; wb, o1, o2 at kb + q (q <= PROD_L)
+    T U[2][PROD_L], V[2][PROD_L], O3[2][PROD_L], W[2][PROD_L + 1], O1[2][PROD_L + 1],
+        O2[2][PROD_L + 1];
+    T ubm0[2], vbm0[2];
+    {
+        const long om = base + (long)max(kb - 1, 0) * P;
+        ld_pair(a.ub + om, ubm0[0], ubm0[1]);
+        ld_pair(a.vb + om, vbm0[0], vbm0[1]);
     }
-    double wb0 = a.wb[base + (long)kb * P], o10 = a.o1[base + (long)kb * P],
-           o20 = a.o2[base + (long)kb * P];
+#pragma unroll
+    for (int q = 0; q <= PROD_L; ++q) {
+        const long ow = base + (long)min(kb + q, nz) * P;
+        ld_pair(a.wb + ow, W[0][q], W[1][q]);
+        ld_pair(a.o1 + ow, O1[0][q], O1[1][q]);
+        ld_pair(a.o2 + ow, O2[0][q], O2[1][q]);
+        if (q < PROD_L) {
+            const long ou = base + (long)min(kb + q, ke) * P;
+            ld_pair(a.ub + ou, U[0][q], U[1][q]);
+            ld_pair(a.vb + ou, V[0][q], V[1][q]);
+            ld_pair(a.o3 + ou, O3[0][q], O3[1][q]);
+        }
+    }
+    // per point: ub, vb at k - 1; wb, o1, o2 at k
+    double ubm[2], vbm[2], wb0[2], o10[2], o20[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        ubm[e] = kb >= 2 ? (double)ubm0[e] : 0.0;
+        vbm[e] = kb >= 2 ? (double)vbm0[e] : 0.0;
+        wb0[e] = W[e][0];
+        o10[e] = O1[e][0];
+        o20[e] = O2[e][0];
+    }
 #pragma unroll
     for (int q = 0; q < PROD_L; ++q) {
         const int k = kb + q;
         if (k <= ke) {
             const long o = base + (long)k * P;
-            const double ub0 = a.ub[o], vb0 = a.vb[o];
-            double wb1 = 0.0, o11 = 0.0, o21 = 0.0;
-            if (k <= nz - 1) {
-                wb1 = a.wb[o + P];
-                o11 = a.o1[o + P];
-                o21 = a.o2[o + P];
-                const double o30 = a.o3[o];
-                double x, y;
-                if (k == 1) {
-                    x = c * (vb0 * (-o30) + h * wb1 * o21);
-                    y = c * (ub0 * o30 + h * wb1 * (-o11));
-                } else if (k == nz - 1) {
-                    x = c * (vb0 * (-o30) + h * wb0 * o20);
-                    y = c * (ub0 * o30 + h * wb0 * (-o10));
-                } else {
-                    x = c * (vb0 * (-o30) + h * (wb1 * o21 + wb0 * o20));
-                    y = c * (ub0 * o30 + h * (wb1 * (-o11) + wb0 * (-o10)));
+            double x[2] = {0.0, 0.0}, y[2] = {0.0, 0.0}, z[2] = {0.0, 0.0};
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const double ub0 = U[e][q], vb0 = V[e][q];
+                double wb1 = 0.0, o11 = 0.0, o21 = 0.0;
+                if (k <= nz - 1) {
+                    wb1 = W[e][q + 1];
+                    o11 = O1[e][q + 1];
+                    o21 = O2[e][q + 1];
+                    const double o30 = O3[e][q];
+                    if (k == 1) {
+                        x[e] = c * (vb0 * (-o30) + h * wb1 * o21);
+                        y[e] = c * (ub0 * o30 + h * wb1 * (-o11));
+                    } else if (k == nz - 1) {
+                        x[e] = c * (vb0 * (-o30) + h * wb0[e] * o20[e]);
+                        y[e] = c * (ub0 * o30 + h * wb0[e] * (-o10[e]));
+                    } else {
+                        x[e] = c * (vb0 * (-o30) + h * (wb1 * o21 + wb0[e] * o20[e]));
+                        y[e] = c * (ub0 * o30 + h * (wb1 * (-o11) + wb0[e] * (-o10[e])));
+                    }
                 }
-                a.cx[o] = (T)x;
-                a.cy[o] = (T)y;
+                if (k >= 2 && k <= nz - 1)
+                    z[e] = (c * h) * ((ub0 + ubm[e]) * (-o20[e]) + (vb0 + vbm[e]) * o10[e]);
+                ubm[e] = ub0;
+                vbm[e] = vb0;
+                wb0[e] = wb1;
+                o10[e] = o11;
+                o20[e] = o21;
             }
-            double z = 0.0;
-            if (k >= 2 && k <= nz - 1) z = (c * h) * ((ub0 + ubm) * (-o20) + (vb0 + vbm) * o10);
-            a.cz[o] = (T)z;
-            ubm = ub0;
-            vbm = vb0;
-            wb0 = wb1;
-            o10 = o11;
-            o20 = o21;
+            if (k <= nz - 1) {
+                st_pair(a.cx + o, (T)x[0], (T)x[1], both);
+                st_pair(a.cy + o, (T)y[0], (T)y[1], both);
+            }
+            st_pair(a.cz + o, (T)z[0], (T)z[1], both);
         }
     }
 }
