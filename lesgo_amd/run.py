@@ -12,14 +12,16 @@ reading and writing the reference's files in ``path``:
   every ``wbase`` steps (main.f90:395-436); ``output/check_ke.dat`` every
   ``nenergy`` steps (io.f90:99-141); ``vel.out``, ``grid.out``,
   ``total_time.dat`` every ``checkpoint_nskip`` steps with
-  ``checkpoint_data`` and at the end (io.f90:1173-1267); ``output/vel.<jt>.c0.bin``
-  and ``output/pres.<jt>.c0.bin`` with ``domain_calc`` (io.f90:786-884, one
-  process = coord 0); the time averages ``output/{veluv,velw,vel2,tau,pres,
+  ``checkpoint_data`` and at the end (io.f90:1173-1267); ``output/vel.<jt>.c0.bin``,
+  ``output/vort.<jt>.c0.bin`` and ``output/pres.<jt>.c0.bin`` with
+  ``domain_calc`` (io.f90:786-884, one process = coord 0); point and x/y/z
+  plane samples with ``point_calc`` / ``*plane_calc`` (io.f90:585-676, the
+  writers of ``lesgo_amd.samples``); the time averages ``output/{veluv,velw,vel2,tau,pres,
   force,cs_opt2,vort}_avg.bin`` and ``output/rs.bin`` with ``tavg_calc``
   (time_average.f90:322-655); ``turbine/u_d_T.dat`` (turbines.f90:657-679).
 
 A new run starts from the seeded log-law field (``--seed``) in place of the
-reference's /dev/urandom ic_les.  Point and plane samples are not written.
+reference's /dev/urandom ic_les.
 Steps replay captured graphs unless a dynamic turbine schedule changes the
 disks between steps; dynamic schedules need a static dt (their time is the
 dimensional time at the end of the step, which with use_cfl_dt is only known
@@ -38,6 +40,7 @@ from .case import Case
 from .checkpoint import read_checkpoint, write_checkpoint
 from .conf import (ConfError, LesgoConf, linear_interp, place_turbines, read_conf,
                    read_control_files, write_param_out)
+from .samples import Sampler, vorticity
 
 
 def _write_direct(fname, arrays, nz):
@@ -74,6 +77,7 @@ def write_tavg(case: Case, c: LesgoConf, member: int = 0):
 
 def write_domain(case: Case, c: LesgoConf, jt: int, member: int = 0):
     """io.f90:786-884 binary branch: u, v, w interpolated to the uv grid;
+    the vorticity on the w grid from the derivative fields (io.f90:823-858);
     p - (u^2 + v^2 + w_uv^2)/2 (levels 1..nz, i < nx)."""
     nz, nx = case.nz, c.nx
     u = case.download("u", member)[:, :, :nx]
@@ -84,9 +88,22 @@ def write_domain(case: Case, c: LesgoConf, jt: int, member: int = 0):
     w_uv[1:nz] = 0.5 * (w[2:nz + 1] + w[1:nz])
     w_uv[nz] = w_uv[nz - 1]
     pr = p - 0.5 * (u * u + w_uv * w_uv + v * v)
+    der = {n: case.download(n, member)[:, :, :nx]
+           for n in ("dudy", "dudz", "dvdx", "dvdz", "dwdx", "dwdy")}
     out = os.path.join(c.path, "output")
     _write_direct(os.path.join(out, f"vel.{jt}.c0.bin"), [u, v, w_uv], nz)
+    _write_direct(os.path.join(out, f"vort.{jt}.c0.bin"), list(vorticity(der, nz)), nz)
     _write_direct(os.path.join(out, f"pres.{jt}.c0.bin"), [pr], nz)
+
+
+def write_samples(case: Case, sampler: Sampler, jt: int, path: str, member: int = 0):
+    """The point and plane samples due at step jt (output_loop, io.f90:585-676)."""
+    due = sampler.due(jt)
+    if not any(due.values()):
+        return
+    u, v, w = (case.download(n, member) for n in ("u", "v", "w"))
+    sampler.write(u, v, w, jt, case.clock(member)["total_time"], os.path.join(path, "output"),
+                  due)
 
 
 class Runner:
@@ -108,6 +125,7 @@ class Runner:
             raise ConfError("dynamic turbine schedules need use_cfl_dt = .false. here")
         self.case = Case(self.cfg)
         os.makedirs(os.path.join(self.path, "output"), exist_ok=True)
+        self.sampler = Sampler(c)
         write_param_out(c)
         self.total_time_dim = 0.0
         if self.restart:
@@ -209,6 +227,7 @@ class Runner:
             if (c.domain_calc and c.domain_nstart <= jt <= c.domain_nend
                     and (jt - c.domain_nstart) % c.domain_nskip == 0):
                 write_domain(case, c, jt)
+            write_samples(case, self.sampler, jt, self.path)
             if c.checkpoint_data and jt % c.checkpoint_nskip == 0:
                 self.finalize_files()
         case.synchronize()

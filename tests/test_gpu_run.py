@@ -7,6 +7,7 @@ import os
 import numpy as np
 import pytest
 
+from lesgo_amd import samples as sm
 from oracle import lesgo_oracle as o
 from gpu_util import gpu_case, rel_l2
 
@@ -82,6 +83,22 @@ OUTPUT {{
   tavg_calc = {tavg}
   tavg_nstart = 2
   tavg_nskip = 1
+  point_calc = .true.
+  point_nstart = 12
+  point_nskip = 6
+  point_loc = 1.0, 0.5, 0.3 // 7.0, -0.4, 0.61
+  xplane_calc = .true.
+  xplane_nstart = 12
+  xplane_nskip = 6
+  xplane_loc = 1.3
+  yplane_calc = .true.
+  yplane_nstart = 12
+  yplane_nskip = 6
+  yplane_loc = 0.77
+  zplane_calc = .true.
+  zplane_nstart = 12
+  zplane_nskip = 6
+  zplane_loc = 0.42
 }}
 TURBINES {{
   num_x = 2
@@ -109,8 +126,10 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
     """A 12-step fresh run from lesgo.conf with Ct' and yaw schedules
     (input_turbines/Ct_prime.dat, theta1.dat at dimensional times), then a
     6-step restart from its vel.out / total_time.dat / u_d_T.dat: the domain
-    output at steps 6, 12, 18, the time averages and check_ke.dat match the
-    oracle driven through the same schedules, at 1e-10."""
+    output (velocity, vorticity) at steps 6, 12, 18, the point and plane
+    samples at step 12, the time averages and check_ke.dat match the oracle
+    driven through the same schedules, at 1e-10 (the samples are the host
+    writers of lesgo_amd.samples applied to the oracle's fields)."""
     from lesgo_amd.conf import place_turbines, read_conf
     from lesgo_amd.run import Runner
     os.makedirs(tmp_path / "input_turbines")
@@ -163,6 +182,37 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
             g = raw[q * n:(q + 1) * n].reshape(17, 32, 32)
             e = rel_l2(g[:16], ref[q][:16])
             assert e <= 1e-10, (jt, name, e)
+        # vorticity (io.f90:823-858) from the oracle's derivative fields; file
+        # record level k is array level k - 1; levels 2..nz-1 compared, level
+        # 1 of vortz is zero
+        der = {nm: s.f[nm][:, :, :32] for nm in ("dudy", "dudz", "dvdx", "dvdz", "dwdx", "dwdy")}
+        vref = sm.vorticity(der, p.nz)
+        raw = np.fromfile(tmp_path / "output" / f"vort.{jt}.c0.bin", dtype=np.float64)
+        for q, name in enumerate(("vortx", "vorty", "vortz")):
+            g = raw[q * n:(q + 1) * n].reshape(17, 32, 32)
+            e = rel_l2(g[1:16], vref[q][2:17])
+            assert e <= 1e-10, (jt, name, e)
+        assert not raw[2 * n:2 * n + 32 * 32].any()
+
+    def check_samples(jt):
+        g = sm.Grid(p.nx, p.ny, p.nz, conf.L_x, conf.L_y, conf.L_z, conf.dx, conf.dy, conf.dz)
+        u, v, w = s.f["u"], s.f["v"], s.f["w"]
+        wu = sm.w_on_uv(w, p.nz)
+        out = tmp_path / "output"
+        for pt, name in (((1.0, 0.5, 0.3), "vel.x-1.00000.y-0.50000.z-0.30000.dat"),
+                         ((7.0 - 2 * np.pi, np.pi - 0.4, 0.61), "vel.x-0.71681.y-2.74159.z-0.61000.dat")):
+            rec = np.loadtxt(out / name, ndmin=2)[0]
+            assert abs(rec[0] - jt * 2.0e-4) <= 1e-12
+            ref = [sm.trilinear(a, g, pt) for a in (u, v, wu)]
+            np.testing.assert_allclose(rec[1:], ref, rtol=1e-10, atol=1e-12)
+        for key, loc, fn, shape in (("x", 1.3, sm.x_plane, (17, 32)), ("y", 0.77, sm.y_plane, (17, 32)),
+                                    ("z", 0.42, sm.z_plane, (32, 32))):
+            raw = np.fromfile(out / f"vel.{key}-{sm.fmt_real(loc)}.{jt}.c0.bin", dtype=np.float64)
+            m = shape[0] * shape[1]
+            assert raw.size == 3 * m
+            for q, a in enumerate((u, v, wu)):
+                e = rel_l2(raw[q * m:(q + 1) * m].reshape(shape), fn(a, g, loc))
+                assert e <= 1e-10, (key, q, e)
 
     for jt in range(1, 13):
         oracle_step()
@@ -171,6 +221,8 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
             ke.append(o.energy(p, s))
         if jt % 6 == 0:
             check_domain(jt)
+        if jt == 12:
+            check_samples(jt)
     kd = np.loadtxt(tmp_path / "output" / "check_ke.dat")
     np.testing.assert_allclose(kd[:, 1], ke, rtol=1e-10)
     n = 32 * 32 * 17
