@@ -261,6 +261,13 @@ int lg_turbines_init_member(lg_case* c, int m, int nturb, const double* spec8,
                             double alpha1, double alpha2, double filter_cutoff,
                             int adm_correction);
 int lg_turbines_get_member(lg_case* c, int m, double* u_d_T, double* f_n, int nturb);
+/* The per-turbine history values of the last forcing (the record of
+ * turbine_<s>.dat, turbines.f90:589-593, written every tbase steps):
+ * out[5 s + 0..4] = u, v, w_uv at the disk centre cell (icp, jcp, kcp of
+ * turbines.f90:196-198, sampled at the forcing, 538-547), u_d and u_d_T
+ * (the file holds -u_d, -u_d_T).  A centre index past the grid is taken
+ * periodically (the reference reads out of bounds there). */
+int lg_turbines_history_member(lg_case* c, int m, double* out, int nturb);
 /* lg_step replayed from captured HIP graphs: the same launches as lg_step,
  * captured once per (RHS storage parity, dynamic-model step) and replayed;
  * recaptured after lg_set_active or a disk change. */

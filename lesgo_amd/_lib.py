@@ -95,6 +95,7 @@ SIGNATURES = {
     "lg_turbines_init_member": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_double, C.c_double,
                                           C.c_double, C.c_int]),
     "lg_turbines_get_member": (C.c_int, [_P, C.c_int, _D, _D, C.c_int]),
+    "lg_turbines_history_member": (C.c_int, [_P, C.c_int, _D, C.c_int]),
     "lg_turbines_time_filter": (C.c_int, [_P, C.c_double, C.c_double, C.c_double]),
     "lg_turbines_set_ct_member": (C.c_int, [_P, C.c_int, _D, C.c_int]),
     "lg_turbines_set_udt_member": (C.c_int, [_P, C.c_int, _D, C.c_int]),

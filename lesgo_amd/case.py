@@ -332,6 +332,15 @@ class Case:
                                            fn.ctypes.data_as(_lib._D), n), "turbines_get")
         return ud, fn
 
+    def turbines_history(self, member: int = 0):
+        """[n, 5]: u, v, w_uv at each disk's centre cell, u_d and u_d_T, as
+        the last forcing saw them (turbines.f90:538-547, 589-593)."""
+        n = self.nturb[member]
+        out = np.empty((n, 5))
+        check(lib().lg_turbines_history_member(self._h, member, out.ctypes.data_as(_lib._D), n),
+              "turbines_history")
+        return out
+
     def turbines_time_filter(self, T_avg_dim: float, z_i: float = 1.0, u_star: float = 1.0):
         """u_d_T filter of every disk (turbines.f90:561-566); T_avg_dim <= 0 is
         none."""

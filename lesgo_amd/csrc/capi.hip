@@ -427,6 +427,26 @@ int lg_turbines_get_member(lg_case* h, int m, double* u_d_T, double* f_n, int nt
     });
 }
 
+int lg_turbines_history_member(lg_case* h, int m, double* out, int nturb) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(out, "null output");
+        REQUIRE(c.turb && c.turb->n > 0 && nturb == c.mturb[m].n(), "turbine count mismatch");
+        c.pull_turbine_state();
+        std::vector<double> ctr(3 * (size_t)nturb);
+        LG_HIP(hipMemcpy(ctr.data(), c.turb->center + 3L * c.turb->first[m],
+                         ctr.size() * sizeof(double), hipMemcpyDeviceToHost));
+        const auto& st = c.mturb[m].state;
+        for (int s = 0; s < nturb; ++s) {
+            out[5 * s + 0] = ctr[3 * s + 0];
+            out[5 * s + 1] = ctr[3 * s + 1];
+            out[5 * s + 2] = ctr[3 * s + 2];
+            out[5 * s + 3] = st[3 * s + 0];
+            out[5 * s + 4] = st[3 * s + 1];
+        }
+    });
+}
+
 int lg_turbines_get(lg_case* h, double* u_d_T, double* f_n, int nturb) {
     return lg_turbines_get_member(h, 0, u_d_T, f_n, nturb);
 }

@@ -116,6 +116,9 @@ TurbineSet::~TurbineSet() {
     if (node_tid) (void)hipFree(node_tid);
     if (coef) (void)hipFree(coef);
     if (state) (void)hipFree(state);
+    if (center_lin) (void)hipFree(center_lin);
+    if (center_wlin) (void)hipFree(center_wlin);
+    if (center) (void)hipFree(center);
 }
 
 // ---------------------------------------------------------------------------

@@ -66,6 +66,9 @@ struct TurbineSet {
     bool has_fz = false;
     double* coef = nullptr;          // per turbine: TC_N coefficients
     double* state = nullptr;         // per turbine: u_d, u_d_T, f_n
+    long* center_lin = nullptr;      // per turbine: the centre cell, and the
+    long* center_wlin = nullptr;     //   w level below its w_uv (turbines.f90:196-198)
+    double* center = nullptr;        // per turbine: u, v, w_uv at the centre (last forcing)
     ~TurbineSet();
 };
 

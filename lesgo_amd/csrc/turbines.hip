@@ -104,10 +104,19 @@ struct TurbFilter {
     const double* clk;
 };
 
+// The disk centre samples written with the history records (turbines.f90:
+// 538-547): u, v and w_uv at each disk's centre cell.
+struct TurbCenter {
+    const long* lin;                 // u, v offset of the centre cell
+    const long* wlin;                // w offset of the level below its w_uv
+    double* out;                     // 3 per turbine
+};
+
 template <typename T>
 __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* node_lin,
                               const double* node_ind, const int* off, const double* coef,
-                              double* state, long plane, double dV, TurbFilter tf) {
+                              double* state, long plane, double dV, TurbFilter tf,
+                              TurbCenter tc) {
     __shared__ double sh[256];
     const int s = blockIdx.x;
     const double n1 = coef[TC_N * s + 0], n2 = coef[TC_N * s + 1], n3 = coef[TC_N * s + 2];
@@ -140,6 +149,10 @@ __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* no
         state[3 * s + 0] = ud;
         state[3 * s + 1] = udt;
         state[3 * s + 2] = fn;
+        const long c = tc.lin[s], cw = tc.wlin[s];
+        tc.out[3 * s + 0] = (double)u[c];
+        tc.out[3 * s + 1] = (double)v[c];
+        tc.out[3 * s + 2] = 0.5 * ((double)w[cw + plane] + (double)w[cw]);
     }
 }
 
@@ -279,6 +292,7 @@ void Case<T>::rebuild_turbines() {
     std::vector<long> lin;
     std::vector<int> tid, off(1, 0);
     std::vector<double> ind, coef, st;
+    std::vector<long> clin, cwlin;
     ts->first.push_back(0);
     int s = 0;
     int kmin = d.nz + 1, kmax = 0;
@@ -297,6 +311,16 @@ void Case<T>::rebuild_turbines() {
             coef.insert(coef.end(), mt.coef.begin() + TC_N * q, mt.coef.begin() + TC_N * (q + 1));
             coef[coef.size() - TC_N + 6] = (double)m;
             st.insert(st.end(), mt.state.begin() + 3 * q, mt.state.begin() + 3 * q + 3);
+            // centre cell (turbines.f90:196-198: Fortran nint of xloc/dx, yloc/dy,
+            // height/dz + 0.5); a centre index past the grid (0 or nx + 1) is
+            // taken periodically and the level kept within 1..nz
+            const double* sp = &mt.spec[8 * q];
+            const long ic = ((std::lround(sp[0] / d.dx) - 1) % d.nx + d.nx) % d.nx;
+            const long jc = ((std::lround(sp[1] / d.dy) - 1) % d.ny + d.ny) % d.ny;
+            const long kc = std::min<long>(std::max<long>(std::lround(sp[2] / d.dz + 0.5), 1), d.nz);
+            const long kw = std::min<long>(kc, d.nz - 1);
+            clin.push_back((long)m * d.field + (kc * d.ny + jc) * d.ld + ic);
+            cwlin.push_back((long)m * d.field + (kw * d.ny + jc) * d.ld + ic);
             if (mt.coef[TC_N * q + 2] != 0.0) ts->has_fz = true;
         }
         ts->first.push_back(s);
@@ -330,6 +354,9 @@ void Case<T>::rebuild_turbines() {
     up(ts->off, off);
     up(ts->coef, coef);
     up(ts->state, st);
+    up(ts->center_lin, clin);
+    up(ts->center_wlin, cwlin);
+    up(ts->center, std::vector<double>(3 * (size_t)s, 0.0));
     turb = std::move(ts);
 }
 
@@ -342,7 +369,8 @@ void Case<T>::turbines_forcing() {
                        (const T*)fld[LG_V], (const T*)fld[LG_W], (const long*)t.node_lin,
                        (const double*)t.node_ind, (const int*)t.off, (const double*)t.coef,
                        t.state, d.plane, d.dx * d.dy * d.dz,
-                       TurbFilter{t_avg_dim, z_i, u_star, d.prm.dt, clk()});
+                       TurbFilter{t_avg_dim, z_i, u_star, d.prm.dt, clk()},
+                       TurbCenter{t.center_lin, t.center_wlin, t.center});
     // the node set is static, so fxa/fya/fza are zero off the nodes
     T* fz_uv = t.has_fz ? tmp[8] : fld[LG_FZA];
     hipLaunchKernelGGL(k_turb_scatter<T>, dim3((t.total_nodes + 255) / 256), dim3(256), 0, stream,

@@ -18,7 +18,9 @@ reading and writing the reference's files in ``path``:
   plane samples with ``point_calc`` / ``*plane_calc`` (io.f90:585-676, the
   writers of ``lesgo_amd.samples``); the time averages ``output/{veluv,velw,vel2,tau,pres,
   force,cs_opt2,vort}_avg.bin`` and ``output/rs.bin`` with ``tavg_calc``
-  (time_average.f90:322-655); ``turbine/u_d_T.dat`` (turbines.f90:657-679).
+  (time_average.f90:322-655); ``turbine/u_d_T.dat`` (turbines.f90:657-679)
+  and the per-turbine histories ``turbine/turbine_<s>.dat`` every ``tbase``
+  steps (turbines.f90:589-593).
 
 A new run starts from the seeded log-law field (``--seed``) in place of the
 reference's /dev/urandom ic_les.
@@ -151,12 +153,34 @@ class Runner:
             if os.path.exists(fu):
                 vals = [float(x) for x in open(fu).read().split()]
                 self.case.turbines_set_udt(vals[:len(self.specs)])
+            # the per-turbine history files, opened for appending at
+            # initialisation (turbines.f90:262-267)
+            os.makedirs(os.path.join(self.path, "turbine"), exist_ok=True)
+            for q in range(len(self.specs)):
+                open(self._turbine_file(q), "a").close()
         if c.tavg_calc:
             self.case.stats_enable(c.tavg_nstart, c.tavg_nend, c.tavg_nskip)
 
     def say(self, s):
         if not self.quiet:
             print(s, flush=True)
+
+    def _turbine_file(self, q):
+        return os.path.join(self.path, "turbine", f"turbine_{q + 1}.dat")
+
+    def write_turbine_history(self):
+        """One record per disk (turbines.f90:589-593): total_time_dim, u, v,
+        w_uv at the disk centre, -u_d, -u_d_T, theta1, theta2, Ct_prime of the
+        step's forcing.  With use_cfl_dt the dimensional time is the device
+        clock's total time times z_i / u_star."""
+        c = self.conf
+        t_dim = (self.case.clock()["total_time"] * c.z_i / c.u_star if c.use_cfl_dt
+                 else self.total_time_dim)
+        h = self.case.turbines_history()
+        for q, sp in enumerate(self.specs):
+            vals = [t_dim, h[q, 0], h[q, 1], h[q, 2], -h[q, 3], -h[q, 4], sp[5], sp[6], sp[7]]
+            with open(self._turbine_file(q), "a") as f:
+                f.write(" " + " ".join(repr(float(x)) for x in vals) + "\n")
 
     def _apply_schedules(self, t_dim) -> str:
         """Interpolate the dynamic schedules at t_dim into self.specs; returns
@@ -228,6 +252,8 @@ class Runner:
                     and (jt - c.domain_nstart) % c.domain_nskip == 0):
                 write_domain(case, c, jt)
             write_samples(case, self.sampler, jt, self.path)
+            if self.specs and jt % c.tbase == 0:
+                self.write_turbine_history()
             if c.checkpoint_data and jt % c.checkpoint_nskip == 0:
                 self.finalize_files()
         case.synchronize()

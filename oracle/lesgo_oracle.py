@@ -1276,6 +1276,15 @@ class Turbines:
         self.u_d_T = np.full(len(self.specs), -1.0)
         self.u_d = np.zeros(len(self.specs))
         self.f_n = np.zeros(len(self.specs))
+        # disk centre cells (turbines.f90:196-198), 0-based x, y; an index
+        # past the grid is taken periodically, the level kept within 1..nz
+        self.centers = []
+        for t in self.specs:
+            ic = (_nint(t.xloc / dx) - 1) % nx
+            jc = (_nint(t.yloc / dy) - 1) % ny
+            kc = min(max(_nint(t.height / dz + 0.5), 1), p.nz)
+            self.centers.append((ic, jc, kc))
+        self.center_vel = np.zeros((len(self.specs), 3))
 
     def copy(self):
         t = object.__new__(Turbines)
@@ -1283,6 +1292,7 @@ class Turbines:
         t.u_d_T = self.u_d_T.copy()
         t.u_d = self.u_d.copy()
         t.f_n = self.f_n.copy()
+        t.center_vel = self.center_vel.copy()
         return t
 
     def forcing(self, s: State):
@@ -1313,6 +1323,9 @@ class Turbines:
             if self.adm_correction:
                 ud = ud / (1 + 0.25 * (1 - self.ind_funcs[n].M) * t.ct_prime)
             self.u_d[n] = ud
+            # disk centre velocity (turbines.f90:538-547)
+            ic, jc, kc = self.centers[n]
+            self.center_vel[n] = (s.u[kc, jc, ic], s.v[kc, jc, ic], w_uv[kc, jc, ic])
             self.u_d_T[n] = (1. - eps) * self.u_d_T[n] + eps * ud
             udt = self.u_d_T[n]
             fn = -0.5 * t.ct_prime * abs(udt) * udt * 0.25 * PI * t.dia ** 2

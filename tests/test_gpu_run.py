@@ -118,6 +118,7 @@ TURBINES {{
   alpha2 = 1.5
   filter_cutoff = 1e-2
   adm_correction = .true.
+  tbase = 3
 }}
 """
 
@@ -127,7 +128,8 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
     (input_turbines/Ct_prime.dat, theta1.dat at dimensional times), then a
     6-step restart from its vel.out / total_time.dat / u_d_T.dat: the domain
     output (velocity, vorticity) at steps 6, 12, 18, the point and plane
-    samples at step 12, the time averages and check_ke.dat match the oracle
+    samples at step 12, the per-turbine histories every 3 steps, the time
+    averages and check_ke.dat match the oracle
     driven through the same schedules, at 1e-10 (the samples are the host
     writers of lesgo_amd.samples applied to the oracle's fields)."""
     from lesgo_amd.conf import place_turbines, read_conf
@@ -157,6 +159,20 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
     s.turbines = o.Turbines(p, specs)
     st = o.Stats(p, 2, 50000, 1)
     t_dim, ke = 0.0, []
+    hist = [[] for _ in specs]       # turbines.f90:589-593 records, tbase = 3
+
+    def record_history():
+        tb = s.turbines
+        for q, t in enumerate(tb.specs):
+            hist[q].append([t_dim, *tb.center_vel[q], -tb.u_d[q], -tb.u_d_T[q], t.theta1,
+                            t.theta2, t.ct_prime])
+
+    def check_history():
+        for q in range(len(specs)):
+            got = np.loadtxt(tmp_path / "turbine" / f"turbine_{q + 1}.dat", ndmin=2)
+            ref = np.asarray(hist[q])
+            assert got.shape == ref.shape, (q, got.shape, ref.shape)
+            np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-13)
 
     def oracle_step():
         nonlocal t_dim
@@ -223,6 +239,9 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
             check_domain(jt)
         if jt == 12:
             check_samples(jt)
+        if jt % 3 == 0:
+            record_history()
+    check_history()
     kd = np.loadtxt(tmp_path / "output" / "check_ke.dat")
     np.testing.assert_allclose(kd[:, 1], ke, rtol=1e-10)
     n = 32 * 32 * 17
@@ -239,6 +258,9 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
     r.close()
     for jt in range(13, 19):
         oracle_step()
+        if jt % 3 == 0:
+            record_history()
     check_domain(18)
+    check_history()
     w = open(tmp_path / "total_time.dat").read().split()
     assert int(w[0]) == 18

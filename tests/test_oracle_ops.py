@@ -272,3 +272,23 @@ def test_lagrange_floor_sensitivity_is_intrinsic():
     o.step(sp, s)
     o.step(sp, t)
     assert rel(t.u[X], s.u[X]) > 1e-11
+
+
+def test_turbine_center_cells_and_sample():
+    """Disk centre cells (turbines.f90:196-198): Fortran nint (half away from
+    zero) of xloc/dx, yloc/dy, height/dz + 0.5, 1-based; 0-based x, y here,
+    a centre left of the first cell taken periodically; the centre sample is
+    u, v, w_uv there (turbines.f90:538-547)."""
+    p = o.Params(32, 16, 16)
+    dx, dy, dz = p.dx, p.dy, p.dz
+    specs = [o.TurbineSpec(3.3 * dx, 5.6 * dy, 2.2 * dz, 2 * dy, 0.0),      # (3, 6, 3)
+             o.TurbineSpec(0.4 * dx, 7.49 * dy, 4.0 * dz, 2 * dy, 0.0)]     # (0 -> 32, 7, 5): 4.5 -> 5
+    t = o.Turbines(p, specs, N=256)
+    assert t.centers == [(2, 5, 3), (31, 6, 5)]
+    s = o.initial_condition(p, 0)
+    s.u[...] = np.arange(s.u.size, dtype=float).reshape(s.u.shape)
+    s.v[...] = 2.0
+    s.w[...] = np.arange(p.nz + 1, dtype=float)[:, None, None]
+    t.forcing(s)
+    np.testing.assert_array_equal(t.center_vel[0], [s.u[3, 5, 2], 2.0, 3.5])
+    np.testing.assert_array_equal(t.center_vel[1], [s.u[5, 6, 31], 2.0, 5.5])
