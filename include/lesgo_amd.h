@@ -292,6 +292,15 @@ int lg_stats_enable(lg_case* c, int nstart, int nend, int nskip);
 int lg_stats_info(lg_case* c, double* total_time, int* initialized);
 /* Time average of one quantity of member m, host double [nz+1][ny][nx]. */
 int lg_stats_mean(lg_case* c, int m, int quantity, double* host);
+/* The raw time-weighted sum behind lg_stats_mean (the arrays tavg.out holds,
+ * time_average.f90:665-716). */
+int lg_stats_sums(lg_case* c, int m, int quantity, double* host);
+/* Continue a previous run's sums (tavg%init reading tavg.out,
+ * time_average.f90:119-166): member m's sums of all LG_TAVG_N quantities
+ * (LG_TAVG_N arrays [nz+1][ny][nx] in quantity order) and the averaging
+ * time, after lg_stats_enable and before the summation starts (without
+ * use_cfl_dt every member shares one averaging time). */
+int lg_stats_restore(lg_case* c, int m, double total_time, const double* sums);
 /* Streamwise spectrum (0 u, 1 v, 2 w) of member m, host double [nz+1][nx/2]:
  * E(kx, z) = c_kx sum_ky |f_hat(kx, ky, z)|^2 of the normalised x-y
  * transform on levels 1..nz-1, time-averaged; c_0 = 1, c_kx = 2 otherwise,

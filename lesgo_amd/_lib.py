@@ -111,6 +111,8 @@ SIGNATURES = {
     "lg_stats_enable": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "lg_stats_info": (C.c_int, [_P, _D, _I]),
     "lg_stats_mean": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_stats_sums": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lg_stats_restore": (C.c_int, [_P, C.c_int, C.c_double, _D]),
     "lg_stats_spectrum": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "lg_diff_l2": (C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
     "lg_mem_info": (C.c_int, [_D, _D]),

@@ -277,6 +277,24 @@ class Case:
                                   a.ctypes.data_as(_lib._D)), "stats_mean")
         return a
 
+    def stats_sums(self, name: str, member: int = 0):
+        """The raw time-weighted sum behind ``stats_mean`` (tavg.out's array)."""
+        a = np.empty((self.nz + 1, self.ny, self.cfg.nx))
+        check(lib().lg_stats_sums(self._h, member, _lib.TAVG_NAMES.index(name),
+                                  a.ctypes.data_as(_lib._D)), "stats_sums")
+        return a
+
+    def stats_restore(self, total_time: float, sums: dict, member: int = 0):
+        """Continue a previous run's sums (tavg%init reading tavg.out): call
+        after ``stats_enable``; quantities missing from ``sums`` start at zero."""
+        shp = (self.nz + 1, self.ny, self.cfg.nx)
+        buf = np.zeros((len(_lib.TAVG_NAMES),) + shp)
+        for q, n in enumerate(_lib.TAVG_NAMES):
+            if n in sums:
+                buf[q] = np.asarray(sums[n], dtype=np.float64).reshape(shp)
+        check(lib().lg_stats_restore(self._h, member, float(total_time),
+                                     buf.ctypes.data_as(_lib._D)), "stats_restore")
+
     def stats_profile(self, name: str, member: int = 0):
         return self.stats_mean(name, member).mean(axis=(1, 2))
 

@@ -115,3 +115,38 @@ def read_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "
     if set_counters:
         case.set_counters(jt_total, jt_total)
     return jt_total, t
+
+
+# tavg.out: the averaging time, then the running sums in this order (p is
+# summed but not written, time_average.f90:665-716), each (nx, ny, lbz:nz)
+TAVG_CHECKPOINT = ("u", "v", "w", "u_w", "v_w", "w_uv", "u2", "v2", "w2", "uv", "uw", "vw",
+                   "txx", "tyy", "tzz", "txy", "txz", "tyz", "fx", "fy", "fz", "cs_opt2",
+                   "vortx", "vorty", "vortz")
+
+
+def write_tavg_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "tavg.out"):
+    """tavg%checkpoint (time_average.f90:665-716): one record with the
+    averaging time, then one per array."""
+    total_time, initialized = case.stats_info()
+    if not initialized:
+        return False
+    with open(os.path.join(path, fname), "wb") as f:
+        write_record(f, struct.pack("<d", total_time))
+        for n in TAVG_CHECKPOINT:
+            write_record(f, np.ascontiguousarray(case.stats_sums(n, member)).astype("<f8").tobytes())
+    return True
+
+
+def read_tavg_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "tavg.out"):
+    """tavg%init with an existing tavg.out (time_average.f90:113-166): the
+    sums continue from the file (the pressure sum from zero, as the file has
+    none); call after ``stats_enable``.  Returns False without a file."""
+    fn = os.path.join(path, fname)
+    if not os.path.exists(fn):
+        return False
+    shp = (case.nz + 1, case.ny, case.cfg.nx)
+    with open(fn, "rb") as f:
+        (total_time,) = struct.unpack("<d", read_record(f))
+        sums = {n: np.frombuffer(read_record(f), dtype="<f8").reshape(shp) for n in TAVG_CHECKPOINT}
+    case.stats_restore(total_time, sums, member)
+    return True

@@ -599,6 +599,23 @@ int lg_stats_mean(lg_case* h, int m, int quantity, double* host) {
     });
 }
 
+int lg_stats_sums(lg_case* h, int m, int quantity, double* host) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(host, "null host pointer");
+        c.stats_sums(m, quantity, host);
+    });
+}
+
+int lg_stats_restore(lg_case* h, int m, double total_time, const double* sums) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(sums, "null host pointer");
+        REQUIRE(total_time >= 0.0, "averaging time must be >= 0");
+        c.stats_restore(m, total_time, sums);
+    });
+}
+
 int lg_stats_spectrum(lg_case* h, int m, int component, double* host) {
     return dispatch(h, [&](auto& c) {
         check_member(c, m);

@@ -274,6 +274,8 @@ struct Case {
     void stats_after_step();
     double stats_time(int m);
     void stats_mean(int m, int q, double* host);
+    void stats_sums(int m, int q, double* host);
+    void stats_restore(int m, double total_time, const double* sums);
     void stats_spectrum(int m, int c, double* host);
 };
 

@@ -196,6 +196,44 @@ void Case<T>::stats_mean(int m, int q, double* host) {
     for (long i = 0; i < pm; ++i) host[i] = tt > 0 ? host[i] / tt : 0.0;
 }
 
+// Raw time-weighted sums of quantity q of member m: the arrays of tavg.out
+// (time_average.f90:665-716).
+template <typename T>
+void Case<T>::stats_sums(int m, int q, double* host) {
+    if (!tavg || q < 0 || q >= TQ_N) {
+        set_error("statistics not enabled or quantity out of range");
+        throw ArgError();
+    }
+    const long pm = (long)(d.nz + 1) * d.ny * d.nx;
+    LG_HIP(hipStreamSynchronize(stream));
+    LG_HIP(hipMemcpy(host, tavg + (long)q * M * pm + (long)m * pm, pm * sizeof(double),
+                     hipMemcpyDeviceToHost));
+}
+
+// Continue the sums of a previous run (tavg%init reading tavg.out,
+// time_average.f90:119-166): member m's sums of every quantity (TQ_N arrays
+// in quantity order) and the averaging time.  Without use_cfl_dt the
+// averaging time is one for every member.  Called after stats_enable and
+// before the summation starts; the first summation step then only starts
+// the summation, as after tavg%init.
+template <typename T>
+void Case<T>::stats_restore(int m, double total_time, const double* sums) {
+    if (!tavg) {
+        set_error("statistics not enabled");
+        throw ArgError();
+    }
+    const long pm = (long)(d.nz + 1) * d.ny * d.nx;
+    LG_HIP(hipStreamSynchronize(stream));
+    for (int q = 0; q < TQ_N; ++q)
+        LG_HIP(hipMemcpy(tavg + (long)q * M * pm + (long)m * pm, sums + (long)q * pm,
+                         pm * sizeof(double), hipMemcpyHostToDevice));
+    if (cfl_mode)
+        LG_HIP(hipMemcpy(d_clock + (long)CK_N * m + CK_STIME, &total_time, sizeof(double),
+                         hipMemcpyHostToDevice));
+    else
+        st.total_time = total_time;
+}
+
 template <typename T>
 void Case<T>::stats_spectrum(int m, int c, double* host) {
     if (!tspec || c < 0 || c > 2) {
@@ -279,6 +317,8 @@ void diff_l2(Case<TA>& a, int ma, Case<TB>& b, int mb, int field, int k0, int k1
     template void Case<T>::stats_enable(int, int, int);               \
     template void Case<T>::stats_after_step();                        \
     template void Case<T>::stats_mean(int, int, double*);             \
+    template void Case<T>::stats_sums(int, int, double*);             \
+    template void Case<T>::stats_restore(int, double, const double*); \
     template void Case<T>::stats_spectrum(int, int, double*);
 LG_INST_STATS(double)
 LG_INST_STATS(float)

@@ -18,7 +18,9 @@ reading and writing the reference's files in ``path``:
   plane samples with ``point_calc`` / ``*plane_calc`` (io.f90:585-676, the
   writers of ``lesgo_amd.samples``); the time averages ``output/{veluv,velw,vel2,tau,pres,
   force,cs_opt2,vort}_avg.bin`` and ``output/rs.bin`` with ``tavg_calc``
-  (time_average.f90:322-655); ``turbine/u_d_T.dat`` (turbines.f90:657-679)
+  (time_average.f90:322-655) and their restart file ``tavg.out``
+  (time_average.f90:665-716, read back on a restart as tavg%init does);
+  ``turbine/u_d_T.dat`` (turbines.f90:657-679)
   and the per-turbine histories ``turbine/turbine_<s>.dat`` every ``tbase``
   steps (turbines.f90:589-593).
 
@@ -39,7 +41,8 @@ import time
 import numpy as np
 
 from .case import Case
-from .checkpoint import read_checkpoint, write_checkpoint
+from .checkpoint import (read_checkpoint, read_tavg_checkpoint, write_checkpoint,
+                         write_tavg_checkpoint)
 from .conf import (ConfError, LesgoConf, linear_interp, place_turbines, read_conf,
                    read_control_files, write_param_out)
 from .samples import Sampler, vorticity
@@ -160,6 +163,9 @@ class Runner:
                 open(self._turbine_file(q), "a").close()
         if c.tavg_calc:
             self.case.stats_enable(c.tavg_nstart, c.tavg_nend, c.tavg_nskip)
+            # earlier sums continue (tavg%init reads tavg.out when present)
+            if read_tavg_checkpoint(self.case, self.path):
+                self.say("--> Continuing the time averages from tavg.out")
 
     def say(self, s):
         if not self.quiet:
@@ -271,6 +277,8 @@ class Runner:
             # dt_dim = dt z_i / u_star summed: the device clock's time scaled
             self.total_time_dim = self.case.clock()["total_time"] * c.z_i / c.u_star
         write_checkpoint(self.case, self.path, total_time_dim=self.total_time_dim)
+        if c.tavg_calc:
+            write_tavg_checkpoint(self.case, self.path)      # io.f90:1229
         if self.specs:
             os.makedirs(os.path.join(self.path, "turbine"), exist_ok=True)
             udt, _ = self.case.turbines_state()

@@ -129,7 +129,10 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
     6-step restart from its vel.out / total_time.dat / u_d_T.dat: the domain
     output (velocity, vorticity) at steps 6, 12, 18, the point and plane
     samples at step 12, the per-turbine histories every 3 steps, the time
-    averages and check_ke.dat match the oracle
+    averages (continued through tavg.out by the restart: the sums go on from
+    the file, the pressure sum from zero as the file has none, and the first
+    summation step after the restart only starts the summation) and
+    check_ke.dat match the oracle
     driven through the same schedules, at 1e-10 (the samples are the host
     writers of lesgo_amd.samples applied to the oracle's fields)."""
     from lesgo_amd.conf import place_turbines, read_conf
@@ -250,17 +253,28 @@ def test_run_from_conf_with_schedules_and_restart(tmp_path):
     assert e <= 1e-10, e
     assert os.path.exists(tmp_path / "output" / "lesgo_param.out")
     # restart for 6 more steps
+    assert os.path.exists(tmp_path / "tavg.out")
     with open(tmp_path / "lesgo.conf", "w") as f:
-        f.write(CONF.format(nsteps=18, tavg=".false."))
+        f.write(CONF.format(nsteps=18, tavg=".true."))
     r = Runner(str(tmp_path), quiet=True)
     assert r.restart and r.case.counters()[1] == 12
     r.run()
     r.close()
+    st2 = o.Stats(p, 2, 50000, 1)       # tavg%init from tavg.out at step 13
+    st2.sum = {k: (np.zeros_like(a) if k == "p" else a.copy()) for k, a in st.sum.items()}
+    st2.total_time = st.total_time
     for jt in range(13, 19):
         oracle_step()
+        st2.after_step(sp, s)
         if jt % 3 == 0:
             record_history()
     check_domain(18)
     check_history()
+    assert st2.total_time > st.total_time
+    for q, (fname, name) in enumerate((("veluv_avg.bin", "u"), ("vel2_avg.bin", "u2"),
+                                       ("pres_avg.bin", "p"))):
+        avg = np.fromfile(tmp_path / "output" / fname, dtype=np.float64)
+        e = rel_l2(avg[:n].reshape(17, 32, 32)[:16], st2.mean(name)[1:17])
+        assert e <= 1e-10, (fname, e)
     w = open(tmp_path / "total_time.dat").read().split()
     assert int(w[0]) == 18
