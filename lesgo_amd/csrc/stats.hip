@@ -218,8 +218,8 @@ void Case<T>::stats_sums(int m, int q, double* host) {
 // the summation, as after tavg%init.
 template <typename T>
 void Case<T>::stats_restore(int m, double total_time, const double* sums) {
-    if (!tavg) {
-        set_error("statistics not enabled");
+    if (!tavg || st.initialized) {
+        set_error(tavg ? "the summation has already started" : "statistics not enabled");
         throw ArgError();
     }
     const long pm = (long)(d.nz + 1) * d.ny * d.nx;
