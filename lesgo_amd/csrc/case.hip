@@ -132,6 +132,18 @@ static T* dalloc(long n) {
     return p;
 }
 
+// Per-member step flags written in stream order from a launch's arguments
+// (up to FLAGS_ARG_MAX values: 3 KB of the 4 KB argument space).
+constexpr int FLAGS_ARG_MAX = 768;
+struct FlagArgs {
+    int n;
+    int v[FLAGS_ARG_MAX];
+};
+
+__global__ void k_set_flags(int* dst, FlagArgs a) {
+    for (int i = threadIdx.x; i < a.n; i += blockDim.x) dst[i] = a.v[i];
+}
+
 // The Lagrangian model's fused row passes (k_lag_prep_rows, k_lag_rows):
 // where nx has a kernel for them and LESGO_AMD_LAG_ROWS is not 0.  They
 // need no physical quantity fields and 21 complex + 4 FP64 filtered slots
@@ -1339,9 +1351,18 @@ bool Case<T>::refresh_flags(bool force_lag) {
     }
     lag_now = any_lag;
     if (hflags != dflags_now) {
-        LG_HIP(hipStreamSynchronize(stream));
-        LG_HIP(hipMemcpy(d_flags, hflags.data(), hflags.size() * sizeof(int),
-                         hipMemcpyHostToDevice));
+        if ((int)hflags.size() <= FLAGS_ARG_MAX) {
+            // in stream order, from the kernel's arguments: no host wait
+            FlagArgs fa;
+            fa.n = (int)hflags.size();
+            std::copy(hflags.begin(), hflags.end(), fa.v);
+            hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(256), 0, stream, d_flags, fa);
+            LG_HIP(hipGetLastError());
+        } else {
+            LG_HIP(hipStreamSynchronize(stream));
+            LG_HIP(hipMemcpy(d_flags, hflags.data(), hflags.size() * sizeof(int),
+                             hipMemcpyHostToDevice));
+        }
         dflags_now = hflags;
     }
     return special;
