@@ -531,8 +531,9 @@ void Case<T>::inv2d(int njobs, const cplx<T>* const* src, const int* ops, long s
             c.dst[nc] = tmpspec[nc];
             c.op[nc] = ops[j0 + i];
             rdst[nc] = dst[j0 + i];
-            nce[nc] = ops[j0 + i] == OP_G1 ? g_nzc[0] : ops[j0 + i] == OP_G2 ? g_nzc[1] : 0;
-            if (ops[j0 + i] == OP_G1 || ops[j0 + i] == OP_G2) {
+            const bool g1 = ops[j0 + i] == OP_G1 || ops[j0 + i] == OP_G1B;
+            nce[nc] = g1 ? g_nzc[0] : ops[j0 + i] == OP_G2 ? g_nzc[1] : 0;
+            if (g1 || ops[j0 + i] == OP_G2) {
                 // test-filtered: nothing outside the filter's band
                 const int f = ops[j0 + i] == OP_G2 ? 1 : 0;
                 c.nzc[nc] = g_nzc[f];
@@ -809,15 +810,27 @@ void Case<T>::test_filter_plane(const T* in, T* out, int op) {
 // ---------------------------------------------------------------------------
 // wallstress.f90 (lbc_mom = 2, ubc_mom = 0)
 // ---------------------------------------------------------------------------
+// su, sv (the spectral-resident step): the velocity spectra, normalised by
+// 1/(nx ny), whose inverse is the step's u, v; the test filter of the wall
+// level is then their G_test band, with no forward transform (the same
+// values to rounding: G_test is 1/(nx ny) inside a sharp band and zero
+// outside it and on the Nyquist row; the band mask also drops the kx Nyquist
+// column, which the physical u, v do not carry).
 template <typename T>
-void Case<T>::wallstress() {
+void Case<T>::wallstress(const cplx<T>* su, const cplx<T>* sv) {
     const T one = (T)1;
-    const T* src[2] = {lev(LG_U, 1), lev(LG_V, 1)};
-    const T sc[2] = {one, one};
-    cplx<T>* S0[2] = {spec[4], spec[5]};
-    fwd2d(2, src, sc, d.plane, 0, d.field, S0, d.splane, 0, msp, 1);
     const cplx<T>* s2[2] = {spec[4], spec[5]};
-    const int ops[2] = {OP_G1, OP_G1};
+    int ops[2] = {OP_G1, OP_G1};
+    if (su && sv) {
+        s2[0] = su + d.splane;
+        s2[1] = sv + d.splane;
+        ops[0] = ops[1] = OP_G1B;
+    } else {
+        const T* src[2] = {lev(LG_U, 1), lev(LG_V, 1)};
+        const T sc[2] = {one, one};
+        cplx<T>* S0[2] = {spec[4], spec[5]};
+        fwd2d(2, src, sc, d.plane, 0, d.field, S0, d.splane, 0, msp, 1);
+    }
     T* dst[2] = {tmp[0], tmp[1]};
     cplx<T>* tsp[2] = {spec[6], spec[7]};
     inv2d(2, s2, ops, d.splane, 0, msp, dst, d.plane, 0, d.field, 1, tsp);
@@ -1465,7 +1478,7 @@ void Case<T>::step_body_spec(bool cold) {
     }
     filt_da_uvw(false);
     ddz_uvw();
-    wallstress();
+    wallstress(sU, sV);
     sgs_stag();
     divstress_spec();
     convec(true, true);

@@ -211,7 +211,7 @@ struct Case {
     void ddz_w(int f, int dfdz);
     void ddz_uvw();        // ddz_uv(u), ddz_uv(v), ddz_w(w): one pass, the same values
     void test_filter_plane(const T* in, T* out, int op);
-    void wallstress();
+    void wallstress(const cplx<T>* su = nullptr, const cplx<T>* sv = nullptr);
     void calc_sij();
     void sgs_stag();
     void interpolag_sdep();

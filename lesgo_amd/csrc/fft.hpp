@@ -401,6 +401,8 @@ enum ColOp : int {
     OP_G1 = 3,         // .MULR. G_test       (test_filtermodule.f90:141)
     OP_G2 = 4,         // .MULR. G_test_test  (test_filtermodule.f90:164)
     OP_DIVXY = 5,      // (.MULI. kx) src + (.MULI. ky) src2: d/dx + d/dy of two fields
+    OP_G1B = 6,        // G_test's band with unit weight, kx Nyquist column zero: the
+                       // test-filtered field from spectra already normalised by 1/(nx ny)
 };
 
 enum ColMap : int {
@@ -478,6 +480,10 @@ __device__ __forceinline__ cplx<T> apply_op(int op, cplx<T> v, int c, int j, con
             k = a.kys * (T)m;
         }
         return cplx<T>{-v.y * k, v.x * k};
+    }
+    if (op == OP_G1B) {
+        const bool in = a.G1[(long)j * a.ncols + c] != (T)0 && c != a.ncols - 1;
+        return in ? v : cplx<T>{(T)0, (T)0};
     }
     const T g = (op == OP_G1 ? a.G1 : a.G2)[(long)j * a.ncols + c];
     return cplx<T>{g * v.x, g * v.y};
