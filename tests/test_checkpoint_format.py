@@ -72,3 +72,51 @@ def test_records_interoperate_with_fortran_io(tmp_path):
     np.testing.assert_array_equal(raw[:n].reshape(a.shape), 3 * b)
     np.testing.assert_array_equal(raw[n:].reshape(a.shape), a)
     shutil.rmtree(tmp_path, ignore_errors=True)
+
+
+class _StatsStub:
+    """The part of a Case the tavg.out writer and reader use."""
+
+    class _Cfg:
+        nx = 6
+
+    def __init__(self):
+        self.nz, self.ny, self.cfg = 4, 5, self._Cfg()
+        self.restored = None
+
+    def stats_info(self):
+        return 0.375, True
+
+    def stats_sums(self, name, member=0):
+        q = ck.TAVG_CHECKPOINT.index(name) if name in ck.TAVG_CHECKPOINT else 99
+        return np.arange(5 * 5 * 6, dtype=float).reshape(5, 5, 6) + 1000.0 * q
+
+    def stats_restore(self, total_time, sums, member=0):
+        self.restored = (total_time, sums)
+
+
+def test_tavg_checkpoint_layout_and_round_trip(tmp_path):
+    """tavg.out (time_average.f90:665-716): a record with the averaging time,
+    then one per running sum in the reference's order with no pressure
+    record, each (nx, ny, 0:nz) doubles; reading it back restores exactly
+    those sums (the pressure sum absent, so it restarts from zero)."""
+    c = _StatsStub()
+    assert ck.write_tavg_checkpoint(c, str(tmp_path))
+    size = os.path.getsize(tmp_path / "tavg.out")
+    recs = []
+    with open(tmp_path / "tavg.out", "rb") as f:
+        while f.tell() < size:
+            recs.append(ck.read_record(f))
+    assert len(recs) == 26
+    assert np.frombuffer(recs[0], "<f8").tolist() == [0.375]
+    assert all(len(r) == 5 * 5 * 6 * 8 for r in recs[1:])
+    assert len(ck.TAVG_CHECKPOINT) == 25 and "p" not in ck.TAVG_CHECKPOINT
+    assert ck.TAVG_CHECKPOINT[:6] == ("u", "v", "w", "u_w", "v_w", "w_uv")
+    np.testing.assert_array_equal(np.frombuffer(recs[4], "<f8").reshape(5, 5, 6),
+                                  c.stats_sums("u_w"))
+    assert ck.read_tavg_checkpoint(c, str(tmp_path))
+    t, sums = c.restored
+    assert t == 0.375 and set(sums) == set(ck.TAVG_CHECKPOINT)
+    for n in ck.TAVG_CHECKPOINT:
+        np.testing.assert_array_equal(sums[n], c.stats_sums(n))
+    assert not ck.read_tavg_checkpoint(c, str(tmp_path / "none"))
