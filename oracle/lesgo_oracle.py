@@ -546,7 +546,10 @@ def trilinear_interp_w(p: Params, var, px, py, pz):
     index 1 or N; near the wall (lbc_mom > 0, pz < zw(2)) the uvp level 1 is
     used with a zero-gradient clamp below z(1); otherwise kstart =
     floor((pz - zw(1))/dz) + 1 with the nz-1 clamp at zw(nz).
-    Pinned by: tests/test_oracle_ops.py::test_interp_linear_exact.
+    Pinned by: tests/test_oracle_ops.py::test_interp_linear_exact (interior
+    cells), tests/test_oracle_lagrange.py::test_interp_near_wall_cells_linear_in_z
+    (zero-gradient and half cells at the wall, the lid point) and
+    ::test_interp_periodic_seam_matches_numpy_interp (the seam, MODULO wrap).
     """
     nx, ny, nz = p.nx, p.ny, p.nz
     dx, dy, dz = p.dx, p.dy, p.dz
@@ -632,8 +635,11 @@ def lagrange_Sdep(sp: Spectral, s: State):
     call at jt = cs_count or DYN_init sets F_MM = MM, F_LM = 0.03 MM (and
     F_NN, F_QN alike), padding columns 1 (266-277, 326-337).  All levels
     1..nz are processed at once (each level's work is independent).
-    Parity unpinned by an external vector (no reference run exists);
-    pinned by invariants in tests/test_oracle_ops.py::test_lagrange_*."""
+    Pinned by tests/test_oracle_lagrange.py: single-mode fields whose
+    filtered products, L, Q, M, N and contractions are known in closed form,
+    the u_bar/w_bar staggering, the clips, beta, the relaxation weights
+    (epsi = 1 and 1/2) and the first-call initialisation; plus translation
+    invariance (tests/test_oracle_ops.py)."""
     p = sp.p
     nz = p.nz
     opftdelta = 1.5 * p.delta
@@ -1179,7 +1185,10 @@ class IndicatorFunction:
     Gaussian of width delta2 with the unit-disk indicator, normalised so
     that int 2 pi r R2 dr = 1; M = pi int 2 pi r R2^2 dr.  val(r, x) = R1(x)
     R2(r) / ell^3 with R1 the erf-smoothed slab of thickness thk.
-    Pinned by: tests/test_oracle_turbines.py (normalisation, M bounds)."""
+    Pinned by: tests/test_oracle_turbines.py (the table against the exact
+    Hankel-transform profile sampled as the reference's circular convolution
+    samples it, M against the sampled and the continuum integrals and its
+    delta -> 0 limit, the slab's integral and second moment)."""
 
     def __init__(self, delta1, delta2, thk, dia, N=2048):
         self.ell = 0.5 * dia
@@ -1213,7 +1222,10 @@ class IndicatorFunction:
 
 class Turbines:
     """Actuator-disk model state (turbines.f90:123-638): node lists with
-    normalised indicator weights, and u_d_T per turbine."""
+    normalised indicator weights, and u_d_T per turbine.
+    Pinned by: tests/test_oracle_turbines.py (node weights against the exact
+    indicator at the nodes' coordinates, disk velocity, thrust and total
+    force in uniform flow with yaw and tilt, the time filter's weight)."""
 
     def __init__(self, p: Params, specs, alpha1=1.5, alpha2=1.5,
                  filter_cutoff=1e-2, adm_correction=True, N=2048, T_avg_dim=-1.0,
