@@ -5,11 +5,16 @@ Ly = pi, Lz = 1, 4 x 4 aligned actuator-disk array (D = 0.1, hub 0.1,
 7D x 5D, Ct' = 1.33), Lagrangian scale-dependent SGS, equilibrium wall
 model z0 = 1e-4, dt = 1e-4, seeded log-law start (seed 7) run as a fresh
 start (Euler first step, Smagorinsky Cs^2 = 0.03 until DYN_init = 100, then
-the dynamic model every 5 steps).  The first DYN_init steps are run before
-the warm-up (untimed), so the timed steps carry the dynamic model at its
-cadence.  Both the plain FP64 step and the mixed-precision step are timed;
-`value` is the mixed step's grid-pts * steps / s, the FP64 figure and the
-ratio are reported beside it.
+the dynamic model every 5 steps).  The FP64 and the mixed-precision case
+run the first DYN_init steps untimed; their velocity difference is measured
+on the device at that point; then each runs a warm-up and three passes of
+the timed steps: graph-replayed with no per-launch events (`value`, the
+headline), eager with no events, and eager with HIP events around every
+launch of the transform and tridiagonal kernel classes (the per-kernel
+roofline).  `value` is the mixed step's grid-pts * steps / s; the FP64
+figure and the ratio are reported beside it, with the whole-step roofline
+(algorithmic bytes and flops per step against the HBM and vector peaks).
+The ensemble, serving and CPU legs follow.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs one case
 per GPU with no sharding.
@@ -30,6 +35,8 @@ METRIC = "Mpts·steps/s at 256³-class, mixed vs FP64; ensemble p50/p99 step lat
 TUNE_FILE = os.path.join(ROOT, "lesgo_amd", "tuned_gfx950.json")
 TUNE_MODE = "auto"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 WORKLOADS = {
     # name: (nx, ny, Nz, dt, seed, turbines)
@@ -82,39 +89,73 @@ def prepare_tuning(c):
     return "live"
 
 
-def time_precision(name, precision, steps, warmup, spinup=None):
-    cfg, c = make_case(name, precision)
-    npts = cfg.nx * cfg.ny * cfg.Nz
-    tuning = prepare_tuning(c)
-    c.step_graph(cfg.dyn_init if spinup is None else spinup)   # untimed spin-up
-    c.step(warmup)
+def step_figures(name, precision, ms_per_step):
+    """Whole-step roofline: the algorithmic HBM bytes and FFT flops of one
+    step (compulsory traffic with plane-local fusion and the one z barrier of
+    the pressure tridiagonal: 31.8 full-field passes, FP64 254.4 B/pt,
+    mixed 27.8 passes at 4 B + 4 at 8 B = 143.2 B/pt; flops
+    N [54.8 x 2.5 log2(nx ny) + 9 x 2.25 x 2.5 log2(nx2 ny2)]) against the
+    HBM and vector peaks."""
+    import math
+    nx, ny, Nz = WORKLOADS[name][:3]
+    npts = nx * ny * Nz
+    b_alg = npts * (254.4 if precision == "fp64" else 143.2)
+    f_alg = npts * (54.8 * 2.5 * math.log2(nx * ny)
+                    + 9 * 2.25 * 2.5 * math.log2(1.5 * nx * 1.5 * ny))
+    peak_f = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
+    t = ms_per_step * 1e-3
+    hbm = b_alg / (HBM_PEAK_GBS * 1e9 * t)
+    flop = f_alg / (peak_f * 1e12 * t)
+    return {"B_alg_bytes": b_alg, "hbm_frac": hbm, "F_alg_flop": f_alg,
+            "flop_frac": flop, "peak_tflops": peak_f, "frac": max(hbm, flop)}
+
+
+def timed_passes(name, precision, c, steps, warmup):
+    """From the case's current state: warm-up, then three passes of `steps`
+    steps each: graph-replayed with no per-launch events (the headline),
+    eager with no events, and eager with HIP events around every launch of
+    the transform and tridiagonal kernel classes (the per-kernel figures)."""
+    nx, ny, Nz = WORKLOADS[name][:3]
+    npts = nx * ny * Nz
+    c.step_graph(warmup)
     c.synchronize()
+    t0 = time.perf_counter()
+    c.step_graph(steps)
+    c.synchronize()
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.synchronize()
+    t_eager = time.perf_counter() - t0
     c.timing(True)
     c.synchronize()
     t0 = time.perf_counter()
     c.step(steps)
     c.synchronize()
-    t1 = time.perf_counter()
+    t_events = time.perf_counter() - t0
     rep = c.timing_report()
     c.timing(False)
     rms = c.rmsdiv()
     ke = c.energy()
-    c.close()
     if not (ke == ke and ke < 1e6):
         raise RuntimeError(f"{name} {precision}: the run diverged (energy {ke})")
-    dt = t1 - t0
+    ms = 1e3 * t_graph / steps
     res = {
-        "value": npts * steps / dt / 1e6,
-        "ms_per_step": 1e3 * dt / steps,
-        "tuning": tuning,
+        "value": npts * steps / t_graph / 1e6,
+        "ms_per_step": ms,
+        "eager_value": npts * steps / t_eager / 1e6,
+        "eager_ms_per_step": 1e3 * t_eager / steps,
+        "events_ms_per_step": 1e3 * t_events / steps,
+        "kernel_classes_ms_per_step": sum(v[0] for v in rep.values()) / steps,
         "rmsdiv": rms,
         "energy": ke,
         "kernels": {k: {"ms_total": v[0], "launches": v[1], "gbytes": v[2] / 1e9}
                     for k, v in rep.items()},
+        "step_roofline": step_figures(name, precision, ms),
     }
     dom = max(rep.items(), key=lambda kv: kv[1][0])
-    ms, nl, by = dom[1]
-    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    kms, nl, by = dom[1]
+    achieved = by / nl / (kms / nl * 1e-3) / 1e9 if kms > 0 else 0.0
     res["roofline"] = {
         "kernel": dom[0],
         "bound": "hbm",
@@ -123,11 +164,18 @@ def time_precision(name, precision, steps, warmup, spinup=None):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(name, precision, dom[0]),
+        "alg_bytes_per_launch": by / nl if nl else 0.0,
         "launches": nl,
-        "avg_launch_us": 1e3 * ms / nl if nl else 0.0,
-        "share_of_step": ms / (1e3 * dt),
+        "avg_launch_us": 1e3 * kms / nl if nl else 0.0,
+        "share_of_step": kms / (1e3 * t_events),
     }
     return res
+
+
+def mixed_vs_fp64(c64, c32, Nz):
+    """Relative L2 of the mixed case's velocity against the FP64 case's on
+    the device (u, v on levels 1..Nz, w on 1..Nz+1, x < nx)."""
+    return {f: c32.diff_l2(c64, f, 1, Nz if f != "w" else Nz + 1) for f in ("u", "v", "w")}
 
 
 ENSEMBLE = dict(members=48, nx=96, ny=96, Nz=64, dt=2e-4, steps=200, dia=0.15, hub=0.1,
@@ -219,59 +267,62 @@ def ensemble_leg(steps):
     }
 
 
-def serving_leg(minutes):
+def serving_leg(minutes, saturation_jobs):
     """The serving loop on a Poisson stream (2 jobs/s, grids {64^3, 96x48x64,
     128x64x96, 128^3}, 200-2000 steps, z0 log-uniform, 30 % with a 2x2 disk
-    array, mixed precision, 200 GB budget), drained; with the time the same
-    jobs would take one after another in plain FP64."""
+    array, mixed precision, 200 GB budget), drained; then the same arrival
+    process served one job at a time in plain FP64 (first in first out,
+    measured), and the first `saturation_jobs` jobs submitted at once (the
+    sustained job rate)."""
     from lesgo_amd import serve
     jobs = serve.job_stream(minutes, 2.0, seed=0)
     res = serve.run_stream(jobs, 200.0, 16, progress=log)
-    ser = serve.fp64_serial_estimate(jobs)
-    res["fp64_serial_s"] = ser["seconds"]
-    res["fp64_serial_Mpts_steps_s"] = ser["Mpts_steps_s"]
-    res["speedup_vs_fp64_serial"] = ser["seconds"] / res["wall_s"]
+    log(f"[bench] served: job p50 {res['job_latency_s_p50']:.2f} s; serial FP64 replay")
+    ser = serve.run_serial_fp64(jobs, progress=log)
+    res["fp64_serial"] = ser
+    res["job_latency_p50_serial_over_served"] = ser["job_latency_s_p50"] / res["job_latency_s_p50"]
+    res["job_latency_p99_serial_over_served"] = ser["job_latency_s_p99"] / res["job_latency_s_p99"]
+    if saturation_jobs > 0:
+        log(f"[bench] saturation: {saturation_jobs} jobs at once")
+        res["saturation"] = serve.saturation(jobs[:saturation_jobs], 200.0, 16, progress=log)
     return res
 
 
-def cpu_baseline(name):
-    """The numpy oracle on this host: one plain step of the full workload,
-    scaled by the dynamic/plain step-time ratio measured at 1/8 the points
-    (every 5th step runs the Lagrangian update)."""
-    import numpy as np  # noqa: F401
+CPU_SAMPLE_NZ = 48
 
+
+def cpu_baseline(name):
+    """The numpy oracle on this host's cores, timed over five consecutive
+    steps of a bounded sample of the workload: the same horizontal grid,
+    disks and level spacing with the lowest CPU_SAMPLE_NZ of its levels
+    (Lz scaled with them, so dz is the workload's), run as a fresh start
+    with DYN_init = 5 so that the five steps are four Smagorinsky steps and
+    one Lagrangian dynamic step, the workload's 1-in-cs_count cadence.
+    Every x-y plane costs the same, so Mpts*steps/s carries over."""
     from oracle import lesgo_oracle as o
     nx, ny, Nz, dt, seed, turb = WORKLOADS[name]
-
-    def steps_time(nx_, ny_, Nz_, with_dyn):
-        p = o.Params(nx_, ny_, Nz_, dt=dt)
-        sp = o.Spectral(p)
-        s = o.initial_condition(p, seed)
-        if turb:
-            s.turbines = o.Turbines(p, o.aligned_farm(p))
-        s.jt_total = 3
-        t0 = time.perf_counter()
+    nzs = min(Nz, CPU_SAMPLE_NZ)
+    p = o.Params(nx, ny, nzs, lz=nzs / Nz, dt=dt, initu=False, dyn_init=5)
+    sp = o.Spectral(p)
+    s = o.initial_condition(p, seed)
+    if turb:
+        s.turbines = o.Turbines(p, o.aligned_farm(p))
+    t0 = time.perf_counter()
+    per = []
+    for _ in range(5):
+        a = time.perf_counter()
         o.step(sp, s)
-        t_plain = time.perf_counter() - t0
-        t_dyn = None
-        if with_dyn:
-            t0 = time.perf_counter()
-            o.step(sp, s)
-            t_dyn = time.perf_counter() - t0
-        return t_plain, t_dyn
-
-    sp_, sd_ = steps_time(nx // 2, ny // 2, Nz // 2, True)
-    ratio = sd_ / sp_
-    tp, _ = steps_time(nx, ny, Nz, False)
-    t_avg = tp * (4 + ratio) / 5
+        per.append(time.perf_counter() - a)
+    t = time.perf_counter() - t0
     return {
-        "value": nx * ny * Nz / t_avg / 1e6,
+        "value": nx * ny * nzs * 5 / t / 1e6,
         "unit": "Mpts*steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"numpy oracle, 1 plain step at {nx}x{ny}x{Nz} ({tp:.1f} s) scaled by the "
-                   f"dynamic/plain ratio {ratio:.2f} measured at {nx//2}x{ny//2}x{Nz//2} "
-                   f"(1 of every 5 steps is dynamic)"),
+        "sample": (f"numpy FP64 oracle (pocketfft, one thread), 5 consecutive steps "
+                   f"({', '.join(f'{x:.1f}' for x in per)} s; the 5th runs the Lagrangian "
+                   f"model) of {nx}x{ny}x{nzs}: the workload's grid, disks and dz on its "
+                   f"lowest {nzs} levels"),
     }
 
 
@@ -287,6 +338,9 @@ def main():
                     help="steps of the 48-member ensemble leg (0 skips it)")
     ap.add_argument("--serve-minutes", type=float, default=1.0,
                     help="length of the Poisson job stream of the serving leg (0 skips it)")
+    ap.add_argument("--saturation-jobs", type=int, default=40,
+                    help="jobs of the stream submitted at once for the serving loop's "
+                         "sustained rate (0 skips it)")
     ap.add_argument("--tune", default="auto", choices=["auto", "tune", "off"],
                     help="launch choices: recorded file if present (auto), tune now, or "
                          "built-in heuristics")
@@ -327,16 +381,29 @@ def main():
                                    "run with --gpus 1 (multi-GPU scaling is not built)"}))
         sys.exit(2)
     nx, ny, Nz, dt, seed, turb = WORKLOADS[args.workload]
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    cases = {}
+    for prec in precs:
+        cfg, cases[prec] = make_case(args.workload, prec)
+    tuning = prepare_tuning(cases[precs[0]])
+    spin = cfg.dyn_init if args.spinup is None else args.spinup
+    for prec in precs:
+        log(f"[bench] {prec} {args.workload}: {spin} untimed spin-up steps")
+        cases[prec].step_graph(spin)
+        cases[prec].synchronize()
+    accuracy = None
+    if len(precs) == 2:
+        accuracy = {"after_steps": spin,
+                    "vel_rel_l2": mixed_vs_fp64(cases["fp64"], cases["mixed"], Nz)}
+        log(f"[bench] mixed vs fp64 after {spin} steps: {accuracy['vel_rel_l2']}")
     out = {}
-    if args.precision in ("both", "fp64"):
-        log(f"[bench] fp64 {args.workload}: warmup {args.warmup}, steps {args.steps}")
-        out["fp64"] = time_precision(args.workload, "fp64", args.steps, args.warmup, args.spinup)
-        log(f"[bench] fp64 done: {out['fp64']['value']:.1f} Mpts*steps/s")
-    if args.precision in ("both", "mixed"):
-        log(f"[bench] mixed {args.workload}: warmup {args.warmup}, steps {args.steps}")
-        out["mixed"] = time_precision(args.workload, "mixed", args.steps, args.warmup,
-                                      args.spinup)
-        log(f"[bench] mixed done: {out['mixed']['value']:.1f} Mpts*steps/s")
+    for prec in precs:
+        log(f"[bench] {prec}: warmup {args.warmup}, 3 x {args.steps} timed steps")
+        out[prec] = timed_passes(args.workload, prec, cases[prec], args.steps, args.warmup)
+        out[prec]["tuning"] = tuning
+        cases[prec].close()
+        log(f"[bench] {prec} done: {out[prec]['value']:.1f} Mpts*steps/s graph-replayed, "
+            f"{out[prec]['eager_value']:.1f} eager")
     head = out.get("mixed") or out["fp64"]
     line = {
         "metric": METRIC,
@@ -354,14 +421,27 @@ def main():
         "config": {
             "workload": (f"{args.workload}: {nx}x{ny}x{Nz}, Lx=2pi Ly=pi Lz=1, dt={dt}, "
                          "Lagrangian scale-dependent SGS, equilibrium wall z0=1e-4"
-                         + (", 4x4 ADM D=0.1 hub=0.1 7Dx5D Ct'=1.33" if turb else "")),
+                         + (", 4x4 ADM D=0.1 hub=0.1 7Dx5D Ct'=1.33" if turb else "")
+                         + f"; fresh start (Euler step 1, Cs^2=0.03 until DYN_init="
+                         f"{cfg.dyn_init}, then the dynamic model every 5 steps), {spin} "
+                         f"untimed spin-up steps, {args.warmup} warm-up steps, timed steps "
+                         "graph-replayed (lg_step_graph) with no per-launch events"),
             "grid": [nx, ny, Nz],
             "precision_of_value": "mixed" if "mixed" in out else "fp64",
         },
         "roofline": {k: v for k, v in head["roofline"].items()},
+        "step_roofline": head["step_roofline"],
+        "eager_value": head["eager_value"],
+        "timing_ms_per_step": {k: head[k] for k in ("ms_per_step", "eager_ms_per_step",
+                                                    "events_ms_per_step",
+                                                    "kernel_classes_ms_per_step")},
     }
+    if accuracy is not None:
+        line["mixed_vs_fp64"] = accuracy
     if "fp64" in out:
-        line["fp64"] = {k: out["fp64"][k] for k in ("value", "ms_per_step", "roofline")}
+        line["fp64"] = {k: out["fp64"][k] for k in ("value", "ms_per_step", "eager_value",
+                                                    "eager_ms_per_step", "roofline",
+                                                    "step_roofline")}
     if "fp64" in out and "mixed" in out:
         line["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     line["kernels"] = head["kernels"]
@@ -372,11 +452,11 @@ def main():
             f"{line['ensemble']['step_ms_p50']:.2f} ms p99 {line['ensemble']['step_ms_p99']:.2f} ms")
     if args.serve_minutes > 0:
         log(f"[bench] serving: {args.serve_minutes} min Poisson stream at 2 jobs/s")
-        line["serving"] = serving_leg(args.serve_minutes)
+        line["serving"] = serving_leg(args.serve_minutes, args.saturation_jobs)
         log(f"[bench] serving done: job p50 {line['serving']['job_latency_s_p50']:.2f} s "
             f"p99 {line['serving']['job_latency_s_p99']:.2f} s")
     if not args.no_cpu:
-        log("[bench] cpu baseline (numpy oracle)")
+        log(f"[bench] cpu baseline (numpy oracle, {CPU_SAMPLE_NZ} levels, 5 steps)")
         line["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(line), flush=True)
 

@@ -129,6 +129,10 @@ int lg_filt_da(lg_case* c, int f, int dfdx, int dfdy);
 int lg_ddx(lg_case* c, int f, int dfdx);
 int lg_ddy(lg_case* c, int f, int dfdy);
 int lg_ddxy(lg_case* c, int f, int dfdx, int dfdy);
+/* test_filtermodule::test_filter / test_test_filter (test_filtermodule.f90:
+ * 126-169) on one host plane f(ld, ny) in place: width 2 applies G_test (the
+ * 2-delta cutoff), width 4 G_test_test; the padding columns come back 0. */
+int lg_test_filter(lg_case* c, int width, double* plane);
 /* derivatives::ddz_uv / ddz_w (derivatives.f90:214-311) */
 int lg_ddz_uv(lg_case* c, int f, int dfdz);
 int lg_ddz_w(lg_case* c, int f, int dfdz);
@@ -150,6 +154,22 @@ int lg_convec(lg_case* c);
 int lg_press_stag_array(lg_case* c);
 /* forcing::project (forcing.f90:149-244) */
 int lg_project(lg_case* c);
+/* tridag_array(a, b, c, r, u) (tridag_array.f90:165-246) on its own: the
+ * batched Thomas solve for every (jx < lh, jy) with real coefficients a, b,
+ * c (lh, ny, nz+1) and interleaved complex r, u (ld = 2 lh, ny, nz+1), all
+ * FP64 in the reference's column-major layout (C [nz+1][ny][lh] and
+ * [nz+1][ny][ld]).  Row 1 is solved for every system; rows 2..nz+1 skip
+ * jy = ny/2+1 and (jx, jy) = (1, 1), whose entries of u keep their values,
+ * as does column jx = lh.  A zero pivot returns LG_ERR_PIVOT naming jx, jy
+ * and the row (the reference prints and stops, 190-193, 217-222).  nzp1 is
+ * nz+1.  lg_tridag_array takes host arrays; lg_tridag_array_device takes
+ * device arrays and a hipStream_t (NULL: the default stream), allocates the
+ * Thomas scratch gam (lh ny (nz+1) doubles) for the call and waits for the
+ * stream before it returns. */
+int lg_tridag_array(int lh, int ny, int nzp1, const double* a, const double* b, const double* c,
+                    const double* r, double* u);
+int lg_tridag_array_device(int lh, int ny, int nzp1, const double* a, const double* b,
+                           const double* c, const double* r, double* u, void* stream);
 /* Inline blocks of the time loop in main.f90, exposed so a host driver can
  * mirror the loop call by call:
  *   lg_copy_field        RHSx_f = RHSx etc. (main.f90:155-157)
@@ -211,16 +231,6 @@ int lg_timing_report(lg_case* c, int kind, double* ms, double* launches, double*
  * times: calibrates HBM byte counters and measures copy bandwidth.  No
  * counterpart in the reference. */
 int lg_calibrate_copy(long nbytes, int reps, double* ms_per_copy);
-/* Device facts (hipGetDeviceProperties of the current device): out[0] LDS
- * bytes per workgroup, [1] LDS bytes per CU, [2] CUs, [3] L2 bytes, [4] max
- * threads per block, [5] wavefront size, [6] memory clock kHz, [7] memory bus
- * width bits.  n >= 8. */
-int lg_device_info(long* out, int n);
-/* Bandwidth probes over nbytes (multiple of 16), mean ms of reps launches:
- * 1 copy one float4 per thread, 2 copy four float4 per thread, 3 as 2 with
- * non-temporal loads/stores, 4 read only, 5 write only.  No counterpart in
- * the reference. */
-int lg_bw_probe(int variant, long nbytes, int reps, double* ms);
 
 /* --- batches of independent cases (no counterpart in the reference, which
  * runs one case per process).  A batch holds `members` cases of one grid,
@@ -290,6 +300,9 @@ int lg_stats_enable(lg_case* c, int nstart, int nend, int nskip);
 /* total_time: the averaging time so far (member 0's with use_cfl_dt, where
  * every member weighs by its own dt) */
 int lg_stats_info(lg_case* c, double* total_time, int* initialized);
+/* The averaging time of member m (tavg%total_time, time_average.f90:665-716;
+ * with use_cfl_dt each member weighs by its own dt). */
+int lg_stats_time(lg_case* c, int m, double* total_time);
 /* Time average of one quantity of member m, host double [nz+1][ny][nx]. */
 int lg_stats_mean(lg_case* c, int m, int quantity, double* host);
 /* The raw time-weighted sum behind lg_stats_mean (the arrays tavg.out holds,

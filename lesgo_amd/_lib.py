@@ -63,6 +63,10 @@ SIGNATURES = {
     "lg_press_stag_array": (C.c_int, [_P]),
     "lg_project": (C.c_int, [_P]),
     "lg_copy_field": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lg_test_filter": (C.c_int, [_P, C.c_int, _D]),
+    "lg_tridag_array": (C.c_int, [C.c_int, C.c_int, C.c_int, _D, _D, _D, _D, _D]),
+    "lg_tridag_array_device": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lg_rhs_ab2": (C.c_int, [_P]),
     "lg_rhs_gradp": (C.c_int, [_P]),
     "lg_step": (C.c_int, [_P, C.c_int]),
@@ -110,14 +114,13 @@ SIGNATURES = {
     "lg_case_bytes": (C.c_int, [C.POINTER(LgParams), C.c_int, _D]),
     "lg_stats_enable": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "lg_stats_info": (C.c_int, [_P, _D, _I]),
+    "lg_stats_time": (C.c_int, [_P, C.c_int, _D]),
     "lg_stats_mean": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "lg_stats_sums": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "lg_stats_restore": (C.c_int, [_P, C.c_int, C.c_double, _D]),
     "lg_stats_spectrum": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "lg_diff_l2": (C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
     "lg_mem_info": (C.c_int, [_D, _D]),
-    "lg_device_info": (C.c_int, [C.POINTER(C.c_long), C.c_int]),
-    "lg_bw_probe": (C.c_int, [C.c_int, C.c_long, C.c_int, _D]),
     "lg_server_create": (C.c_int, [C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "lg_server_destroy": (C.c_int, [_P]),
     "lg_server_now": (C.c_int, [_P, _D]),

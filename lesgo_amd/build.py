@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblesgo_amd.so")
 SOURCES = ["case.hip", "passes_f64.hip", "passes_f32.hip", "tune.hip", "turbines.hip", "capi.hip",
-           "server.hip", "stats.hip", "probe.hip"]
+           "server.hip", "stats.hip", "calibrate.hip", "tridag.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -34,24 +34,28 @@ def _compile(src: str) -> str:
 
 
 FSRC = os.path.join(HERE, "fortran")
-FORTRAN = ["lesgo_amd_c.f90", "lesgo_state.f90", "lesgo_gpu.f90", "main.f90"]
+FORTRAN_LIB = ["lesgo_amd_c.f90", "lesgo_state.f90", "lesgo_gpu.f90"]
+# executables: the time-loop driver and the drop-in check program
+PROGRAMS = {"lesgo_amd_main": "main.f90", "lesgo_amd_dropins": "dropins.f90"}
 DRIVER = os.path.join(HERE, "lesgo_amd_main")
 FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 
 
 def build_driver(verbose: bool = True) -> str:
-    """Fortran driver (amdflang) linked against liblesgo_amd.so."""
-    mod = os.path.join(OBJ, "fmod")
-    os.makedirs(mod, exist_ok=True)
-    cmd = [FC, "-O2", "-module-dir", mod, *[os.path.join(FSRC, f) for f in FORTRAN],
-           "-L" + HERE, "-llesgo_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
-           "-o", DRIVER + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
-    if r.returncode != 0:
-        raise RuntimeError(f"amdflang failed:\n{r.stderr[-6000:]}")
-    os.replace(DRIVER + ".tmp", DRIVER)
-    if verbose:
-        print(f"built {DRIVER}")
+    """Fortran programs (amdflang) linked against liblesgo_amd.so."""
+    for exe, main in PROGRAMS.items():
+        mod = os.path.join(OBJ, "fmod_" + exe)
+        os.makedirs(mod, exist_ok=True)
+        out = os.path.join(HERE, exe)
+        srcs = [os.path.join(FSRC, f) for f in FORTRAN_LIB + [main]]
+        cmd = [FC, "-O2", "-module-dir", mod, *srcs, "-L" + HERE, "-llesgo_amd",
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", out + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {main}:\n{r.stderr[-6000:]}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print(f"built {out}")
     return DRIVER
 
 

@@ -175,6 +175,15 @@ class Case:
     def ddxy(self, f, dfdx, dfdy):
         check(lib().lg_ddxy(self._h, _fid(f), _fid(dfdx), _fid(dfdy)), "ddxy")
 
+    def test_filter(self, plane, width: int = 2):
+        """test_filter (width 2) or test_test_filter (width 4) of one
+        (ny, ld) plane; returns the filtered plane."""
+        a = np.array(plane, dtype=np.float64, copy=True, order="C")
+        if a.shape != (self.ny, self.ld):
+            raise ValueError(f"expected a ({self.ny}, {self.ld}) plane, got {a.shape}")
+        check(lib().lg_test_filter(self._h, width, a.ctypes.data_as(_lib._D)), "test_filter")
+        return a
+
     def ddz_uv(self, f, dfdz):
         check(lib().lg_ddz_uv(self._h, _fid(f), _fid(dfdz)), "ddz_uv")
 
@@ -269,6 +278,12 @@ class Case:
         t, i = C.c_double(), C.c_int()
         check(lib().lg_stats_info(self._h, C.byref(t), C.byref(i)), "stats_info")
         return t.value, bool(i.value)
+
+    def stats_time(self, member: int = 0) -> float:
+        """The averaging time of a member (each member's own with use_cfl_dt)."""
+        t = C.c_double()
+        check(lib().lg_stats_time(self._h, member, C.byref(t)), "stats_time")
+        return t.value
 
     def stats_mean(self, name: str, member: int = 0):
         """Time average of a tavg quantity, [nz+1, ny, nx] float64."""

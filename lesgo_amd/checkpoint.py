@@ -69,8 +69,11 @@ def write_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = 
     parts = [np.ascontiguousarray(case.download(n, member)[1:nz + 1]) for n in CHECKPOINT_FIELDS]
     with open(os.path.join(path, fname), "wb") as f:
         write_record(f, b"".join(a.astype("<f8").tobytes() for a in parts))
+    # io.f90:1220 writes the reference's Nz, which is param's nz (Fortran
+    # names ignore case): nz_tot/nproc + 1 = the input level count + 1 on one
+    # process (input_util.f90:196-201), the value initial.f90:211-214 checks
     with open(os.path.join(path, "grid.out"), "wb") as f:
-        write_record(f, struct.pack("<4i3d", 1, cfg.nx, cfg.ny, cfg.Nz, cfg.lx, cfg.ly, cfg.lz))
+        write_record(f, struct.pack("<4i3d", 1, cfg.nx, cfg.ny, nz, cfg.lx, cfg.ly, cfg.lz))
     # the cfl written is the target with use_cfl_dt, the measured one
     # otherwise (io.f90:1232-1238)
     clk = case.clock(member)
@@ -127,9 +130,10 @@ TAVG_CHECKPOINT = ("u", "v", "w", "u_w", "v_w", "w_uv", "u2", "v2", "w2", "uv", 
 def write_tavg_checkpoint(case: Case, path: str = ".", member: int = 0, fname: str = "tavg.out"):
     """tavg%checkpoint (time_average.f90:665-716): one record with the
     averaging time, then one per array."""
-    total_time, initialized = case.stats_info()
+    _, initialized = case.stats_info()
     if not initialized:
         return False
+    total_time = case.stats_time(member)
     with open(os.path.join(path, fname), "wb") as f:
         write_record(f, struct.pack("<d", total_time))
         for n in TAVG_CHECKPOINT:

@@ -284,6 +284,32 @@ int lg_ddxy(lg_case* h, int f, int dfdx, int dfdy) {
     });
 }
 
+int lg_test_filter(lg_case* h, int width, double* plane) {
+    return dispatch(h, [&](auto& c) {
+        REQUIRE(width == 2 || width == 4, "test filter width must be 2 or 4 (delta)");
+        REQUIRE(plane, "null host pointer");
+        using T = typename std::remove_reference<decltype(*c.fld[0])>::type;
+        const long n = c.d.plane;
+        std::vector<T> buf((size_t)n);
+        for (long i = 0; i < n; ++i) buf[i] = (T)plane[i];
+        T* dp = nullptr;
+        LG_HIP(hipMalloc(&dp, 2 * n * sizeof(T)));
+        try {
+            LG_HIP(hipMemcpyAsync(dp, buf.data(), n * sizeof(T), hipMemcpyHostToDevice,
+                                  c.stream));
+            c.test_filter_plane(dp, dp + n, width == 2 ? lg::OP_G1 : lg::OP_G2);
+            LG_HIP(hipMemcpyAsync(buf.data(), dp + n, n * sizeof(T), hipMemcpyDeviceToHost,
+                                  c.stream));
+            LG_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            (void)hipFree(dp);
+            throw;
+        }
+        LG_HIP(hipFree(dp));
+        for (long i = 0; i < n; ++i) plane[i] = (double)buf[i];
+    });
+}
+
 int lg_ddz_uv(lg_case* h, int f, int dfdz) {
     return dispatch_mod(h, [&](auto& c) {
         check_field(f);
@@ -588,6 +614,14 @@ int lg_stats_info(lg_case* h, double* total_time, int* initialized) {
     return dispatch(h, [&](auto& c) {
         if (total_time) *total_time = c.stats_time(0);
         if (initialized) *initialized = c.st.initialized ? 1 : 0;
+    });
+}
+
+int lg_stats_time(lg_case* h, int m, double* total_time) {
+    return dispatch(h, [&](auto& c) {
+        check_member(c, m);
+        REQUIRE(total_time, "null output pointer");
+        *total_time = c.stats_time(m);
     });
 }
 

@@ -139,6 +139,8 @@ struct FlagArgs {
     int n;
     int v[FLAGS_ARG_MAX];
 };
+// k_set_flags(int*, FlagArgs) must fit the 4 KB of kernel arguments
+static_assert(sizeof(FlagArgs) + sizeof(int*) <= 4096, "k_set_flags arguments exceed 4 KB");
 
 __global__ void k_set_flags(int* dst, FlagArgs a) {
     for (int i = threadIdx.x; i < a.n; i += blockDim.x) dst[i] = a.v[i];
@@ -1329,10 +1331,13 @@ void Case<T>::step_body() {
 // Per-member flags of the step the counters describe: Euler update and
 // Cs_opt2 = 0.03 at a fresh member's first step; dynamic-model mode (0 not
 // yet, 1 update, 2 first call with F_* initialisation) when jt_total is a
-// multiple of cs_count.  Uploads them if they changed (this synchronises the
-// stream, so it never happens inside a capture: steps that change flags are
-// eager).  Returns true when the step has a first-step or initialisation
-// event.  force_lag: a direct lagrange_Sdep call, which updates every member.
+// multiple of cs_count.  Writes them to the device if they changed: with at
+// most FLAGS_ARG_MAX values (6 per member, M <= 128) by a k_set_flags launch
+// carrying them as kernel arguments, in stream order with no host wait; for
+// more members by a copy that synchronises the stream.  Steps that change
+// flags run eagerly, never inside a capture.  Returns true when the step has
+// a first-step or initialisation event.  force_lag: a direct lagrange_Sdep
+// call, which updates every member.
 template <typename T>
 bool Case<T>::refresh_flags(bool force_lag) {
     const int cs = d.prm.cs_count;

@@ -80,6 +80,40 @@ interface
         type(c_ptr), value :: c
         integer(c_int), value :: f, dfdx, dfdy
     end function
+    integer(c_int) function lg_ddx(c, f, dfdx) bind(C, name='lg_ddx')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+        integer(c_int), value :: f, dfdx
+    end function
+    integer(c_int) function lg_ddy(c, f, dfdy) bind(C, name='lg_ddy')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+        integer(c_int), value :: f, dfdy
+    end function
+    integer(c_int) function lg_ddxy(c, f, dfdx, dfdy) bind(C, name='lg_ddxy')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+        integer(c_int), value :: f, dfdx, dfdy
+    end function
+    integer(c_int) function lg_test_filter(c, width, plane) bind(C, name='lg_test_filter')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c, plane
+        integer(c_int), value :: width
+    end function
+    integer(c_int) function lg_lagrange_sdep(c) bind(C, name='lg_lagrange_sdep')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+    end function
+    integer(c_int) function lg_interpolag_sdep(c) bind(C, name='lg_interpolag_sdep')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: c
+    end function
+    integer(c_int) function lg_tridag_array(lh, ny, nzp1, a, b, c, r, u)                 &
+        bind(C, name='lg_tridag_array')
+        import :: c_int, c_ptr
+        integer(c_int), value :: lh, ny, nzp1
+        type(c_ptr), value :: a, b, c, r, u
+    end function
     integer(c_int) function lg_ddz_uv(c, f, dfdz) bind(C, name='lg_ddz_uv')
         import :: c_int, c_ptr
         type(c_ptr), value :: c

@@ -59,7 +59,7 @@ save
 real(rprec), dimension(:,:,:), allocatable, target :: u, v, w,                 &
     dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz,                      &
     RHSx, RHSy, RHSz, RHSx_f, RHSy_f, RHSz_f, dpdx, dpdy, dpdz,                &
-    txx, txy, tyy, txz, tyz, tzz, divtx, divty, divtz, p
+    txx, txy, tyy, txz, tyz, tzz, divtx, divty, divtz, p, fxa, fya, fza
 contains
 subroutine sim_param_init()
 use param, only : ld, ny, nz, lbz
@@ -74,23 +74,32 @@ allocate(txx(ld,ny,lbz:nz), txy(ld,ny,lbz:nz), tyy(ld,ny,lbz:nz))
 allocate(txz(ld,ny,lbz:nz), tyz(ld,ny,lbz:nz), tzz(ld,ny,lbz:nz))
 allocate(divtx(ld,ny,lbz:nz), divty(ld,ny,lbz:nz), divtz(ld,ny,lbz:nz))
 allocate(p(ld,ny,0:nz))
-u = 0; v = 0; w = 0; p = 0
+allocate(fxa(ld,ny,lbz:nz), fya(ld,ny,lbz:nz), fza(ld,ny,lbz:nz))
+u = 0; v = 0; w = 0; p = 0; fxa = 0; fya = 0; fza = 0
 end subroutine sim_param_init
 end module sim_param
 
 !*******************************************************************************
 module sgs_param
 !*******************************************************************************
-! Host mirrors of the checkpointed sgs_param arrays (sgs_param.f90:29-184).
+! Host mirrors of the sgs_param arrays (sgs_param.f90:29-184) that the
+! device case holds, all (ld, ny, lbz:nz) like the C ABI's fields (the
+! reference allocates S_ij, Nu_t and Cs_opt2 from level 1; level 0 is
+! unused there).
 use types, only : rprec
 implicit none
 save
-real(rprec), dimension(:,:,:), allocatable, target :: Cs_opt2, F_LM, F_MM, F_QN, F_NN
+real(rprec), dimension(:,:,:), allocatable, target :: Cs_opt2, F_LM, F_MM, F_QN, F_NN, &
+    S11, S12, S13, S22, S23, S33, Nu_t, Beta, Tn_all
 contains
 subroutine sgs_param_init()
 use param, only : ld, ny, nz, lbz
 allocate(Cs_opt2(ld,ny,lbz:nz), F_LM(ld,ny,lbz:nz), F_MM(ld,ny,lbz:nz))
 allocate(F_QN(ld,ny,lbz:nz), F_NN(ld,ny,lbz:nz))
+allocate(S11(ld,ny,lbz:nz), S12(ld,ny,lbz:nz), S13(ld,ny,lbz:nz))
+allocate(S22(ld,ny,lbz:nz), S23(ld,ny,lbz:nz), S33(ld,ny,lbz:nz))
+allocate(Nu_t(ld,ny,lbz:nz), Beta(ld,ny,lbz:nz), Tn_all(ld,ny,lbz:nz))
 Cs_opt2 = 0; F_LM = 0; F_MM = 0; F_QN = 0; F_NN = 0
+S11 = 0; S12 = 0; S13 = 0; S22 = 0; S23 = 0; S33 = 0; Nu_t = 0; Beta = 0; Tn_all = 0
 end subroutine sgs_param_init
 end module sgs_param

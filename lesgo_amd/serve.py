@@ -9,8 +9,9 @@ has its own seed, z0 log-uniform in [1e-5, 1e-3], and a 30 % chance of a
 2x2 aligned actuator-disk array (D = 0.15, hub 0.1, 7D x 5D, Ct' = 1.33).
 Jobs run in mixed precision.  Prints one JSON line with p50/p99 job
 completion latency, p50/p99 per-step (tick) latency and aggregate
-throughput, beside the time the same jobs would take run one after another
-in plain FP64 (per-grid FP64 step times measured here).
+throughput; with --serial beside the same arrival process served one job at
+a time in plain FP64 (measured), with --saturation N the sustained job rate
+of the first N jobs submitted at once.
 """
 from __future__ import annotations
 
@@ -172,6 +173,57 @@ def run_stream(jobs, budget_gb=200.0, pool_members=16, progress=None, drain=True
     }
 
 
+def run_serial_fp64(jobs, progress=None):
+    """The same arrival process served one job at a time in plain FP64,
+    first in first out: job i starts at max(its arrival, the previous job's
+    completion) on a case of its own (created, seeded, graph-replayed for
+    its steps, destroyed), as a host loop without batching would run it.
+    Returns the latency distribution and the wall time, measured."""
+    lat, busy = [], 0.0
+    t_start = time.perf_counter()
+    t_last = t_start
+    for n, j in enumerate(jobs):
+        now = time.perf_counter() - t_start
+        if now < j["t"]:
+            time.sleep(j["t"] - now)
+        cfg = job_config(j, precision="fp64")
+        a = time.perf_counter()
+        with Case(cfg) as c:
+            c.seed_state(j["seed"])
+            disks = job_disks(cfg, j["disks"])
+            if disks:
+                c.turbines_init(disks)
+            c.set_counters(0, 0)
+            c.step_graph(j["nsteps"])
+            c.synchronize()
+        b = time.perf_counter()
+        busy += b - a
+        lat.append(b - t_start - j["t"])
+        if progress and b - t_last > 20.0:
+            t_last = b
+            progress(f"[serial fp64] t={b - t_start:.0f}s done={n + 1}/{len(jobs)}")
+    wall = time.perf_counter() - t_start
+    pts = sum(j["grid"][0] * j["grid"][1] * j["grid"][2] * j["nsteps"] for j in jobs)
+    lat = np.array(lat)
+    return {"wall_s": wall, "busy_s": busy,
+            "job_latency_s_p50": float(np.percentile(lat, 50)),
+            "job_latency_s_p99": float(np.percentile(lat, 99)),
+            "throughput_Mpts_steps_s": pts / wall / 1e6,
+            "busy_Mpts_steps_s": pts / busy / 1e6}
+
+
+def saturation(jobs, budget_gb=200.0, pool_members=16, progress=None):
+    """Every job of `jobs` submitted at once: the server's sustained job
+    rate (jobs/s) and throughput for this job mix, i.e. the arrival rate
+    beyond which its queue grows without bound."""
+    burst = [dict(j, t=0.0) for j in jobs]
+    res = run_stream(burst, budget_gb, pool_members, progress=progress)
+    return {"jobs": len(jobs), "wall_s": res["wall_s"],
+            "jobs_per_s": len(jobs) / res["wall_s"],
+            "throughput_Mpts_steps_s": res["throughput_Mpts_steps_s"],
+            "step_ms_p50": res["step_ms_p50"], "step_ms_p99": res["step_ms_p99"]}
+
+
 def fp64_serial_estimate(jobs):
     """Seconds the jobs would take one after another in plain FP64: per grid,
     the FP64 step time before DYN_init (Smagorinsky phase) and after it
@@ -212,16 +264,27 @@ def main(argv=None):
     ap.add_argument("--pool-members", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-steps", type=int, default=2000)
+    ap.add_argument("--serial", action="store_true",
+                    help="also replay the stream one job at a time in plain FP64 (measured)")
+    ap.add_argument("--saturation", type=int, default=0, metavar="NJOBS",
+                    help="also submit the first NJOBS jobs at once (sustained job rate)")
     args = ap.parse_args(argv)
     jobs = job_stream(args.minutes, args.rate, args.seed, steps=(200, args.max_steps))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     log(f"[serve] {len(jobs)} jobs over {args.minutes} min at {args.rate}/s")
     res = run_stream(jobs, args.budget_gb, args.pool_members, progress=log)
-    ser = fp64_serial_estimate(jobs)
-    res["fp64_serial_s"] = ser["seconds"]
-    res["fp64_serial_Mpts_steps_s"] = ser["Mpts_steps_s"]
-    res["fp64_step_ms"] = ser["fp64_step_ms"]
-    res["speedup_vs_fp64_serial"] = ser["seconds"] / res["wall_s"]
+    if args.serial:
+        ser = run_serial_fp64(jobs, progress=log)
+        res["fp64_serial"] = ser
+        res["job_latency_p50_ratio_serial_over_served"] = (ser["job_latency_s_p50"]
+                                                           / res["job_latency_s_p50"])
+    else:
+        ser = fp64_serial_estimate(jobs)
+        res["fp64_serial_estimate_s"] = ser["seconds"]
+        res["fp64_step_ms"] = ser["fp64_step_ms"]
+    if args.saturation:
+        res["saturation"] = saturation(jobs[:args.saturation], args.budget_gb,
+                                       args.pool_members, progress=log)
     print(json.dumps(res), flush=True)
 
 

@@ -87,6 +87,9 @@ class _StatsStub:
     def stats_info(self):
         return 0.375, True
 
+    def stats_time(self, member=0):
+        return 0.375 + member
+
     def stats_sums(self, name, member=0):
         q = ck.TAVG_CHECKPOINT.index(name) if name in ck.TAVG_CHECKPOINT else 99
         return np.arange(5 * 5 * 6, dtype=float).reshape(5, 5, 6) + 1000.0 * q
@@ -120,3 +123,65 @@ def test_tavg_checkpoint_layout_and_round_trip(tmp_path):
     for n in ck.TAVG_CHECKPOINT:
         np.testing.assert_array_equal(sums[n], c.stats_sums(n))
     assert not ck.read_tavg_checkpoint(c, str(tmp_path / "none"))
+
+
+GRID_PROG = """
+program grid
+implicit none
+! the reference's param declares Nz only; nz in the code is the same name
+! (Fortran ignores case), set to nz_tot/nproc + 1 (input_util.f90:196-201)
+integer :: Nx = 8, Ny = 6, Nz = 4
+real(8) :: L_x = 6.283185307179586d0, L_y = 3.141592653589793d0, L_z = 1d0
+nz = Nz + 1
+open(11, file='grid.out', form='unformatted', status='replace')
+write(11) 1, Nx, Ny, Nz, L_x, L_y, L_z
+close(11)
+end program grid
+"""
+
+
+class _CaseStub:
+    """The part of a Case the vel.out / grid.out / total_time.dat writer uses."""
+
+    def __init__(self, nx, ny, Nz):
+        from lesgo_amd.case import Config
+        self.cfg = Config(nx, ny, Nz)
+        self.nz, self.ny, self.ld = Nz + 1, ny, nx + 2
+
+    def resync(self):
+        pass
+
+    def counters(self):
+        return 7, 7
+
+    def download(self, name, member=0):
+        return np.zeros((self.nz + 1, self.ny, self.ld))
+
+    def clock(self, member=0):
+        return {"total_time": 0.25, "dt": 2e-4}
+
+    def max_cfl(self, member=0):
+        return 0.1
+
+
+@pytest.mark.skipif(not os.path.exists(FC), reason="no Fortran compiler")
+def test_grid_out_holds_the_references_nz(tmp_path):
+    """grid.out (io.f90:1220) holds (nproc, Nx, Ny, Nz, L_x, L_y, L_z) with
+    the reference's Nz, i.e. param's nz = the input level count + 1 on one
+    process; the bytes equal those of the Fortran statement compiled with
+    the same case-insensitive names."""
+    src = tmp_path / "grid.f90"
+    src.write_text(GRID_PROG)
+    d = tmp_path / "f"
+    d.mkdir()
+    subprocess.run([FC, "-O1", str(src), "-o", str(d / "grid")], check=True, cwd=d,
+                   capture_output=True, timeout=120)
+    subprocess.run([str(d / "grid")], check=True, cwd=d, capture_output=True, timeout=60)
+    py = tmp_path / "py"
+    py.mkdir()
+    ck.write_checkpoint(_CaseStub(8, 6, 4), str(py))
+    with open(py / "grid.out", "rb") as f:
+        rec = ck.read_record(f)
+    import struct
+    assert struct.unpack("<4i3d", rec)[:4] == (1, 8, 6, 5)
+    assert (py / "grid.out").read_bytes() == (d / "grid.out").read_bytes()

@@ -137,6 +137,35 @@ def test_cfl_statistics_weights():
         assert e <= 1e-10, e
 
 
+def test_cfl_tavg_checkpoint_takes_the_members_own_time(tmp_path):
+    """With use_cfl_dt every member of a batch averages over its own dt
+    sequence: tavg.out of member 1 carries member 1's averaging time (the
+    same as the case run alone), not member 0's."""
+    import struct
+
+    import lesgo_amd
+    from lesgo_amd.checkpoint import read_record, write_tavg_checkpoint
+    cfg = lesgo_amd.Config(32, 32, 16, use_cfl_dt=True, cfl=0.05)
+    with lesgo_amd.Case(cfg, members=2) as c:
+        c.seed_state(3, 0.1, member=0)
+        c.seed_state(9, 0.3, member=1)
+        c.set_counters(0, 0)
+        c.stats_enable(2, 12, 1)
+        c.step(12)
+        t0, t1 = c.stats_time(0), c.stats_time(1)
+        assert t0 != t1
+        assert write_tavg_checkpoint(c, str(tmp_path), member=1)
+    with open(tmp_path / "tavg.out", "rb") as f:
+        (t_file,) = struct.unpack("<d", read_record(f))
+    assert t_file == t1
+    with lesgo_amd.Case(cfg) as a:
+        a.seed_state(9, 0.3, member=0)
+        a.set_counters(0, 0)
+        a.stats_enable(2, 12, 1)
+        a.step(12)
+        assert a.stats_time(0) == t1 == a.stats_info()[0]
+
+
 def test_cfl_mixed_precision_dt_close():
     """Mixed precision takes nearly the FP64 dt sequence.  dt follows the
     velocity maximum at one point: the two runs agree to ~3e-7 until the flow

@@ -48,6 +48,9 @@ def test_config0_100_steps_fp64():
     errs, s, _ = run_pair(p, 100, 10)
     worst = max(max(e[k] for k in ("u", "v", "w", "p")) for e in errs)
     assert worst <= 1e-10, worst
+    # rmsdiv every 10 steps (wbase = 10): roundoff of the projection in both
+    for e in errs:
+        assert abs(e["rmsdiv_gpu"] - e["rmsdiv_ref"]) <= 1e-10 * e["rmsdiv_ref"] + 1e-12, e
     assert errs[-1]["rmsdiv_gpu"] < 1e-12
 
 
