@@ -170,6 +170,16 @@ int lg_tridag_array(int lh, int ny, int nzp1, const double* a, const double* b, 
                     const double* r, double* u);
 int lg_tridag_array_device(int lh, int ny, int nzp1, const double* a, const double* b,
                            const double* c, const double* r, double* u, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lg_tridag_array_device with the same arguments and
+ * results, its scratch (gam and the status words) in a static device array
+ * of the library instead of a per-call allocation.  Limit: lh * ny * nzp1
+ * <= 2^23 (64 MiB, held on each GPU the library runs on until the process
+ * ends); larger sizes return LG_ERR_ARG.  Calls that overlap on one GPU
+ * share that array and race. */
+int lg_tridag_array_device_noalloc(int lh, int ny, int nzp1, const double* a, const double* b,
+                                   const double* c, const double* r, double* u, void* stream);
 /* Inline blocks of the time loop in main.f90, exposed so a host driver can
  * mirror the loop call by call:
  *   lg_copy_field        RHSx_f = RHSx etc. (main.f90:155-157)

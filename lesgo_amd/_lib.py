@@ -67,6 +67,9 @@ SIGNATURES = {
     "lg_tridag_array": (C.c_int, [C.c_int, C.c_int, C.c_int, _D, _D, _D, _D, _D]),
     "lg_tridag_array_device": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lg_tridag_array_device_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]),
     "lg_rhs_ab2": (C.c_int, [_P]),
     "lg_rhs_gradp": (C.c_int, [_P]),
     "lg_step": (C.c_int, [_P, C.c_int]),

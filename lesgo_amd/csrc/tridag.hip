@@ -5,6 +5,7 @@
 // (C order [nz+1][ny][lh] and [nz+1][ny][ld]).  The time step itself uses
 // the fused k_tridag of pressure.hpp, which assembles the same coefficients
 // from the wavenumbers; this is the drop-in for callers that pass their own.
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -145,6 +146,79 @@ extern "C" int lg_tridag_array_device(int lh, int ny, int nzp1, const double* a,
     (void)hipFree(gam);
     (void)hipFree(status);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of lg_tridag_array_device.  Its scratch (the status
+// words, then gam) is one file-scope device array, so a call makes no
+// hipMalloc/hipFree.  Its limit: lh * ny * (nz+1) <= TRI_NOALLOC_MAX_POINTS
+// (2^23 forward-sweep ratios, 64 MiB, held on every GPU the library runs on
+// until the process ends; the 256 x 256 x 192 grid needs 129 * 256 * 194 =
+// 6.4 M).  Every call on one GPU uses that GPU's copy of the array: calls
+// that overlap on one GPU (other host threads, other streams) race.
+namespace lg {
+constexpr long TRI_NOALLOC_MAX_POINTS = 1L << 23;
+constexpr long TRI_SCRATCH_HEAD = TRI_STATUS_INTS / 2;   // status words, in doubles
+constexpr int TRI_MAX_DEVICES = 64;
+}  // namespace lg
+
+static __device__ __align__(256) double scratch[lg::TRI_SCRATCH_HEAD + lg::TRI_NOALLOC_MAX_POINTS];
+
+namespace {
+// Device address of `scratch` per device ordinal, looked up once per device
+// with that device current (the array's host-side name is not a device
+// address and is never passed to a kernel or a copy).
+std::mutex g_scratch_mu;
+double* g_scratch_dev[lg::TRI_MAX_DEVICES] = {};
+
+double* scratch_on_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= lg::TRI_MAX_DEVICES) {
+        lg::set_error("tridag_array (no-alloc): no usable current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            lg::set_error(std::string("tridag_array (no-alloc): hipGetSymbolAddress failed: ") +
+                          hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
+// The same solve as lg_tridag_array_device (the same kernels) with its
+// scratch in the library's static device array; sizes above the limit are
+// refused with LG_ERR_ARG.  The status words are cleared by a kernel at the
+// start of every call; gam needs no clearing (every entry the backward sweep
+// reads is written by the same call's forward sweep).
+extern "C" int lg_tridag_array_device_noalloc(int lh, int ny, int nzp1, const double* a,
+                                              const double* b, const double* c, const double* r,
+                                              double* u, void* stream) {
+    if (!lg::tridag_array_args(lh, ny, nzp1, a, b, c, r, u)) return LG_ERR_ARG;
+    const long ng = (long)lh * ny * nzp1;
+    if (ng > lg::TRI_NOALLOC_MAX_POINTS) {
+        lg::set_error("tridag_array (no-alloc): lh * ny * (nz+1) = " + std::to_string(ng) +
+                      " exceeds the static scratch of " +
+                      std::to_string(lg::TRI_NOALLOC_MAX_POINTS) +
+                      " points; use lg_tridag_array_device");
+        return LG_ERR_ARG;
+    }
+    double* base = scratch_on_current_device();
+    if (!base) return LG_ERR_HIP;
+    int* status = reinterpret_cast<int*>(base);
+    double* gam = base + lg::TRI_SCRATCH_HEAD;
+    try {
+        return lg::tridag_array_run(lh, ny, nzp1, a, b, c, r, u, gam, status,
+                                    reinterpret_cast<hipStream_t>(stream));
+    } catch (const lg::HipError&) {
+        return LG_ERR_HIP;
+    }
 }
 
 // Host arrays in and out, the reference's own call (tridag_array(a, b, c,
