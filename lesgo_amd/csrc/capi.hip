@@ -436,6 +436,14 @@ int lg_turbines_init(lg_case* h, int nturb, const double* spec8, double alpha1, 
 int lg_turbines_forcing(lg_case* h) {
     return dispatch_mod(h, [&](auto& c) {
         REQUIRE(c.turb, "turbines not initialised");
+        // fxa, fya, fza are written on the disk nodes only: a buffer that
+        // still holds forcing from before a disk-set change is cleared first
+        if (c.fclear_pending > 0) {
+            for (int id : {LG_FXA, LG_FYA, LG_FZA})
+                LG_HIP(hipMemsetAsync(c.fld[id], 0, c.M * c.d.field * sizeof(*c.fld[id]),
+                                      c.stream));
+            c.fclear_pending = 0;
+        }
         c.turbines_forcing();
     });
 }

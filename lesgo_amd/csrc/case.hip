@@ -119,6 +119,7 @@ TurbineSet::~TurbineSet() {
     if (center_lin) (void)hipFree(center_lin);
     if (center_wlin) (void)hipFree(center_wlin);
     if (center) (void)hipFree(center);
+    if (fz_uv) (void)hipFree(fz_uv);
 }
 
 // ---------------------------------------------------------------------------
@@ -196,6 +197,7 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     dflags_now.assign(6 * M, 0);
     d_flags = dalloc<int>(6L * M);
     mcold.assign(M, 1);
+    rhsf_phys.assign(M, 1);
     cfl_mode = p.use_cfl_dt != 0;
     d_clock = dalloc<double>((long)CK_N * M);
     d_cmax = dalloc<unsigned long long>(3L * M);
@@ -1529,7 +1531,15 @@ void Case<T>::swap_rhs() {
     std::swap(fld[LG_TZZ], tprev[2]);
     const bool rhs_stale = (stale & SG_RHS) != 0;
     stale = (stale & ~(unsigned)(SG_RHS | SG_RHSF)) | (rhs_stale ? (unsigned)SG_RHSF : 0u);
-    rhsf_input = !rhs_stale;
+    for (int m = 0; m < M; ++m) rhsf_phys[m] = mcold[m];
+    forced_prev = forced_last;
+    forced_last = turb && turb->total_nodes > 0 && kf1 >= kf0;
+    // the buffer now written holds the forcing from before a disk-set
+    // change (zero off the old nodes, not the new ones): cleared once the
+    // step it fed has been made physical and left behind
+    if (fclear_pending > 0 && --fclear_pending == 0)
+        for (int id : {LG_FXA, LG_FYA, LG_FZA})
+            LG_HIP(hipMemsetAsync(fld[id], 0, M * d.field * sizeof(T), stream));
 }
 
 template <typename T>
@@ -1601,7 +1611,6 @@ void Case<T>::materialize(unsigned groups) {
     groups &= stale;
     if (!groups) return;
     const int nz = d.nz, n1 = nz - 1;
-    const bool tb = turb && turb->total_nodes > 0;
     // gc: this step's remainder; gq: the previous step's (or the high part of
     // a physical RHS_f); hs: the stress high parts for divt
     T* gc[3] = {tmp[0], tmp[1], tmp[2]};
@@ -1612,7 +1621,7 @@ void Case<T>::materialize(unsigned groups) {
     if (groups & need_g) {
         T* f[3] = {fld[LG_FXA], fld[LG_FYA], fld[LG_FZA]};
         T* t[3] = {fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ]};
-        rhs_high(tb ? f : nullptr, t, gc, hs);
+        rhs_high(forced_last ? f : nullptr, t, gc, hs);
     }
     if (groups & SG_DIVT) {
         const cplx<T>* is[3] = {sD[0], sD[1], sD[2]};
@@ -1630,12 +1639,30 @@ void Case<T>::materialize(unsigned groups) {
                      {LG_DIVTY, nz, BOGUS}, {LG_DIVTZ, 0, BOGUS}});
     }
     if (groups & (SG_UVW | SG_RHSF)) {
-        if (rhsf_input) {
-            T* f[3] = {fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F]};
-            high_part(f, gq, 3, 1, n1);
-        } else {
+        // per member: the step before's stresses and forcing, or, for a
+        // member whose last step started from its physical fields, the high
+        // part of its physical RHS_f (selected by the step's cold flags)
+        int nphys = 0;
+        for (int m = 0; m < Ma; ++m) nphys += rhsf_phys[m] ? 1 : 0;
+        if (nphys < Ma) {
             T* t[3] = {tprev[0], tprev[1], tprev[2]};
-            rhs_high(tb ? fprev : nullptr, t, gq, hs);
+            rhs_high(forced_prev ? fprev : nullptr, t, gq, hs);
+        }
+        if (nphys > 0) {
+            T* f[3] = {fld[LG_RHSX_F], fld[LG_RHSY_F], fld[LG_RHSZ_F]};
+            high_part(f, nphys == Ma ? gq : hs, 3, 1, n1);
+            if (nphys < Ma) {
+                // whole fields as complex pairs (d.field is even)
+                SelArgs<T> sa{};
+                for (int q = 0; q < 3; ++q) {
+                    sa.src[q] = reinterpret_cast<const cplx<T>*>(hs[q]);
+                    sa.dst[q] = reinterpret_cast<cplx<T>*>(gq[q]);
+                }
+                const long n = d.field / 2;
+                hipLaunchKernelGGL(k_select_members<T>,
+                                   dim3((unsigned)((n + BLK - 1) / BLK), Ma, 3), dim3(BLK), 0,
+                                   stream, sa, (const int*)(d_flags + 5 * M), n, n);
+            }
         }
         have_gq = true;
     }

@@ -64,6 +64,10 @@ struct TurbineSet {
     int* off = nullptr;              // n+1 offsets
     int* node_tid = nullptr;         // turbine of each node
     bool has_fz = false;
+    // with tilted or yawed-up disks: fza on the uv grid before its move to
+    // the w grid, one field per member, zero off the nodes (its own buffer:
+    // the step's scratch fields are reused between steps)
+    void* fz_uv = nullptr;
     double* coef = nullptr;          // per turbine: TC_N coefficients
     double* state = nullptr;         // per turbine: u_d, u_d_T, f_n
     long* center_lin = nullptr;      // per turbine: the centre cell, and the
@@ -123,7 +127,16 @@ struct Case {
     // keeps the 42 filtered fields
     bool lag_fused = true;
     std::vector<char> mcold;
-    bool rhsf_input = false;         // the physical RHS_f was valid when the last step began
+    // members whose last step started from their physical fields (their
+    // RHS_f high part is in the physical RHS_f only); the others take it
+    // from the step before's stresses and forcing (tprev, fprev), so what a
+    // member's fields read back as never depends on earlier reads
+    std::vector<char> rhsf_phys;
+    bool forced_last = false;        // the last step wrote disk forcing
+    bool forced_prev = false;        // the step before it did
+    // steps until the forcing buffer that held the forcing of the step
+    // before a disk-set change becomes the write target and is cleared
+    int fclear_pending = 0;
     unsigned stale = 0;
     cplx<T>* sR[3] = {};             // RHS spectra (after - grad p)
     cplx<T>* sRf[3] = {};            // RHS_f spectra

@@ -325,11 +325,19 @@ void Case<T>::rebuild_turbines() {
         }
         ts->first.push_back(s);
     }
-    for (int id : {LG_FXA, LG_FYA, LG_FZA})
-        LG_HIP(hipMemsetAsync(fld[id], 0, M * d.field * sizeof(T), stream));
-    for (int q = 0; q < 3; ++q)
-        if (fprev[q]) LG_HIP(hipMemsetAsync(fprev[q], 0, M * d.field * sizeof(T), stream));
-    LG_HIP(hipMemsetAsync(tmp[8], 0, M * d.field * sizeof(T), stream));
+    // The forcing is written on the nodes only, so the buffer the next step
+    // writes is cleared here.  In the spectral step that is fprev (the
+    // buffers exchange at the step's start); fxa/fya/fza keep the last
+    // step's forcing, whose high part the next step's physical fields
+    // still carry, and are cleared when they next become the write target.
+    if (spec_on) {
+        for (int q = 0; q < 3; ++q)
+            LG_HIP(hipMemsetAsync(fprev[q], 0, M * d.field * sizeof(T), stream));
+        fclear_pending = 2;
+    } else {
+        for (int id : {LG_FXA, LG_FYA, LG_FZA})
+            LG_HIP(hipMemsetAsync(fld[id], 0, M * d.field * sizeof(T), stream));
+    }
     LG_HIP(hipStreamSynchronize(stream));
     // levels where the forcing can be non-zero: the nodes' levels and, for
     // fza on the w grid, one above
@@ -357,6 +365,11 @@ void Case<T>::rebuild_turbines() {
     up(ts->center_lin, clin);
     up(ts->center_wlin, cwlin);
     up(ts->center, std::vector<double>(3 * (size_t)s, 0.0));
+    if (ts->has_fz) {
+        LG_HIP(hipMalloc(&ts->fz_uv, M * d.field * sizeof(T)));
+        LG_HIP(hipMemsetAsync(ts->fz_uv, 0, M * d.field * sizeof(T), stream));
+        LG_HIP(hipStreamSynchronize(stream));
+    }
     turb = std::move(ts);
 }
 
@@ -372,7 +385,7 @@ void Case<T>::turbines_forcing() {
                        TurbFilter{t_avg_dim, z_i, u_star, d.prm.dt, clk()},
                        TurbCenter{t.center_lin, t.center_wlin, t.center});
     // the node set is static, so fxa/fya/fza are zero off the nodes
-    T* fz_uv = t.has_fz ? tmp[8] : fld[LG_FZA];
+    T* fz_uv = t.has_fz ? static_cast<T*>(t.fz_uv) : fld[LG_FZA];
     hipLaunchKernelGGL(k_turb_scatter<T>, dim3((t.total_nodes + 255) / 256), dim3(256), 0, stream,
                        fld[LG_FXA], fld[LG_FYA], fz_uv, (const long*)t.node_lin,
                        (const double*)t.node_ind, (const int*)t.node_tid, (const double*)t.coef,
@@ -380,7 +393,7 @@ void Case<T>::turbines_forcing() {
     if (t.has_fz) {
         Geo g{d.nx, d.ny, d.nz, d.ld, d.plane, 1, d.nz, d.field};
         hipLaunchKernelGGL(k_interp_w<T>, dim3((unsigned)((d.ld * d.ny + 255) / 256), d.nz * Ma),
-                           dim3(256), 0, stream, (const T*)tmp[8], fld[LG_FZA], g);
+                           dim3(256), 0, stream, (const T*)fz_uv, fld[LG_FZA], g);
     }
 }
 
