@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "fft3.hpp"
+#include "fft4.hpp"
 #include "lag_rows.hpp"
 #include "passes.hpp"
 
@@ -91,6 +92,32 @@ static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const Col
     return false;
 }
 
+// The inverse column pass specialised by index map (fft4.hpp); false when
+// the split or the map has no instance (the caller launches k_cols_v2).
+template <typename T>
+static bool v4_cols_inv(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
+    const int N = n1 * n2;
+    const size_t l4 = lds + (size_t)N * sizeof(cplx<T>);
+    if (l4 > LDS_MAX) return false;
+    if (a.map == MAP_ID && a.ny == N) {
+#define LG_CASE(A, B)                                                                    \
+        if (n1 == A && n2 == B) {                                                        \
+            LG_LAUNCH((k_cols_inv_v4<T, A, B, MAP_ID>), g, dim3(PBLK), l4, s, a);        \
+            return true;                                                                 \
+        }
+        LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    } else if (a.map == MAP_PAD && 2 * N == 3 * a.ny) {
+#define LG_CASE(A, B)                                                                    \
+        if (n1 == A && n2 == B) {                                                        \
+            LG_LAUNCH((k_cols_inv_v4<T, A, B, MAP_PAD>), g, dim3(PBLK), l4, s, a);       \
+            return true;                                                                 \
+        }
+        LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    }
+    return false;
+}
 
 // Is the choice launchable: LDS within bounds, and for the four-step r2c
 // pass every row pair's first-step tasks in one round of the block
@@ -105,7 +132,9 @@ bool choice_ok(int kind, const PassChoice& c, int N, int other) {
         for (auto& p : splits_for(N)) known |= (p.first == c.n1 && p.second == c.n2);
         if (!known) return false;
     }
-    if (c.v != 0 && !(c.v == 1 && c.n1 && (kind == PK_R2C || kind == PK_C2R))) return false;
+    if (c.v != 0 && !(c.v == 1 && c.n1 && (kind == PK_R2C || kind == PK_C2R)) &&
+        !(c.v == 2 && c.n1 && (kind == PK_COLS_INV || kind == PK_COLS_PAD)))
+        return false;
     // real rows of a length-N row pass have stride N + 2; ncols <= N / 2
     const size_t lds = row_lds_bytes<T>(c, N, N + 2, N / 2);
     if (lds > LDS_MAX) return false;
@@ -279,6 +308,8 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
     dim3 grid(nj, (cmax + c.TC - 1) / c.TC, nlev * M);
     const double bytes = (double)nlev * M * sizeof(cplx<T>) * cols_rw;
     tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
+        if (DIR > 0 && ch.n1 && ch.v != 2 && v4_cols_inv<T>(ch.n1, ch.n2, grid, lds, s, c))
+            return;
         if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
             LG_LAUNCH((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);
     });
