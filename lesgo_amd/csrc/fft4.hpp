@@ -216,4 +216,175 @@ __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Inverse row pass (paired rows) specialised by band: the same four-step
+// transform and arithmetic as k_rows_c2r_v2, with the Hermitian extension's
+// cases (DC, below / above the Nyquist column, outside the stored band)
+// resolved per unrolled element at compile time wherever the element's
+// columns lie on one side, 32-bit offsets from a block-uniform base and the
+// twiddles in LDS.  BAND: 0 every stored column (ncols = N/2), 3 the 3/2
+// grid's band (ncols = N/3), 1 a per-job band nce < ncols.
+// ---------------------------------------------------------------------------
+enum : int { BAND_FULL = 0, BAND_JOB = 1, BAND_THIRD = 3 };
+
+template <typename T, int N1, int N2, int BAND>
+__global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    constexpr int half = N / 2;
+    extern __shared__ __align__(16) unsigned char smem_r4[];
+    cplx<T>* ltw = reinterpret_cast<cplx<T>*>(smem_r4);
+    cplx<T>* lds = ltw + N;
+    const int job = blockIdx.z;
+    const int mem = blockIdx.y / a.nlev;
+    const int plane = blockIdx.y - mem * a.nlev;
+    const int npairs = a.nrows / 2;
+    const int pair0 = blockIdx.x * a.S;
+    const int S = min(a.S, npairs - pair0);
+    const unsigned nc = (unsigned)a.ncols;
+    const int ld = a.dst_ld;
+    // bases at the block's first row
+    const cplx<T>* sb = a.src[job] + mem * a.src_mstride +
+                        (long)(a.lev0_src + plane) * a.src_plane + (long)(2 * pair0) * nc;
+    const long dofs = mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane +
+                      (long)(2 * pair0) * ld;
+    T* const ob1 = a.dst[job] + dofs;
+    T* const ob2 = a.dst2[job] ? a.dst2[job] + dofs : nullptr;
+    const int nce = BAND == BAND_JOB ? (a.nce[job] ? a.nce[job] : (int)nc)
+                                     : (BAND == BAND_THIRD ? N / 3 : half);
+    {
+        const auto* tw = plan_tw<T>(a.plan);
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const auto w = tw[i];
+            ltw[i] = cplx<T>{(T)w.x, (T)w.y};
+        }
+    }
+    __syncthreads();
+    for (int pass = 0; pass < (ob2 ? 2 : 1); ++pass) {
+        T* const out = pass ? ob2 : ob1;
+        for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
+            const int s = t / N2, n2 = t - s * N2;
+            const unsigned ra = 2u * (unsigned)s * nc;          // row a, in elements
+            auto ld2 = [&](int col, cplx<T>& A, cplx<T>& B) {
+                const unsigned o = (ra + (unsigned)col) * (unsigned)sizeof(cplx<T>);
+                A = at_bytes(sb, o);
+                B = at_bytes(sb, o + nc * (unsigned)sizeof(cplx<T>));
+                if (pass) {
+                    const T kx = a.kxs * (T)col;
+                    A = cplx<T>{-A.y * kx, A.x * kx};
+                    B = cplx<T>{-B.y * kx, B.x * kx};
+                }
+            };
+            cplx<T> x[N1];
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int k = n2 + N2 * n1;
+                // (lo, hi: the element's column range over the tasks; constant
+                // once unrolled)
+                const int lo = N2 * n1, hi = N2 * n1 + N2 - 1;
+                cplx<T> z{(T)0, (T)0};
+                auto below = [&](int kk) {        // 0 < kk < half
+                    const bool in = (BAND == BAND_FULL) || kk < nce;
+                    if (in) {
+                        cplx<T> A, B;
+                        ld2(kk, A, B);
+                        z = cplx<T>{A.x - B.y, A.y + B.x};
+                    }
+                };
+                auto above = [&](int kk) {        // kk = N - k, 0 < kk < half
+                    const bool in = (BAND == BAND_FULL) || kk < nce;
+                    if (in) {
+                        cplx<T> A, B;
+                        ld2(kk, A, B);
+                        z = cplx<T>{A.x + B.y, B.x - A.y};
+                    }
+                };
+                if (hi < half) {
+                    if (lo == 0 && k == 0) {
+                        if (!pass) {
+                            const cplx<T> A = at_bytes(sb, ra * (unsigned)sizeof(cplx<T>));
+                            const cplx<T> B = at_bytes(sb, (ra + nc) * (unsigned)sizeof(cplx<T>));
+                            z = cplx<T>{A.x, B.x};
+                        }
+                    } else {
+                        below(k);
+                    }
+                } else if (lo > half) {
+                    above(N - k);
+                } else {
+                    if (k < half) below(k);
+                    else if (k > half) above(N - k);
+                }
+                x[n1] = z;
+            }
+            dft_reg<T, N1, +1>(x);
+            cplx<T>* Y = lds + s * SP::YS;
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) {
+                cplx<T> v = x[k1];
+                if (k1 != 0 && n2 != 0) {
+                    const cplx<T> w = ltw[n2 * k1];
+                    v = cmul(v, cplx<T>{w.x, (T)(-1) * w.y});
+                }
+                Y[k1 * SP::YR + n2] = v;
+            }
+        }
+        __syncthreads();
+        bool staged = false;
+        if constexpr (N2 <= 16) staged = S * N1 <= (int)blockDim.x;
+        if (staged) {
+            const int t = threadIdx.x;
+            const bool act = t < S * N1;
+            const int s = act ? t / N1 : 0, k1 = act ? t - s * N1 : 0;
+            cplx<T> y[N2];
+            if (act) {
+                const cplx<T>* Y = lds + s * SP::YS + k1 * SP::YR;
+#pragma unroll
+                for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+                dft_reg<T, N2, +1>(y);
+            }
+            __syncthreads();
+            if (act) {
+                cplx<T>* X = lds + s * SP::YS;
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) X[k1 + N1 * k2] = y[k2];
+            }
+            __syncthreads();
+            constexpr int NP = N / 2;
+            for (int idx = threadIdx.x; idx < S * NP; idx += blockDim.x) {
+                const int s2 = idx / NP, q = idx - s2 * NP;
+                const cplx<T>* X = lds + s2 * SP::YS + 2 * q;
+                const cplx<T> p0 = X[0], p1 = X[1];
+                const unsigned o = 2u * (unsigned)s2 * (unsigned)ld + 2u * (unsigned)q;
+                st_pair2(out + o, p0.x, p1.x);
+                st_pair2(out + o + ld, p0.y, p1.y);
+            }
+        } else {
+            for (int t = threadIdx.x; t < S * N1; t += blockDim.x) {
+                const int s = t / N1, k1 = t - s * N1;
+                const cplx<T>* Y = lds + s * SP::YS + k1 * SP::YR;
+                cplx<T> y[N2];
+#pragma unroll
+                for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+                dft_reg<T, N2, +1>(y);
+                const unsigned o = 2u * (unsigned)s * (unsigned)ld + (unsigned)k1;
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) {
+                    out[o + N1 * k2] = y[k2].x;
+                    out[o + ld + N1 * k2] = y[k2].y;
+                }
+            }
+        }
+        // padding columns N..ld-1
+        const int npad = ld - N;
+        for (int idx = threadIdx.x; idx < S * npad; idx += blockDim.x) {
+            const int s = idx / npad, q = idx - s * npad;
+            const unsigned o = 2u * (unsigned)s * (unsigned)ld + (unsigned)(N + q);
+            out[o] = (T)0;
+            out[o + ld] = (T)0;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace lg

@@ -80,6 +80,41 @@ static bool v3_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC
     return false;
 }
 
+// The inverse row pass specialised by band (fft4.hpp); band: BAND_FULL,
+// BAND_THIRD or BAND_JOB.
+template <typename T, int A, int B>
+static void launch_c2r_v4(int band, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+    if constexpr ((A * B) % 3 == 0) {
+        if (band == BAND_THIRD) {
+            LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_THIRD>), g, dim3(PBLK), lds, s, a);
+            return;
+        }
+    }
+    if (band == BAND_FULL) LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_FULL>), g, dim3(PBLK), lds, s, a);
+    else LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_JOB>), g, dim3(PBLK), lds, s, a);
+}
+
+template <typename T>
+static bool v4_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+    const int N = n1 * n2;
+    const size_t l4 = lds + (size_t)N * sizeof(cplx<T>);
+    if (l4 > LDS_MAX) return false;
+    bool per_job = false;
+    for (int i = 0; i < (int)g.z; ++i) per_job |= a.nce[i] != 0;
+    const int band = per_job ? BAND_JOB
+                             : (2 * a.ncols == N ? BAND_FULL
+                                                 : ((3 * a.ncols == N && N % 3 == 0) ? BAND_THIRD
+                                                                                      : BAND_JOB));
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        launch_c2r_v4<T, A, B>(band, g, l4, s, a);                                       \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
 template <typename T, int DIR>
 static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
@@ -133,7 +168,7 @@ bool choice_ok(int kind, const PassChoice& c, int N, int other) {
         if (!known) return false;
     }
     if (c.v != 0 && !(c.v == 1 && c.n1 && (kind == PK_R2C || kind == PK_C2R)) &&
-        !(c.v == 2 && c.n1 && (kind == PK_COLS_INV || kind == PK_COLS_PAD)))
+        !(c.v == 2 && c.n1 && (kind == PK_COLS_INV || kind == PK_COLS_PAD || kind == PK_C2R)))
         return false;
     // real rows of a length-N row pass have stride N + 2; ncols <= N / 2
     const size_t lds = row_lds_bytes<T>(c, N, N + 2, N / 2);
@@ -275,6 +310,7 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
                              (cin * sizeof(cplx<T>) + (double)nout * dst_ld * sizeof(T));
         tm.run(s, KK_ROWS_C2R, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
+            if (ch.n1 && ch.v == 0 && v4_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (!(ch.n1 && v2_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)))
                 LG_LAUNCH(k_rows_c2r<T>, grid, dim3(PBLK), lds, s, a);
         });
