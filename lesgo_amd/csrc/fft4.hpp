@@ -13,7 +13,8 @@
 //   i kx A + i ky B) is uniform over a block and chosen once per task, not
 //   per element;
 // * addresses are 32-bit byte offsets from a block-uniform base;
-// * the plan's twiddles are read from a copy in LDS.
+// * the plan's twiddles are read from a copy in LDS (TWL) or from the plan's
+//   table in global memory, which leaves the LDS to more resident blocks.
 #pragma once
 
 #include "fft2.hpp"
@@ -28,6 +29,31 @@ __device__ __forceinline__ const cplx<T>& at_bytes(const cplx<T>* base, unsigned
 template <typename T>
 __device__ __forceinline__ void st_bytes(cplx<T>* base, unsigned off, cplx<T> v) {
     *reinterpret_cast<cplx<T>*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Twiddle W_N^e of the plan: from the block's LDS copy (TWL) or the plan's
+// table in global memory (cached; leaves the LDS to more resident blocks).
+template <typename T, bool TWL>
+__device__ __forceinline__ cplx<T> tw_at(const cplx<T>* ltw, const typename TwT<T>::type* gtw,
+                                         int e) {
+    if constexpr (TWL) {
+        return ltw[e];
+    } else {
+        const auto w = gtw[e];
+        return cplx<T>{(T)w.x, (T)w.y};
+    }
+}
+
+// Copies the plan's N twiddles into LDS (TWL) and waits for the block.
+template <typename T, bool TWL>
+__device__ __forceinline__ void tw_stage(cplx<T>* ltw, const typename TwT<T>::type* gtw, int N) {
+    if constexpr (TWL) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const auto w = gtw[i];
+            ltw[i] = cplx<T>{(T)w.x, (T)w.y};
+        }
+        __syncthreads();
+    }
 }
 
 // ky index of small-grid row j (j != ny/2)
@@ -136,13 +162,14 @@ __device__ __forceinline__ void cols_inv_load_divxy(cplx<T> (&x)[N1], const cplx
     }
 }
 
-template <typename T, int N1, int N2, int MAP>
+template <typename T, int N1, int N2, int MAP, bool TWL>
 __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
     using SP = Split<N1, N2>;
     constexpr int N = SP::N;
     extern __shared__ __align__(16) unsigned char smem_c4[];
     cplx<T>* ltw = reinterpret_cast<cplx<T>*>(smem_c4);
-    cplx<T>* lds = ltw + N;
+    cplx<T>* lds = TWL ? ltw + N : ltw;
+    const auto* gtw = plan_tw<T>(a.plan);
     const int job = blockIdx.x;
     const int TC = a.TC;
     const int c0 = blockIdx.y * TC;
@@ -157,14 +184,7 @@ __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
     const int op = a.op[job];
     const int ykeep = MAP == MAP_ID ? a.ykeep[job] : 0;
     const bool nyq0 = MAP == MAP_ID && a.nyq0[job] != 0;
-    {
-        const auto* tw = plan_tw<T>(a.plan);
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            const auto w = tw[i];
-            ltw[i] = cplx<T>{(T)w.x, (T)w.y};
-        }
-    }
-    __syncthreads();
+    tw_stage<T, TWL>(ltw, gtw, N);
     const int CS = SP::YS;
     for (int t = threadIdx.x; t < TC * N2; t += blockDim.x) {
         const int cl = t % TC, n2 = t / TC;
@@ -191,7 +211,7 @@ __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
         for (int k1 = 0; k1 < N1; ++k1) {
             cplx<T> v = x[k1];
             if (k1 != 0 && n2 != 0) {
-                const cplx<T> w = ltw[n2 * k1];
+                const cplx<T> w = tw_at<T, TWL>(ltw, gtw, n2 * k1);
                 v = cmul(v, cplx<T>{w.x, (T)(-1) * w.y});
             }
             Y[k1 * SP::YR + n2] = v;
@@ -227,14 +247,15 @@ __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
 // ---------------------------------------------------------------------------
 enum : int { BAND_FULL = 0, BAND_JOB = 1, BAND_THIRD = 3 };
 
-template <typename T, int N1, int N2, int BAND>
+template <typename T, int N1, int N2, int BAND, bool TWL>
 __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
     using SP = Split<N1, N2>;
     constexpr int N = SP::N;
     constexpr int half = N / 2;
     extern __shared__ __align__(16) unsigned char smem_r4[];
     cplx<T>* ltw = reinterpret_cast<cplx<T>*>(smem_r4);
-    cplx<T>* lds = ltw + N;
+    cplx<T>* lds = TWL ? ltw + N : ltw;
+    const auto* gtw = plan_tw<T>(a.plan);
     const int job = blockIdx.z;
     const int mem = blockIdx.y / a.nlev;
     const int plane = blockIdx.y - mem * a.nlev;
@@ -252,14 +273,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
     T* const ob2 = a.dst2[job] ? a.dst2[job] + dofs : nullptr;
     const int nce = BAND == BAND_JOB ? (a.nce[job] ? a.nce[job] : (int)nc)
                                      : (BAND == BAND_THIRD ? N / 3 : half);
-    {
-        const auto* tw = plan_tw<T>(a.plan);
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            const auto w = tw[i];
-            ltw[i] = cplx<T>{(T)w.x, (T)w.y};
-        }
-    }
-    __syncthreads();
+    tw_stage<T, TWL>(ltw, gtw, N);
     for (int pass = 0; pass < (ob2 ? 2 : 1); ++pass) {
         T* const out = pass ? ob2 : ob1;
         for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
@@ -323,7 +337,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
             for (int k1 = 0; k1 < N1; ++k1) {
                 cplx<T> v = x[k1];
                 if (k1 != 0 && n2 != 0) {
-                    const cplx<T> w = ltw[n2 * k1];
+                    const cplx<T> w = tw_at<T, TWL>(ltw, gtw, n2 * k1);
                     v = cmul(v, cplx<T>{w.x, (T)(-1) * w.y});
                 }
                 Y[k1 * SP::YR + n2] = v;
@@ -384,6 +398,100 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
             out[o + ld] = (T)0;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward column pass (no multiplier, the form every forward pass of the
+// step uses) specialised by index map: MAP_ID (N = ny rows, the ky = ny/2
+// row zeroed when zero_nyq, only rows |m| < ykeep written when ykeep > 0)
+// and MAP_UNPAD (N = 3 ny / 2 rows in, the small grid's rows out, unpadd of
+// fft.f90:69-99, its ky = ny/2 row zero).  The same arithmetic as
+// k_cols_v2<T, N1, N2, -1>.
+// ---------------------------------------------------------------------------
+template <typename T, int N1, int N2, int MAP, bool TWL>
+__global__ void __launch_bounds__(256) k_cols_fwd_v4(ColArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    extern __shared__ __align__(16) unsigned char smem_f4[];
+    cplx<T>* ltw = reinterpret_cast<cplx<T>*>(smem_f4);
+    cplx<T>* lds = TWL ? ltw + N : ltw;
+    const auto* gtw = plan_tw<T>(a.plan);
+    const int job = blockIdx.x;
+    const int TC = a.TC;
+    const int c0 = blockIdx.y * TC;
+    const int nzc = a.nzc[job] ? a.nzc[job] : a.ncols;
+    if (c0 >= nzc) return;
+    const int tc = min(TC, nzc - c0);
+    const int mem = blockIdx.z / a.nlev;
+    const int plane = blockIdx.z - mem * a.nlev;
+    const cplx<T>* sb = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane;
+    cplx<T>* db = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane;
+    const int ykeep = MAP == MAP_ID ? a.ykeep[job] : 0;
+    const bool znyq = a.zero_nyq != 0;
+    const unsigned nc = (unsigned)a.ncols;
+    constexpr unsigned CB = (unsigned)sizeof(cplx<T>);
+    tw_stage<T, TWL>(ltw, gtw, N);
+    const int CS = SP::YS;
+    for (int t = threadIdx.x; t < TC * N2; t += blockDim.x) {
+        const int cl = t % TC, n2 = t / TC;
+        if (cl >= tc) continue;
+        const unsigned cb = (unsigned)(c0 + cl) * CB;
+        cplx<T> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1)
+            x[n1] = at_bytes(sb, (unsigned)(n2 + N2 * n1) * nc * CB + cb);
+        dft_reg<T, N1, -1>(x);
+        cplx<T>* Y = lds + cl * CS;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+            cplx<T> v = x[k1];
+            if (k1 != 0 && n2 != 0) v = cmul(v, tw_at<T, TWL>(ltw, gtw, n2 * k1));
+            Y[k1 * SP::YR + n2] = v;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < TC * N1; t += blockDim.x) {
+        const int cl = t % TC, k1 = t / TC;
+        if (cl >= tc) continue;
+        const unsigned cb = (unsigned)(c0 + cl) * CB;
+        const cplx<T>* Y = lds + cl * CS + k1 * SP::YR;
+        cplx<T> y[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+        dft_reg<T, N2, -1>(y);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int jb = k1 + N1 * k2;
+            // (row range of this element over the tasks: constant once unrolled)
+            const int lo = N1 * k2, hi = N1 * k2 + N1 - 1;
+            cplx<T> v = y[k2];
+            if constexpr (MAP == MAP_ID) {
+                constexpr int NH = N / 2;
+                const bool can_nyq = lo <= NH && NH <= hi;
+                if (can_nyq && znyq && jb == NH) v = cplx<T>{(T)0, (T)0};
+                bool keep = true;
+                if (ykeep) {
+                    const int m = hi < NH ? jb : (lo > NH ? N - jb : (jb < NH ? jb : N - jb));
+                    keep = !(can_nyq && jb == NH) && m < ykeep;
+                }
+                if (keep) st_bytes(db, (unsigned)jb * nc * CB + cb, v);
+            } else {
+                // MAP_UNPAD: rows [0, N/3) -> j = jb, (2N/3, N) -> jb - N/3
+                constexpr int LO = N / 3, HI = 2 * N / 3;
+                int j = -1;
+                if (hi < LO) j = jb;
+                else if (lo > HI) j = jb - LO;
+                else if (!(lo >= LO && hi <= HI)) j = jb < LO ? jb : (jb > HI ? jb - LO : -1);
+                if (j >= 0) st_bytes(db, (unsigned)j * nc * CB + cb, v);
+            }
+        }
+    }
+    if constexpr (MAP == MAP_UNPAD) {
+        // the ky = ny/2 row of the small output is zero (fft.f90:84-86)
+        constexpr int NYH = N / 3;
+        for (int c = threadIdx.x; c < tc; c += blockDim.x)
+            st_bytes(db, (unsigned)NYH * nc * CB + (unsigned)(c0 + c) * CB, cplx<T>{(T)0, (T)0});
     }
 }
 

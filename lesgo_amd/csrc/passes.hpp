@@ -22,9 +22,10 @@ struct PassChoice {
     int n1 = 0, n2 = 0, S = 0;
     int v = 0;          // rows: 1 = the block's rows staged through LDS (fft3.hpp);
                         // inverse rows: 0 = specialised by band (fft4.hpp);
-                        // inverse columns: 0 = specialised by map (fft4.hpp);
+                        // columns: 0 = specialised by map (fft4.hpp);
                         // inverse rows and columns: 2 = the unspecialised
-                        // four-step kernel (fft2.hpp)
+                        // four-step kernel (fft2.hpp), 3 = the specialised
+                        // kernel with twiddles from global memory (0: in LDS)
 };
 
 // Pass kinds of the tuning table (column passes by direction and map).

@@ -14,6 +14,15 @@ namespace lg {
 
 static constexpr int PBLK = 256;
 
+// Block size of a four-step pass with S sequences (rows pairs or columns)
+// per block: its larger phase's task count S max(N1, N2) in whole waves, at
+// most PBLK, so that neither phase leaves most of the block's waves idle
+// (S = 8 at N = 16 x 24: 192 and 128 tasks, 192 threads).
+inline unsigned four_step_block(int S, int n1, int n2) {
+    const int t = ((S * std::max(n1, n2) + 63) / 64) * 64;
+    return (unsigned)std::min(std::max(t, 64), PBLK);
+}
+
 // ---------------------------------------------------------------------------
 // Four-step kernel instances: every (N1, N2) split a transform length may
 // use.  split_of() gives the default; the tuner may pick another.
@@ -36,7 +45,7 @@ template <typename T>
 static bool v2_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        LG_LAUNCH((k_rows_r2c_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        LG_LAUNCH((k_rows_r2c_v2<T, A, B>), g, dim3(four_step_block(a.S, A, B)), lds, s, a);           \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -48,7 +57,7 @@ template <typename T>
 static bool v2_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        LG_LAUNCH((k_rows_c2r_v2<T, A, B>), g, dim3(PBLK), lds, s, a);           \
+        LG_LAUNCH((k_rows_c2r_v2<T, A, B>), g, dim3(four_step_block(a.S, A, B)), lds, s, a);           \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -81,23 +90,25 @@ static bool v3_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC
 }
 
 // The inverse row pass specialised by band (fft4.hpp); band: BAND_FULL,
-// BAND_THIRD or BAND_JOB.
-template <typename T, int A, int B>
+// BAND_THIRD or BAND_JOB; twl: twiddles staged in LDS.
+template <typename T, int A, int B, bool TWL>
 static void launch_c2r_v4(int band, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+    const dim3 blk(four_step_block(a.S, A, B));
     if constexpr ((A * B) % 3 == 0) {
         if (band == BAND_THIRD) {
-            LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_THIRD>), g, dim3(PBLK), lds, s, a);
+            LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_THIRD, TWL>), g, blk, lds, s, a);
             return;
         }
     }
-    if (band == BAND_FULL) LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_FULL>), g, dim3(PBLK), lds, s, a);
-    else LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_JOB>), g, dim3(PBLK), lds, s, a);
+    if (band == BAND_FULL) LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_FULL, TWL>), g, blk, lds, s, a);
+    else LG_LAUNCH((k_rows_c2r_v4<T, A, B, BAND_JOB, TWL>), g, blk, lds, s, a);
 }
 
 template <typename T>
-static bool v4_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC2RArgs<T>& a) {
+static bool v4_c2r(int n1, int n2, bool twl, dim3 g, size_t lds, hipStream_t s,
+                   const RowC2RArgs<T>& a) {
     const int N = n1 * n2;
-    const size_t l4 = lds + (size_t)N * sizeof(cplx<T>);
+    const size_t l4 = lds + (twl ? (size_t)N * sizeof(cplx<T>) : 0);
     if (l4 > LDS_MAX) return false;
     bool per_job = false;
     for (int i = 0; i < (int)g.z; ++i) per_job |= a.nce[i] != 0;
@@ -107,7 +118,8 @@ static bool v4_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC
                                                                                       : BAND_JOB));
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        launch_c2r_v4<T, A, B>(band, g, l4, s, a);                                       \
+        if (twl) launch_c2r_v4<T, A, B, true>(band, g, l4, s, a);                        \
+        else launch_c2r_v4<T, A, B, false>(band, g, l4, s, a);                           \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -119,7 +131,7 @@ template <typename T, int DIR>
 static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
     if (n1 == A && n2 == B) {                                                            \
-        LG_LAUNCH((k_cols_v2<T, A, B, DIR>), g, dim3(PBLK), lds, s, a);          \
+        LG_LAUNCH((k_cols_v2<T, A, B, DIR>), g, dim3(four_step_block(a.TC, A, B)), lds, s, a);          \
         return true;                                                                     \
     }
     LG_SPLITS(LG_CASE)
@@ -129,15 +141,23 @@ static bool v2_cols(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const Col
 
 // The inverse column pass specialised by index map (fft4.hpp); false when
 // the split or the map has no instance (the caller launches k_cols_v2).
+template <typename T, int A, int B, int MAP>
+static void launch_cols_v4(bool twl, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
+    const dim3 blk(four_step_block(a.TC, A, B));
+    if (twl) LG_LAUNCH((k_cols_inv_v4<T, A, B, MAP, true>), g, blk, lds, s, a);
+    else LG_LAUNCH((k_cols_inv_v4<T, A, B, MAP, false>), g, blk, lds, s, a);
+}
+
 template <typename T>
-static bool v4_cols_inv(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
+static bool v4_cols_inv(int n1, int n2, bool twl, dim3 g, size_t lds, hipStream_t s,
+                        const ColArgs<T>& a) {
     const int N = n1 * n2;
-    const size_t l4 = lds + (size_t)N * sizeof(cplx<T>);
+    const size_t l4 = lds + (twl ? (size_t)N * sizeof(cplx<T>) : 0);
     if (l4 > LDS_MAX) return false;
     if (a.map == MAP_ID && a.ny == N) {
 #define LG_CASE(A, B)                                                                    \
         if (n1 == A && n2 == B) {                                                        \
-            LG_LAUNCH((k_cols_inv_v4<T, A, B, MAP_ID>), g, dim3(PBLK), l4, s, a);        \
+            launch_cols_v4<T, A, B, MAP_ID>(twl, g, l4, s, a);                           \
             return true;                                                                 \
         }
         LG_SPLITS(LG_CASE)
@@ -145,7 +165,44 @@ static bool v4_cols_inv(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const
     } else if (a.map == MAP_PAD && 2 * N == 3 * a.ny) {
 #define LG_CASE(A, B)                                                                    \
         if (n1 == A && n2 == B) {                                                        \
-            LG_LAUNCH((k_cols_inv_v4<T, A, B, MAP_PAD>), g, dim3(PBLK), l4, s, a);       \
+            launch_cols_v4<T, A, B, MAP_PAD>(twl, g, l4, s, a);                          \
+            return true;                                                                 \
+        }
+        LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    }
+    return false;
+}
+
+// The forward column pass specialised by index map (fft4.hpp), for jobs
+// without a multiplier; false when the split, the map or an op has none.
+template <typename T, int A, int B, int MAP>
+static void launch_cols_fwd_v4(bool twl, dim3 g, size_t lds, hipStream_t s, const ColArgs<T>& a) {
+    const dim3 blk(four_step_block(a.TC, A, B));
+    if (twl) LG_LAUNCH((k_cols_fwd_v4<T, A, B, MAP, true>), g, blk, lds, s, a);
+    else LG_LAUNCH((k_cols_fwd_v4<T, A, B, MAP, false>), g, blk, lds, s, a);
+}
+
+template <typename T>
+static bool v4_cols_fwd(int n1, int n2, bool twl, dim3 g, size_t lds, hipStream_t s,
+                        const ColArgs<T>& a) {
+    const int N = n1 * n2;
+    for (int i = 0; i < (int)g.x; ++i)
+        if (a.op[i] != OP_NONE) return false;
+    const size_t l4 = lds + (twl ? (size_t)N * sizeof(cplx<T>) : 0);
+    if (l4 > LDS_MAX) return false;
+    if (a.map == MAP_ID && a.ny == N) {
+#define LG_CASE(A, B)                                                                    \
+        if (n1 == A && n2 == B) {                                                        \
+            launch_cols_fwd_v4<T, A, B, MAP_ID>(twl, g, l4, s, a);                       \
+            return true;                                                                 \
+        }
+        LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    } else if (a.map == MAP_UNPAD && 2 * N == 3 * a.ny) {
+#define LG_CASE(A, B)                                                                    \
+        if (n1 == A && n2 == B) {                                                        \
+            launch_cols_fwd_v4<T, A, B, MAP_UNPAD>(twl, g, l4, s, a);                    \
             return true;                                                                 \
         }
         LG_SPLITS(LG_CASE)
@@ -168,7 +225,9 @@ bool choice_ok(int kind, const PassChoice& c, int N, int other) {
         if (!known) return false;
     }
     if (c.v != 0 && !(c.v == 1 && c.n1 && (kind == PK_R2C || kind == PK_C2R)) &&
-        !(c.v == 2 && c.n1 && (kind == PK_COLS_INV || kind == PK_COLS_PAD || kind == PK_C2R)))
+        !((c.v == 2 || c.v == 3) && c.n1 &&
+          (kind == PK_COLS_INV || kind == PK_COLS_PAD || kind == PK_C2R ||
+           kind == PK_COLS_FWD || kind == PK_COLS_UNPAD)))
         return false;
     // real rows of a length-N row pass have stride N + 2; ncols <= N / 2
     const size_t lds = row_lds_bytes<T>(c, N, N + 2, N / 2);
@@ -310,7 +369,9 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
                              (cin * sizeof(cplx<T>) + (double)nout * dst_ld * sizeof(T));
         tm.run(s, KK_ROWS_C2R, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
-            if (ch.n1 && ch.v == 0 && v4_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
+            if (ch.n1 && (ch.v == 0 || ch.v == 3) &&
+                v4_c2r<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, a))
+                return;
             if (!(ch.n1 && v2_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)))
                 LG_LAUNCH(k_rows_c2r<T>, grid, dim3(PBLK), lds, s, a);
         });
@@ -344,7 +405,11 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
     dim3 grid(nj, (cmax + c.TC - 1) / c.TC, nlev * M);
     const double bytes = (double)nlev * M * sizeof(cplx<T>) * cols_rw;
     tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
-        if (DIR > 0 && ch.n1 && ch.v != 2 && v4_cols_inv<T>(ch.n1, ch.n2, grid, lds, s, c))
+        if (DIR > 0 && ch.n1 && (ch.v == 0 || ch.v == 3) &&
+            v4_cols_inv<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, c))
+            return;
+        if (DIR < 0 && ch.n1 && (ch.v == 0 || ch.v == 3) &&
+            v4_cols_fwd<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, c))
             return;
         if (!(ch.n1 && v2_cols<T, DIR>(ch.n1, ch.n2, grid, lds, s, c)))
             LG_LAUNCH((k_cols<T, DIR>), grid, dim3(PBLK), lds, s, c);

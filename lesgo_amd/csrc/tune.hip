@@ -70,12 +70,13 @@ std::vector<PassChoice> candidates(int kind, int N, int other) {
     std::vector<std::pair<int, int>> sp = pass_splits(N);
     sp.insert(sp.begin(), {0, 0});
     const bool rows = kind == PK_R2C || kind == PK_C2R;
-    const bool inv_cols = kind == PK_COLS_INV || kind == PK_COLS_PAD;
+    const bool cols = kind == PK_COLS_INV || kind == PK_COLS_PAD || kind == PK_COLS_FWD ||
+                          kind == PK_COLS_UNPAD;
     for (auto& s : sp)
         for (int S : {1, 2, 4, 8, 12, 16, 24, 32, 48, 64})
-            for (int v : {0, 1, 2}) {
+            for (int v : {0, 1, 2, 3}) {
                 if (v == 1 && !(rows && s.first)) continue;
-                if (v == 2 && !((inv_cols || kind == PK_C2R) && s.first)) continue;
+                if (v >= 2 && !((cols || kind == PK_C2R) && s.first)) continue;
                 PassChoice c{s.first, s.second, S, v};
                 if (choice_ok<T>(kind, c, N, other)) out.push_back(c);
             }

@@ -1,17 +1,19 @@
 """Launch variants of one pass compute the same results.
 
-The inverse column and inverse row passes each have a kernel specialised at
+The column passes and the inverse row pass each have a kernel specialised at
 compile time (by index map and multiplier for columns, by stored band for
-rows; variant 0, the default) and the unspecialised four-step kernel
-(variant 2); both do the same arithmetic.  A 256 x 256 x 24 wind-farm case
+rows; variant 0, the default, and variant 3, the same with the twiddles read
+from global memory instead of an LDS copy) and the unspecialised four-step
+kernel (variant 2); all do the same arithmetic.  A 256 x 256 x 24 wind-farm case
 (disks, fresh start: Euler step, Smagorinsky steps, F_* initialisation and a
 Lagrangian update, so every multiplier, the test-filter bands, the x
 derivative's second output and the 3/2-grid padding run) is stepped with
 each variant of one pass.
 
 Where the specialised kernel sees the same arithmetic graph the results are
-equal bit for bit: the 256-point inverse columns and rows and the (24, 16)
-split of the padded columns.  Where it knows at compile time that some
+equal bit for bit: the 256-point inverse columns and rows, the (24, 16)
+split of the padded columns, the forward columns (plain and unpadding), and
+variants 3 and 0 everywhere.  Where it knows at compile time that some
 inputs are the padding's zeros ((16, 24) padded columns, the 384-point rows
 whose input band is a third of the transform), the compiler folds those
 butterflies and contracts the remaining multiply-adds into FMAs in another
@@ -30,12 +32,20 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "RHSx", "RHSz_f", "Cs_opt2", "F_LM", "F_QN", "txz", "divtx",
           "dudx", "dwdy")
 
-# (pass kind, N, other, n1, n2, per_wg, bit-equal expected)
+# name: ([(pass kind, N, other, n1, n2, per_wg)], (variant a, variant b), bit-equal)
 CASES = {
-    "cols": [("cols_inv", 256, 128, 16, 16, 8, True), ("cols_pad", 384, 128, 24, 16, 8, True)],
-    "cols_pad16x24": [("cols_pad", 384, 128, 16, 24, 8, False)],
-    "rows256": [("c2r", 256, 256, 16, 16, 16, True)],
-    "rows384": [("c2r", 384, 384, 16, 24, 8, False)],
+    "cols": ([("cols_inv", 256, 128, 16, 16, 8), ("cols_pad", 384, 128, 24, 16, 8)], (2, 0),
+             True),
+    "cols_pad16x24": ([("cols_pad", 384, 128, 16, 24, 8)], (2, 0), False),
+    "rows256": ([("c2r", 256, 256, 16, 16, 16)], (2, 0), True),
+    "rows384": ([("c2r", 384, 384, 16, 24, 8)], (2, 0), False),
+    "cols_fwd": ([("cols_fwd", 256, 128, 16, 16, 8), ("cols_unpad", 384, 128, 16, 24, 8)],
+                 (2, 0), True),
+    # twiddles from global memory (3) or the block's LDS copy (0): the same values
+    "twiddles": ([("cols_inv", 256, 128, 16, 16, 8), ("cols_pad", 384, 128, 16, 24, 8),
+                  ("c2r", 256, 256, 16, 16, 16), ("c2r", 384, 384, 16, 24, 8),
+                  ("cols_fwd", 256, 128, 16, 16, 8), ("cols_unpad", 384, 128, 16, 24, 8)],
+                 (3, 0), True),
 }
 
 
@@ -47,10 +57,11 @@ def test_inverse_pass_variants_agree(precision, which):
     cfg = lesgo_amd.Config(256, 256, 24, precision=precision, initu=False, dyn_init=5)
     farm = lesgo_amd.aligned_farm(cfg, 2, 2, dia=0.3, hub=0.3, sx_d=7.0, sy_d=5.0)
     out = {}
+    passes, variants, exact = CASES[which]
     try:
-        for v in (2, 0):
+        for v in variants:
             tune_clear()
-            for kind, N, other, n1, n2, S, _ in CASES[which]:
+            for kind, N, other, n1, n2, S in passes:
                 tune_set(kind, precision, N, other, n1, n2, S, v)
                 assert tune_get(kind, precision, N, other)[3] == v
             with lesgo_amd.Case(cfg) as c:
@@ -60,12 +71,12 @@ def test_inverse_pass_variants_agree(precision, which):
                 out[v] = {f: c.download(f) for f in FIELDS}
     finally:
         tune_clear()
-    exact = all(x[-1] for x in CASES[which])
     for f in FIELDS:
-        a, b = out[2][f][1:24, :, :256], out[0][f][1:24, :, :256]
+        a, b = out[variants[0]][f][1:24, :, :256], out[variants[1]][f][1:24, :, :256]
         d = float(np.max(np.abs(a - b)))
         e = float(np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300))
-        print(f"  {precision} {which} {f}: max |v0 - v2| = {d:.1e}, rel L2 {e:.1e}", flush=True)
+        print(f"  {precision} {which} {f}: max |variant {variants[1]} - {variants[0]}| = {d:.1e}, "
+              f"rel L2 {e:.1e}", flush=True)
         if exact:
             assert np.array_equal(a, b), (f, d)
         else:
