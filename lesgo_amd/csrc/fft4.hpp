@@ -495,4 +495,83 @@ __global__ void __launch_bounds__(256) k_cols_fwd_v4(ColArgs<T> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Forward row pass (paired rows): the same four-step transform and the same
+// arithmetic as k_rows_r2c_v2, with 32-bit offsets from block-uniform bases
+// and the twiddles from LDS (TWL) or global memory.
+// ---------------------------------------------------------------------------
+template <typename T, int N1, int N2, bool TWL>
+__global__ void __launch_bounds__(256) k_rows_r2c_v4(RowR2CArgs<T> a) {
+    using SP = Split<N1, N2>;
+    constexpr int N = SP::N;
+    extern __shared__ __align__(16) unsigned char smem_rf4[];
+    cplx<T>* ltw = reinterpret_cast<cplx<T>*>(smem_rf4);
+    cplx<T>* lds = TWL ? ltw + N : ltw;
+    const auto* gtw = plan_tw<T>(a.plan);
+    const int job = blockIdx.z;
+    const int mem = blockIdx.y / a.nlev;
+    const int plane = blockIdx.y - mem * a.nlev;
+    const int npairs = a.nrows / 2;
+    const int pair0 = blockIdx.x * a.S;
+    const int S = min(a.S, npairs - pair0);
+    const unsigned sld = (unsigned)a.src_ld;
+    const unsigned nc = (unsigned)a.ncols;
+    const T* sb = a.src[job] + mem * a.src_mstride + (long)(a.lev0_src + plane) * a.src_plane +
+                  (long)(2 * pair0) * sld;
+    cplx<T>* db = a.dst[job] + mem * a.dst_mstride + (long)(a.lev0_dst + plane) * a.dst_plane +
+                  (long)(2 * pair0) * nc;
+    const T sc = a.mscale ? (T)a.mscale[(long)mem * a.mscale_stride] : a.scale[job];
+    tw_stage<T, TWL>(ltw, gtw, N);
+    for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
+        const int s = t / N2, n2 = t - s * N2;
+        const unsigned ra = 2u * (unsigned)s * sld + (unsigned)n2;
+        cplx<T> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1)
+            x[n1] = cplx<T>{sc * sb[ra + N2 * n1], sc * sb[ra + sld + N2 * n1]};
+        dft_reg<T, N1, -1>(x);
+        cplx<T>* Y = lds + s * SP::YS;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+            cplx<T> v = x[k1];
+            if (k1 != 0 && n2 != 0) v = cmul(v, tw_at<T, TWL>(ltw, gtw, n2 * k1));
+            Y[k1 * SP::YR + n2] = v;
+        }
+    }
+    __syncthreads();
+    // step 2 in one round (the launcher keeps S * N1 <= blockDim.x): all LDS
+    // reads finish before the barrier, then X overwrites Y in natural order
+    cplx<T> y[N2];
+    const int t2 = threadIdx.x;
+    const bool act = t2 < S * N1;
+    const int s2 = act ? t2 / N1 : 0;
+    const int k1 = act ? t2 - s2 * N1 : 0;
+    if (act) {
+        const cplx<T>* Y = lds + s2 * SP::YS + k1 * SP::YR;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) y[n2] = Y[n2];
+        dft_reg<T, N2, -1>(y);
+    }
+    __syncthreads();
+    if (act) {
+        cplx<T>* X = lds + s2 * SP::YS;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) X[k1 + N1 * k2] = y[k2];
+    }
+    __syncthreads();
+    const int nw = a.nwr ? a.nwr : (int)nc;
+    for (int idx = threadIdx.x; idx < S * nw; idx += blockDim.x) {
+        const int s = idx / nw;
+        const int k = idx - s * nw;
+        const cplx<T>* X = lds + s * SP::YS;
+        const cplx<T> z = X[k];
+        const cplx<T> zc = X[(N - k) % N];
+        const cplx<T> A{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y - zc.y)};
+        const cplx<T> Bm{(T)0.5 * (z.x - zc.x), (T)0.5 * (z.y + zc.y)};
+        const unsigned o = (2u * (unsigned)s * nc + (unsigned)k) * (unsigned)sizeof(cplx<T>);
+        st_bytes(db, o, A);
+        st_bytes(db, o + nc * (unsigned)sizeof(cplx<T>), cplx<T>{Bm.y, -Bm.x});
+    }
+}
+
 }  // namespace lg

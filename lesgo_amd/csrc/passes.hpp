@@ -23,9 +23,10 @@ struct PassChoice {
     int v = 0;          // rows: 1 = the block's rows staged through LDS (fft3.hpp);
                         // inverse rows: 0 = specialised by band (fft4.hpp);
                         // columns: 0 = specialised by map (fft4.hpp);
-                        // inverse rows and columns: 2 = the unspecialised
-                        // four-step kernel (fft2.hpp), 3 = the specialised
-                        // kernel with twiddles from global memory (0: in LDS)
+                        // forward rows: 0 = 32-bit offsets (fft4.hpp);
+                        // rows and columns: 2 = the unspecialised four-step
+                        // kernel (fft2.hpp), 3 = the fft4.hpp kernel with
+                        // twiddles from global memory (0: an LDS copy)
 };
 
 // Pass kinds of the tuning table (column passes by direction and map).

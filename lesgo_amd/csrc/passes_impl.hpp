@@ -65,6 +65,24 @@ static bool v2_c2r(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowC
     return false;
 }
 
+// The forward row pass with 32-bit offsets (fft4.hpp); twl: twiddles in LDS.
+template <typename T>
+static bool v4_r2c(int n1, int n2, bool twl, dim3 g, size_t lds, hipStream_t s,
+                   const RowR2CArgs<T>& a) {
+    const size_t l4 = lds + (twl ? (size_t)n1 * n2 * sizeof(cplx<T>) : 0);
+    if (l4 > LDS_MAX) return false;
+    const dim3 blk(four_step_block(a.S, n1, n2));
+#define LG_CASE(A, B)                                                                    \
+    if (n1 == A && n2 == B) {                                                            \
+        if (twl) LG_LAUNCH((k_rows_r2c_v4<T, A, B, true>), g, blk, l4, s, a);            \
+        else LG_LAUNCH((k_rows_r2c_v4<T, A, B, false>), g, blk, l4, s, a);               \
+        return true;                                                                     \
+    }
+    LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    return false;
+}
+
 template <typename T>
 static bool v3_r2c(int n1, int n2, dim3 g, size_t lds, hipStream_t s, const RowR2CArgs<T>& a) {
 #define LG_CASE(A, B)                                                                    \
@@ -226,7 +244,7 @@ bool choice_ok(int kind, const PassChoice& c, int N, int other) {
     }
     if (c.v != 0 && !(c.v == 1 && c.n1 && (kind == PK_R2C || kind == PK_C2R)) &&
         !((c.v == 2 || c.v == 3) && c.n1 &&
-          (kind == PK_COLS_INV || kind == PK_COLS_PAD || kind == PK_C2R ||
+          (kind == PK_COLS_INV || kind == PK_COLS_PAD || kind == PK_C2R || kind == PK_R2C ||
            kind == PK_COLS_FWD || kind == PK_COLS_UNPAD)))
         return false;
     // real rows of a length-N row pass have stride N + 2; ncols <= N / 2
@@ -317,6 +335,9 @@ void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* con
                              ((double)pl.N * sizeof(T) + (double)(a.nwr ? a.nwr : ncols) * sizeof(cplx<T>));
         tm.run(s, KK_ROWS_R2C, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
+            if (ch.n1 && (ch.v == 0 || ch.v == 3) &&
+                v4_r2c<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, a))
+                return;
             if (!(ch.n1 && v2_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)))
                 LG_LAUNCH(k_rows_r2c<T>, grid, dim3(PBLK), lds, s, a);
         });

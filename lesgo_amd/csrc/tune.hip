@@ -76,7 +76,7 @@ std::vector<PassChoice> candidates(int kind, int N, int other) {
         for (int S : {1, 2, 4, 8, 12, 16, 24, 32, 48, 64})
             for (int v : {0, 1, 2, 3}) {
                 if (v == 1 && !(rows && s.first)) continue;
-                if (v >= 2 && !((cols || kind == PK_C2R) && s.first)) continue;
+                if (v >= 2 && !((cols || rows) && s.first)) continue;
                 PassChoice c{s.first, s.second, S, v};
                 if (choice_ok<T>(kind, c, N, other)) out.push_back(c);
             }

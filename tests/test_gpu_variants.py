@@ -1,8 +1,8 @@
 """Launch variants of one pass compute the same results.
 
-The column passes and the inverse row pass each have a kernel specialised at
-compile time (by index map and multiplier for columns, by stored band for
-rows; variant 0, the default, and variant 3, the same with the twiddles read
+The column and row passes each have a kernel specialised at compile time
+(by index map and multiplier for columns, by stored band for inverse rows,
+32-bit offsets for forward rows; variant 0, the default, and variant 3, the same with the twiddles read
 from global memory instead of an LDS copy) and the unspecialised four-step
 kernel (variant 2); all do the same arithmetic.  A 256 x 256 x 24 wind-farm case
 (disks, fresh start: Euler step, Smagorinsky steps, F_* initialisation and a
@@ -12,8 +12,8 @@ each variant of one pass.
 
 Where the specialised kernel sees the same arithmetic graph the results are
 equal bit for bit: the 256-point inverse columns and rows, the (24, 16)
-split of the padded columns, the forward columns (plain and unpadding), and
-variants 3 and 0 everywhere.  Where it knows at compile time that some
+split of the padded columns, the forward columns (plain and unpadding), the
+forward rows, and variants 3 and 0 everywhere.  Where it knows at compile time that some
 inputs are the padding's zeros ((16, 24) padded columns, the 384-point rows
 whose input band is a third of the transform), the compiler folds those
 butterflies and contracts the remaining multiply-adds into FMAs in another
@@ -41,10 +41,12 @@ CASES = {
     "rows384": ([("c2r", 384, 384, 16, 24, 8)], (2, 0), False),
     "cols_fwd": ([("cols_fwd", 256, 128, 16, 16, 8), ("cols_unpad", 384, 128, 16, 24, 8)],
                  (2, 0), True),
+    "rows_fwd": ([("r2c", 256, 256, 16, 16, 12), ("r2c", 384, 384, 16, 24, 8)], (2, 0), True),
     # twiddles from global memory (3) or the block's LDS copy (0): the same values
     "twiddles": ([("cols_inv", 256, 128, 16, 16, 8), ("cols_pad", 384, 128, 16, 24, 8),
                   ("c2r", 256, 256, 16, 16, 16), ("c2r", 384, 384, 16, 24, 8),
-                  ("cols_fwd", 256, 128, 16, 16, 8), ("cols_unpad", 384, 128, 16, 24, 8)],
+                  ("cols_fwd", 256, 128, 16, 16, 8), ("cols_unpad", 384, 128, 16, 24, 8),
+                  ("r2c", 256, 256, 16, 16, 12), ("r2c", 384, 384, 16, 24, 8)],
                  (3, 0), True),
 }
 
