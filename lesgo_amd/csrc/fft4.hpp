@@ -189,6 +189,7 @@ __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
     for (int t = threadIdx.x; t < TC * N2; t += blockDim.x) {
         const int cl = t % TC, n2 = t / TC;
         if (cl >= tc) continue;
+        __builtin_assume(n2 >= 0 && n2 < N2);
         const int c = c0 + cl;
         cplx<T> x[N1];
         switch (op) {
@@ -278,6 +279,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
         T* const out = pass ? ob2 : ob1;
         for (int t = threadIdx.x; t < S * N2; t += blockDim.x) {
             const int s = t / N2, n2 = t - s * N2;
+            __builtin_assume(n2 >= 0 && n2 < N2);
             const unsigned ra = 2u * (unsigned)s * nc;          // row a, in elements
             auto ld2 = [&](int col, cplx<T>& A, cplx<T>& B) {
                 const unsigned o = (ra + (unsigned)col) * (unsigned)sizeof(cplx<T>);
@@ -297,17 +299,24 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
                 // once unrolled)
                 const int lo = N2 * n1, hi = N2 * n1 + N2 - 1;
                 cplx<T> z{(T)0, (T)0};
-                auto below = [&](int kk) {        // 0 < kk < half
-                    const bool in = (BAND == BAND_FULL) || kk < nce;
-                    if (in) {
+                // column kk of the stored band, kk in [klo, khi] over the
+                // tasks: the band test is decided at compile time unless
+                // the range straddles the band's edge
+                auto in_band = [&](int kk, int klo, int khi) {
+                    if (BAND == BAND_FULL) return true;
+                    if (BAND == BAND_THIRD && khi < N / 3) return true;
+                    if (BAND == BAND_THIRD && klo >= N / 3) return false;
+                    return kk < nce;
+                };
+                auto below = [&](int kk, int klo, int khi) {        // 0 < kk < half
+                    if (in_band(kk, klo, khi)) {
                         cplx<T> A, B;
                         ld2(kk, A, B);
                         z = cplx<T>{A.x - B.y, A.y + B.x};
                     }
                 };
-                auto above = [&](int kk) {        // kk = N - k, 0 < kk < half
-                    const bool in = (BAND == BAND_FULL) || kk < nce;
-                    if (in) {
+                auto above = [&](int kk, int klo, int khi) {        // kk = N - k
+                    if (in_band(kk, klo, khi)) {
                         cplx<T> A, B;
                         ld2(kk, A, B);
                         z = cplx<T>{A.x + B.y, B.x - A.y};
@@ -321,13 +330,13 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
                             z = cplx<T>{A.x, B.x};
                         }
                     } else {
-                        below(k);
+                        below(k, lo, hi);
                     }
                 } else if (lo > half) {
-                    above(N - k);
+                    above(N - k, N - hi, N - lo);
                 } else {
-                    if (k < half) below(k);
-                    else if (k > half) above(N - k);
+                    if (k < half) below(k, lo, half - 1);
+                    else if (k > half) above(N - k, N - hi, half - 1);
                 }
                 x[n1] = z;
             }
@@ -454,6 +463,7 @@ __global__ void __launch_bounds__(256) k_cols_fwd_v4(ColArgs<T> a) {
     for (int t = threadIdx.x; t < TC * N1; t += blockDim.x) {
         const int cl = t % TC, k1 = t / TC;
         if (cl >= tc) continue;
+        __builtin_assume(k1 >= 0 && k1 < N1);
         const unsigned cb = (unsigned)(c0 + cl) * CB;
         const cplx<T>* Y = lds + cl * CS + k1 * SP::YR;
         cplx<T> y[N2];
