@@ -213,12 +213,12 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
         for (int q = 0; q < 3; ++q) {
             sR[q] = dalloc<cplx<T>>(M * msp);
             sRf[q] = dalloc<cplx<T>>(M * msp);
-            sD[q] = dalloc<cplx<T>>(M * msp);
             sF[q] = dalloc<cplx<T>>(M * msp);
             fprev[q] = dalloc<T>(M * d.field);
             tprev[q] = dalloc<T>(M * d.field);
         }
         sP = dalloc<cplx<T>>(M * msp);
+        sDz = dalloc<cplx<T>>(M * msp);
     }
     px.build(d.nx);
     py.build(d.ny);
@@ -292,12 +292,12 @@ Case<T>::~Case() {
     for (int q = 0; q < 3; ++q) {
         fr(sR[q]);
         fr(sRf[q]);
-        fr(sD[q]);
         fr(sF[q]);
         fr(fprev[q]);
         fr(tprev[q]);
     }
     fr(sP);
+    fr(sDz);
     fr(tavg);
     fr(tspec);
     fr(G1);
@@ -450,7 +450,7 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
          21 * M * chunk * d.splane * 2 * T;
     b += 3 * M * sizeof(double) + 1024 * sizeof(double) + 2 * d.splane * T;
     b += M * (CK_N * sizeof(double) + 3 * sizeof(unsigned long long) + 2 * sizeof(int));
-    if (!p.use_cfl_dt) b += 13 * M * msp * 2 * T + 6 * M * d.field * T;   // spectral state
+    if (!p.use_cfl_dt) b += 11 * M * msp * 2 * T + 6 * M * d.field * T;   // spectral state
     return b;
 }
 
@@ -1407,21 +1407,18 @@ static SpecGeo<T> spec_geo(const Dims& d, long msp, int k0, int nlev) {
                       (T)(2.0 * PI_D / d.prm.ly)};
 }
 
-// divstress_uv + divstress_w from the six stress spectra (levels 1..nz) into
-// sD; no inverse transform.
+// The six stress spectra (levels 1..nz) for divstress_uv + divstress_w,
+// which k_spec_ab2 forms from them; no inverse transform.
 template <typename T>
 void Case<T>::divstress_spec() {
+    // the stress spectra go to spec[3..8] (convec has finished with them;
+    // its advection spectra stay in spec[0..2]); k_spec_ab2 forms divt
     const T sc = (T)(1.0 / (d.nx * d.ny));
     const T* src[6] = {fld[LG_TXX], fld[LG_TXY], fld[LG_TYY],
                        fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ]};
     const T scs[6] = {sc, sc, sc, sc, sc, sc};
-    cplx<T>* S[6] = {spec[0], spec[1], spec[2], spec[3], spec[4], spec[5]};
+    cplx<T>* S[6] = {spec[3], spec[4], spec[5], spec[6], spec[7], spec[8]};
     fwd2d(6, src, scs, d.plane, 1, d.field, S, d.splane, 1, msp, d.nz);
-    const int n = d.ny * d.NK;
-    hipLaunchKernelGGL(k_spec_divt<T>, dim3((n + BLK - 1) / BLK, d.nz * Ma), dim3(BLK), 0, stream,
-                       (const cplx<T>*)spec[0], (const cplx<T>*)spec[1], (const cplx<T>*)spec[2],
-                       (const cplx<T>*)spec[3], (const cplx<T>*)spec[4], (const cplx<T>*)spec[5],
-                       sD[0], sD[1], sD[2], spec_geo<T>(d, msp, 1, d.nz), (T)(1.0 / d.dz));
 }
 
 // press_stag_array on u*, v*, w* spectra (in sU, sV, sW; the solve scales
@@ -1440,7 +1437,7 @@ void Case<T>::press_spec() {
     timer.run(stream, KK_TRIDAG, tbytes, [&] {
         LG_LAUNCH(k_tridag<T>, dim3((nsys + tb - 1) / tb), dim3(tb), 0, stream,
                   (const cplx<T>*)sU, (const cplx<T>*)sV, (const cplx<T>*)sW,
-                  (const cplx<T>*)(sD[2] + d.splane), (const cplx<T>*)(sD[2] + d.nz * d.splane),
+                  (const cplx<T>*)(sDz + d.splane), (const cplx<T>*)(sDz + d.nz * d.splane),
                   sP, gam, usc, tg, dev_err);
     });
 }
@@ -1487,8 +1484,8 @@ void Case<T>::step_body_spec(bool cold) {
     ddz_uvw();
     wallstress(sU, sV);
     sgs_stag();
-    divstress_spec();
     convec(true, true);
+    divstress_spec();
     const bool tb = turb && turb->total_nodes > 0 && kf1 >= kf0;
     if (tb) {
         turbines_forcing();
@@ -1499,7 +1496,9 @@ void Case<T>::step_body_spec(bool cold) {
     const int n = d.ny * d.NK;
     const dim3 gs((n + BLK - 1) / BLK, d.nz * Ma);
     const SpecGeo<T> g = spec_geo<T>(d, msp, 1, d.nz);
-    SpecAb2<T> a{spec[0], spec[1], spec[2], sD[0], sD[1], sD[2],
+    SpecAb2<T> a{spec[0], spec[1], spec[2],
+                 StressSpec<T>{spec[3], spec[4], spec[5], spec[6], spec[7], spec[8]}, sDz,
+                 (T)(1.0 / d.dz),
                  tb ? sF[0] : nullptr, tb ? sF[1] : nullptr, tb ? sF[2] : nullptr,
                  sR[0], sR[1], sR[2], sRf[0], sRf[1], sRf[2], sU, sV, sW, d_mpf,
                  (const int*)d_flags, d.prm.use_mean_p_force, kf0, kf1,
@@ -1612,31 +1611,23 @@ void Case<T>::materialize(unsigned groups) {
     if (!groups) return;
     const int nz = d.nz, n1 = nz - 1;
     // gc: this step's remainder; gq: the previous step's (or the high part of
-    // a physical RHS_f); hs: the stress high parts for divt
+    // a physical RHS_f); hs: scratch of rhs_high
     T* gc[3] = {tmp[0], tmp[1], tmp[2]};
     T* gq[3] = {tmp[3], tmp[4], tmp[5]};
     T* hs[3] = {tmp[6], tmp[7], tmp[8]};
     bool have_gq = false;
-    const unsigned need_g = SG_UVW | SG_RHS | SG_RHSF | SG_DIVT;
+    if (groups & SG_DIVT) {
+        // the step keeps no divt spectra (k_spec_ab2 forms divt from the
+        // stress spectra): the reference's divstress_uv/w on the step's
+        // physical stresses (uses spec[0..7] and tmp[0], tmp[2], tmp[4],
+        // before the remainders below take tmp)
+        divstress_uvw();
+    }
+    const unsigned need_g = SG_UVW | SG_RHS | SG_RHSF;
     if (groups & need_g) {
         T* f[3] = {fld[LG_FXA], fld[LG_FYA], fld[LG_FZA]};
         T* t[3] = {fld[LG_TXZ], fld[LG_TYZ], fld[LG_TZZ]};
         rhs_high(forced_last ? f : nullptr, t, gc, hs);
-    }
-    if (groups & SG_DIVT) {
-        const cplx<T>* is[3] = {sD[0], sD[1], sD[2]};
-        T* dst[3] = {fld[LG_DIVTX], fld[LG_DIVTY], fld[LG_DIVTZ]};
-        cplx<T>* tsp[3] = {spec[9], spec[10], spec[11]};
-        const int on[3] = {OP_NONE, OP_NONE, OP_NONE};
-        inv2d(2, is, on, d.splane, 1, msp, dst, d.plane, 1, d.field, n1, tsp);
-        inv2d(1, is + 2, on, d.splane, 1, msp, dst + 2, d.plane, 1, d.field, nz, tsp);
-        // divt = F^-1(divt_hat) + (-(gc) without the forcing): recompute the
-        // stress part alone from hs
-        hipLaunchKernelGGL(k_divt_high<T>, grid_ij(d.ld, d.ny, n1, Ma), dim3(BLK), 0, stream,
-                           dst[0], dst[1], dst[2], (const T*)hs[0], (const T*)hs[1],
-                           (const T*)hs[2], GEO(1, n1), (T)(1.0 / d.dz));
-        fill_levels({{LG_DIVTX, 0, BOGUS}, {LG_DIVTX, nz, BOGUS}, {LG_DIVTY, 0, BOGUS},
-                     {LG_DIVTY, nz, BOGUS}, {LG_DIVTZ, 0, BOGUS}});
     }
     if (groups & (SG_UVW | SG_RHSF)) {
         // per member: the step before's stresses and forcing, or, for a
@@ -1756,8 +1747,7 @@ int Case<T>::count_cold() const {
 template <typename T>
 void Case<T>::copy_spec_member(int src, int dst) {
     if (!spec_on) return;
-    cplx<T>* bufs[13] = {sU, sV, sW, sR[0], sR[1], sR[2], sRf[0], sRf[1], sRf[2],
-                         sP, sD[0], sD[1], sD[2]};
+    cplx<T>* bufs[10] = {sU, sV, sW, sR[0], sR[1], sR[2], sRf[0], sRf[1], sRf[2], sP};
     for (cplx<T>* b : bufs)
         LG_HIP(hipMemcpyAsync(b + (long)dst * msp, b + (long)src * msp, msp * sizeof(cplx<T>),
                               hipMemcpyDeviceToDevice, stream));

@@ -141,7 +141,7 @@ struct Case {
     cplx<T>* sR[3] = {};             // RHS spectra (after - grad p)
     cplx<T>* sRf[3] = {};            // RHS_f spectra
     cplx<T>* sP = nullptr;           // pressure spectrum, levels 0..nz
-    cplx<T>* sD[3] = {};             // stress divergence spectra
+    cplx<T>* sDz = nullptr;          // divtz spectra of the step (levels 1 and nz only)
     cplx<T>* sF[3] = {};             // forcing spectra of the step (disk levels)
     T* fprev[3] = {};                // the previous step's fxa, fya, fza
     T* tprev[3] = {};                // the previous step's txz, tyz, tzz

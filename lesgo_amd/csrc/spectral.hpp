@@ -59,41 +59,36 @@ __device__ __forceinline__ void spec_k(const SpecGeo<T>& g, int j, int c, T& kx,
 template <typename T>
 __device__ __forceinline__ cplx<T> cik(cplx<T> v, T k) { return cplx<T>{-v.y * k, v.x * k}; }
 
-// Divergence of the stresses from their spectra (levels 1..nz of txx, txy,
-// tyy, txz, tyz, tzz): divtx, divty on 1..nz-1 (d/dz of txz, tyz from k to
-// k+1), divtz on 1..nz (d/dz of tzz from k-1 to k on 2..nz-1 only).
+// Divergence of the stresses from their spectra (divstress_uv.f90:21-85,
+// divstress_w.f90:21-116) at mode o of level k: divtx, divty on 1..nz-1
+// (d/dz of txz, tyz from k to k+1), divtz on 1..nz (d/dz of tzz from k-1 to k
+// on 2..nz-1 only); the ky Nyquist row carries no x, y derivative.
 template <typename T>
-__global__ void k_spec_divt(const cplx<T>* txx, const cplx<T>* txy, const cplx<T>* tyy,
-                            const cplx<T>* txz, const cplx<T>* tyz, const cplx<T>* tzz,
-                            cplx<T>* dvx, cplx<T>* dvy, cplx<T>* dvz, SpecGeo<T> g, T rdz) {
-    int m, k, j, c;
-    long o;
-    if (!spec_of(g, m, k, j, c, o)) return;
-    T kx, ky;
-    spec_k(g, j, c, kx, ky);
-    const long S = g.splane;
-    const bool nyq = j == g.ny / 2;
-    if (k <= g.nz - 1) {
-        const cplx<T> a = txx[o], b = txy[o], e = tyy[o];
-        const cplx<T> xz1 = txz[o + S], xz0 = txz[o], yz1 = tyz[o + S], yz0 = tyz[o];
-        const cplx<T> dxy = nyq ? cplx<T>{(T)0, (T)0}
-                                : cplx<T>{-a.y * kx + -b.y * ky, a.x * kx + b.x * ky};
-        const cplx<T> dyy = nyq ? cplx<T>{(T)0, (T)0}
-                                : cplx<T>{-b.y * kx + -e.y * ky, b.x * kx + e.x * ky};
-        dvx[o] = cplx<T>{dxy.x + rdz * (xz1.x - xz0.x), dxy.y + rdz * (xz1.y - xz0.y)};
-        dvy[o] = cplx<T>{dyy.x + rdz * (yz1.x - yz0.x), dyy.y + rdz * (yz1.y - yz0.y)};
-    } else {
-        dvx[o] = cplx<T>{(T)0, (T)0};
-        dvy[o] = cplx<T>{(T)0, (T)0};
-    }
-    const cplx<T> p = txz[o], q = tyz[o];
+struct StressSpec {
+    const cplx<T> *txx, *txy, *tyy, *txz, *tyz, *tzz;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> divt_xy(const cplx<T>* a, const cplx<T>* b, const cplx<T>* zt,
+                                           long o, long S, bool nyq, T kx, T ky, T rdz) {
+    const cplx<T> p = a[o], q = b[o];
+    const cplx<T> z1 = zt[o + S], z0 = zt[o];
+    const cplx<T> dxy = nyq ? cplx<T>{(T)0, (T)0}
+                            : cplx<T>{-p.y * kx + -q.y * ky, p.x * kx + q.x * ky};
+    return cplx<T>{dxy.x + rdz * (z1.x - z0.x), dxy.y + rdz * (z1.y - z0.y)};
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> divt_z(const StressSpec<T>& t, long o, long S, int k, int nz,
+                                          bool nyq, T kx, T ky, T rdz) {
+    const cplx<T> p = t.txz[o], q = t.tyz[o];
     cplx<T> z = nyq ? cplx<T>{(T)0, (T)0}
                     : cplx<T>{-p.y * kx + -q.y * ky, p.x * kx + q.x * ky};
-    if (k >= 2 && k <= g.nz - 1) {
-        const cplx<T> z1 = tzz[o], z0 = tzz[o - S];
+    if (k >= 2 && k <= nz - 1) {
+        const cplx<T> z1 = t.tzz[o], z0 = t.tzz[o - S];
         z = cplx<T>{z.x + rdz * (z1.x - z0.x), z.y + rdz * (z1.y - z0.y)};
     }
-    dvz[o] = z;
+    return z;
 }
 
 // RHS assembly and AB2 update (main.f90:211-296) on the spectra, levels
@@ -101,10 +96,14 @@ __global__ void k_spec_divt(const cplx<T>* txx, const cplx<T>* txy, const cplx<T
 // (+ the disks' forcing on levels kf0..kf1) for x, y on 1..nz-1 and z on
 // 1..nz (no forcing at nz); members with euler[m] take R_f = R; then
 // u* = u + dt (t1 R + t2 R_f) in place of u (x, y on 1..nz-1, z on 1..nz).
+// divt is formed here from the stress spectra; its z component at levels 1
+// and nz (the pressure solve's boundary rows) is also stored in dvz_bc.
 template <typename T>
 struct SpecAb2 {
     const cplx<T> *cx, *cy, *cz;       // advection spectra
-    const cplx<T> *dx, *dy, *dz;       // stress divergence spectra
+    StressSpec<T> ts;                  // stress spectra, levels 1..nz
+    cplx<T>* dvz_bc;                   // divtz spectra (levels 1 and nz written)
+    T rdz;
     const cplx<T> *fx, *fy, *fz;       // forcing spectra (levels kf0..kf1) or null
     cplx<T> *rx, *ry, *rz;             // R out
     cplx<T> *rfx, *rfy, *rfz;          // R_f in (out for Euler members)
@@ -123,9 +122,13 @@ __global__ void k_spec_ab2(SpecAb2<T> a, SpecGeo<T> g) {
     const bool eu = a.euler[m] != 0;
     const bool forced = a.fx && k >= a.kf0 && k <= a.kf1 && k <= g.nz - 1;
     const bool zero_mode = (j == 0 && c == 0);
-    auto upd = [&](const cplx<T>* cv, const cplx<T>* dv, const cplx<T>* fv, cplx<T>* r,
+    T kx, ky;
+    spec_k(g, j, c, kx, ky);
+    const bool nyq = j == g.ny / 2;
+    const long S = g.splane;
+    auto upd = [&](const cplx<T>* cv, const cplx<T> dd, const cplx<T>* fv, cplx<T>* r,
                    cplx<T>* rf, cplx<T>* u, T mp, bool mpf_on) {
-        const cplx<T> cc = cv[o], dd = dv[o];
+        const cplx<T> cc = cv[o];
         cplx<T> R{-cc.x - dd.x, -cc.y - dd.y};
         if (mpf_on && zero_mode) R.x = R.x + mp;
         if (fv) {
@@ -141,12 +144,16 @@ __global__ void k_spec_ab2(SpecAb2<T> a, SpecGeo<T> g) {
                        uu.y + a.dt * (a.t1 * R.y + a.t2 * F.y)};
     };
     if (k <= g.nz - 1) {
-        upd(a.cx, a.dx, forced ? a.fx : nullptr, a.rx, a.rfx, a.ux, (T)a.mpf[2 * m],
+        const cplx<T> dvx = divt_xy(a.ts.txx, a.ts.txy, a.ts.txz, o, S, nyq, kx, ky, a.rdz);
+        const cplx<T> dvy = divt_xy(a.ts.txy, a.ts.tyy, a.ts.tyz, o, S, nyq, kx, ky, a.rdz);
+        upd(a.cx, dvx, forced ? a.fx : nullptr, a.rx, a.rfx, a.ux, (T)a.mpf[2 * m],
             a.use_mpf != 0);
-        upd(a.cy, a.dy, forced ? a.fy : nullptr, a.ry, a.rfy, a.uy, (T)a.mpf[2 * m + 1],
+        upd(a.cy, dvy, forced ? a.fy : nullptr, a.ry, a.rfy, a.uy, (T)a.mpf[2 * m + 1],
             a.use_mpf != 0);
     }
-    upd(a.cz, a.dz, forced ? a.fz : nullptr, a.rz, a.rfz, a.uz, (T)0, false);
+    const cplx<T> dvz = divt_z(a.ts, o, S, k, g.nz, nyq, kx, ky, a.rdz);
+    if (k == 1 || k == g.nz) a.dvz_bc[o] = dvz;
+    upd(a.cz, dvz, forced ? a.fz : nullptr, a.rz, a.rfz, a.uz, (T)0, false);
 }
 
 // RHS -= grad p and the projection (main.f90:321-326, forcing.f90:149-244)
@@ -235,22 +242,6 @@ __global__ void k_rhs_high(T* gx, T* gy, T* gz, const T* hxz, const T* hyz, cons
     gx[o] = fx - rdz * (hxz[o + P] - hxz[o]);
     gy[o] = fy - rdz * (hyz[o + P] - hyz[o]);
     gz[o] = (k >= 2 && k <= g.nz - 1) ? fz - rdz * (hzz[o] - hzz[o - P]) : fz;
-}
-
-// divt += the stress high parts' vertical derivatives (levels 1..nz-1):
-// the physical divtx, divty, divtz of the reference (divstress_uv.f90,
-// divstress_w.f90) from their spectra.
-template <typename T>
-__global__ void k_divt_high(T* dx, T* dy, T* dz, const T* hxz, const T* hyz, const T* hzz,
-                            Geo g, T rdz) {
-    int m, k, j, i;
-    long mb;
-    if (!pt_of(g, g.ld, m, k, j, i, mb)) return;
-    const long o = mb + (long)k * g.plane + (long)j * g.ld + i;
-    const long P = g.plane;
-    dx[o] = dx[o] + rdz * (hxz[o + P] - hxz[o]);
-    dy[o] = dy[o] + rdz * (hyz[o + P] - hyz[o]);
-    if (k >= 2) dz[o] = dz[o] + rdz * (hzz[o] - hzz[o - P]);
 }
 
 // The spectra of members starting from physical fields replace their state:
