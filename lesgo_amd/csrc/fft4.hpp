@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_cols_inv_v4(ColArgs<T> a) {
 enum : int { BAND_FULL = 0, BAND_JOB = 1, BAND_THIRD = 3 };
 
 template <typename T, int N1, int N2, int BAND, bool TWL>
-__global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
+__device__ __forceinline__ void rows_c2r_v4_body(const RowC2RArgs<T>& a) {
     using SP = Split<N1, N2>;
     constexpr int N = SP::N;
     constexpr int half = N / 2;
@@ -409,6 +409,12 @@ __global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
         __syncthreads();
     }
 }
+
+template <typename T, int N1, int N2, int BAND, bool TWL>
+__global__ void __launch_bounds__(256) k_rows_c2r_v4(RowC2RArgs<T> a) {
+    rows_c2r_v4_body<T, N1, N2, BAND, TWL>(a);
+}
+
 
 // ---------------------------------------------------------------------------
 // Forward column pass (no multiplier, the form every forward pass of the

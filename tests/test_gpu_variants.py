@@ -2,9 +2,9 @@
 
 The column and row passes each have a kernel specialised at compile time
 (by index map and multiplier for columns, by stored band for inverse rows,
-32-bit offsets for forward rows; variant 0, the default, and variant 3, the same with the twiddles read
-from global memory instead of an LDS copy) and the unspecialised four-step
-kernel (variant 2); all do the same arithmetic.  A 256 x 256 x 24 wind-farm case
+32-bit offsets for forward rows; variant 0, the default, and variant 3, the
+same with the twiddles read from global memory instead of an LDS copy) and
+the unspecialised four-step kernel (variant 2); all do the same arithmetic.  A 256 x 256 x 24 wind-farm case
 (disks, fresh start: Euler step, Smagorinsky steps, F_* initialisation and a
 Lagrangian update, so every multiplier, the test-filter bands, the x
 derivative's second output and the 3/2-grid padding run) is stepped with
