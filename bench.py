@@ -89,6 +89,20 @@ def prepare_tuning(c):
     return "live"
 
 
+def override_variants(v):
+    """Every recorded four-step launch choice with kernel variant v instead,
+    where the library accepts it for that pass."""
+    from lesgo_amd._lib import LesgoError
+    from lesgo_amd.case import tune_set, tune_table
+    for e in tune_table():
+        if e["n1"] and e["variant"] != v:
+            try:
+                tune_set(e["kind"], e["precision"], e["N"], e["other"], e["n1"], e["n2"],
+                         e["per_wg"], v)
+            except LesgoError:
+                pass
+
+
 def step_figures(name, precision, ms_per_step):
     """Whole-step roofline: the algorithmic HBM bytes and FFT flops of one
     step (compulsory traffic with plane-local fusion and the one z barrier of
@@ -351,6 +365,10 @@ def main():
                          "steps include the dynamic model)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--variant-override", type=int, default=None, metavar="V",
+                    help="after loading the launch choices, run every four-step pass with "
+                         "kernel variant V where it is launchable (2: the unspecialised "
+                         "kernels), for before/after profiles")
     args = ap.parse_args()
     if args.calibrate_copy:
         import ctypes
@@ -386,6 +404,9 @@ def main():
     for prec in precs:
         cfg, cases[prec] = make_case(args.workload, prec)
     tuning = prepare_tuning(cases[precs[0]])
+    if args.variant_override is not None:
+        tuning = f"{tuning}, variant {args.variant_override} forced"
+        override_variants(args.variant_override)
     spin = cfg.dyn_init if args.spinup is None else args.spinup
     for prec in precs:
         log(f"[bench] {prec} {args.workload}: {spin} untimed spin-up steps")
