@@ -35,6 +35,33 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+# 2-D real transforms: numpy.fft (one thread; what the pins run) unless
+# set_fft_workers(n > 1) routes them through scipy.fft's threaded pocketfft
+# (the bench's multi-core CPU baseline).  Both are pocketfft with the same
+# conventions; results agree to rounding (tests/test_oracle_fft.py).
+_FFT_WORKERS = 1
+
+
+def set_fft_workers(n: int) -> None:
+    """Threads for the oracle's 2-D FFTs: 1 = numpy.fft, n > 1 = scipy.fft."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = max(1, int(n))
+
+
+def _rfft2(a, axes=(-2, -1)):
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.rfft2(a, axes=axes, workers=_FFT_WORKERS)
+    return np.fft.rfft2(a, axes=axes)
+
+
+def _irfft2(a, s, axes=(-2, -1), norm=None):
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.irfft2(a, s=s, axes=axes, norm=norm, workers=_FFT_WORKERS)
+    return np.fft.irfft2(a, s=s, axes=axes, norm=norm)
+
+
 BOGUS = -1234567890.0                      # param.f90:93
 PI = 3.1415926535897932384626433           # param.f90:94 (== math.pi in double)
 ZERO_LAG = 1.0e-24                         # lagrange_Sdep.f90:68
@@ -140,7 +167,7 @@ class Spectral:
     # -- transforms -------------------------------------------------------
     def r2c(self, f):
         """dfftw_execute_dft_r2c on (..., ny, ld) planes -> (..., ny, lh)."""
-        return np.fft.rfft2(f[..., : self.p.nx], axes=(-2, -1))
+        return _rfft2(f[..., : self.p.nx], axes=(-2, -1))
 
     def c2r(self, fh, ld=None):
         """dfftw_execute_dft_c2r (unnormalised) -> (..., ny, ld), pads 0."""
@@ -148,7 +175,7 @@ class Spectral:
         ny = fh.shape[-2]
         nx = 2 * (fh.shape[-1] - 1)
         out = np.zeros(fh.shape[:-2] + (ny, nx + 2))
-        out[..., :nx] = np.fft.irfft2(fh, s=(ny, nx), axes=(-2, -1), norm="forward")
+        out[..., :nx] = _irfft2(fh, s=(ny, nx), axes=(-2, -1), norm="forward")
         return out
 
     def zero_oddballs(self, fh):
@@ -843,7 +870,7 @@ def convec(sp: Spectral, s: State):
                         + (v_big[K] + v_big[Km]) * vort1[K])
 
     def from_big(cc):
-        return sp.c2r(sp.unpadd(np.fft.rfft2(cc[..., : p.nx2], axes=(-2, -1))))
+        return sp.c2r(sp.unpadd(_rfft2(cc[..., : p.nx2], axes=(-2, -1))))
 
     s.RHSx[1:nz] = from_big(ccx[1:nz])
     s.RHSy[1:nz] = from_big(ccy[1:nz])
