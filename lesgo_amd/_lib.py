@@ -141,7 +141,11 @@ TAVG_NAMES = ("u", "v", "w", "w_uv", "u_w", "v_w", "u2", "v2", "w2", "uv", "uw",
 
 PASS_KINDS = ("r2c", "c2r", "cols_fwd", "cols_inv", "tridag", "cols_unpad", "cols_pad")
 
-KERNEL_KINDS = ("rows_r2c", "rows_c2r", "cols_fwd", "cols_inv", "tridag")
+# timing classes of lg_timing_report (KernelKind in csrc/timer.hpp): the row
+# and column passes on the nx x ny grid, the 3/2 grid's passes, the pressure
+# tridiagonal and the Lagrangian model's fused row passes
+KERNEL_KINDS = ("rows_r2c", "rows_c2r", "cols_fwd", "cols_inv", "tridag", "rows_r2c_pad",
+                "rows_c2r_pad", "cols_unpad", "cols_pad", "lag_rows", "lag_prep_rows")
 
 
 class LesgoError(RuntimeError):

@@ -333,7 +333,7 @@ void launch_rows_r2c(KernelTimer& tm, const FftPlan& pl, int njobs, const T* con
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
         const double bytes = (double)nj * nlev * M * nrows *
                              ((double)pl.N * sizeof(T) + (double)(a.nwr ? a.nwr : ncols) * sizeof(cplx<T>));
-        tm.run(s, KK_ROWS_R2C, bytes, [&] {
+        tm.run(s, 3 * ncols == pl.N ? KK_ROWS_R2C_PAD : KK_ROWS_R2C, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_r2c<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (ch.n1 && (ch.v == 0 || ch.v == 3) &&
                 v4_r2c<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, a))
@@ -388,7 +388,7 @@ void launch_rows_c2r(KernelTimer& tm, const FftPlan& pl, int njobs, const cplx<T
         dim3 grid((nrows / 2 + a.S - 1) / a.S, nlev * M, nj);
         const double bytes = (double)nlev * M * nrows *
                              (cin * sizeof(cplx<T>) + (double)nout * dst_ld * sizeof(T));
-        tm.run(s, KK_ROWS_C2R, bytes, [&] {
+        tm.run(s, 3 * ncols == pl.N ? KK_ROWS_C2R_PAD : KK_ROWS_C2R, bytes, [&] {
             if (ch.n1 && ch.v == 1 && v3_c2r<T>(ch.n1, ch.n2, grid, lds, s, a)) return;
             if (ch.n1 && (ch.v == 0 || ch.v == 3) &&
                 v4_c2r<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, a))
@@ -425,7 +425,9 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
     }
     dim3 grid(nj, (cmax + c.TC - 1) / c.TC, nlev * M);
     const double bytes = (double)nlev * M * sizeof(cplx<T>) * cols_rw;
-    tm.run(s, DIR < 0 ? KK_COLS_FWD : KK_COLS_INV, bytes, [&] {
+    const int kk = c.map == MAP_PAD ? KK_COLS_PAD
+                   : (c.map == MAP_UNPAD ? KK_COLS_UNPAD : (DIR < 0 ? KK_COLS_FWD : KK_COLS_INV));
+    tm.run(s, kk, bytes, [&] {
         if (DIR > 0 && ch.n1 && (ch.v == 0 || ch.v == 3) &&
             v4_cols_inv<T>(ch.n1, ch.n2, ch.v == 0, grid, lds, s, c))
             return;
@@ -466,7 +468,7 @@ bool launch_lag_rows(KernelTimer& tm, const LagRowArgs<T>& a, int M, hipStream_t
     // fields written
     const double bytes = (double)M * a.nlev * a.nrows *
                          (LAGROW_Q * (double)a.nce * sizeof(cplx<T>) + 2.0 * N * sizeof(double));
-    tm.run(s, KK_ROWS_C2R, bytes, [&] {
+    tm.run(s, KK_LAG_ROWS, bytes, [&] {
 #define LG_CASE(A, B)                                                                \
     if (n1 == A && n2 == B) LG_LAUNCH((k_lag_rows<T, A, B>), grid, dim3(LAGROW_BLK), lds, s, a);
         LG_LAGROW_SPLITS(LG_CASE)
@@ -490,7 +492,7 @@ bool launch_lag_prep_rows(KernelTimer& tm, const LagPrepArgs<T>& a, int M, hipSt
     // spectra written over the band
     const double rows = (double)M * a.nlev * (a.fplane / a.ld);
     const double bytes = rows * (9.0 * N * sizeof(T) + LAGROW_Q * (double)a.nwr * sizeof(cplx<T>));
-    tm.run(s, KK_ROWS_R2C, bytes, [&] {
+    tm.run(s, KK_LAG_PREP, bytes, [&] {
 #define LG_CASE(A, B)                                                                \
     if (n1 == A && n2 == B) LG_LAUNCH((k_lag_prep_rows<T, A, B>), grid, dim3(LAGROW_BLK), lds, s, a);
         LG_LAGROW_SPLITS(LG_CASE)

@@ -27,13 +27,22 @@ inline thread_local LaunchEvents g_launch_events;
             hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                   \
     } while (0)
 
+// Timing classes: one per transform kernel family and grid, so a class
+// holds launches of one kernel on one shape (the 3/2-grid passes and the
+// Lagrangian model's row passes have classes of their own).
 enum KernelKind : int {
-    KK_ROWS_R2C = 0,
-    KK_ROWS_C2R = 1,
-    KK_COLS_FWD = 2,
-    KK_COLS_INV = 3,
-    KK_TRIDAG = 4,
-    KK_NKINDS = 5,
+    KK_ROWS_R2C = 0,          // forward rows, nx points
+    KK_ROWS_C2R = 1,          // inverse rows, nx points
+    KK_COLS_FWD = 2,          // forward columns, ny points
+    KK_COLS_INV = 3,          // inverse columns, ny points
+    KK_TRIDAG = 4,            // pressure tridiagonal
+    KK_ROWS_R2C_PAD = 5,      // forward rows on the 3/2 grid
+    KK_ROWS_C2R_PAD = 6,      // inverse rows on the 3/2 grid
+    KK_COLS_UNPAD = 7,        // forward columns 3/2 grid -> small
+    KK_COLS_PAD = 8,          // inverse columns small -> 3/2 grid
+    KK_LAG_ROWS = 9,          // the Lagrangian model's inverse rows + contractions
+    KK_LAG_PREP = 10,         // the Lagrangian model's forward rows of its quantities
+    KK_NKINDS = 11,
 };
 
 struct KernelTimer {
