@@ -227,12 +227,19 @@ int lg_turbines_set_udt_member(lg_case* c, int m, const double* u_d_T, int nturb
  * below; lg_timing_report returns total milliseconds, launch count and the
  * algorithmic bytes (inputs read + outputs written once) of one class. */
 enum {
-    LG_K_ROWS_R2C = 0,
-    LG_K_ROWS_C2R = 1,
-    LG_K_COLS_FWD = 2,
-    LG_K_COLS_INV = 3,
-    LG_K_TRIDAG = 4,
-    LG_K_NKINDS = 5
+    LG_K_ROWS_R2C = 0,          /* forward rows, nx points */
+    LG_K_ROWS_C2R = 1,          /* inverse rows, nx points */
+    LG_K_COLS_FWD = 2,          /* forward columns, ny points */
+    LG_K_COLS_INV = 3,          /* inverse columns, ny points */
+    LG_K_TRIDAG = 4,            /* pressure tridiagonal */
+    LG_K_ROWS_R2C_PAD = 5,      /* forward rows on the 3/2 grid */
+    LG_K_ROWS_C2R_PAD = 6,      /* inverse rows on the 3/2 grid */
+    LG_K_COLS_UNPAD = 7,        /* forward columns, 3/2 grid to nx x ny */
+    LG_K_COLS_PAD = 8,          /* inverse columns, nx x ny to 3/2 grid */
+    LG_K_LAG_ROWS = 9,          /* Lagrangian model: inverse rows and contractions */
+    LG_K_LAG_PREP = 10,         /* Lagrangian model: forward rows of its quantities */
+    LG_K_ROWS_R2C_PROD = 11,    /* forward rows on the 3/2 grid forming the products */
+    LG_K_NKINDS = 12
 };
 int lg_timing_enable(lg_case* c, int on);
 int lg_timing_report(lg_case* c, int kind, double* ms, double* launches, double* bytes);

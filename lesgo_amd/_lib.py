@@ -145,7 +145,8 @@ PASS_KINDS = ("r2c", "c2r", "cols_fwd", "cols_inv", "tridag", "cols_unpad", "col
 # and column passes on the nx x ny grid, the 3/2 grid's passes, the pressure
 # tridiagonal and the Lagrangian model's fused row passes
 KERNEL_KINDS = ("rows_r2c", "rows_c2r", "cols_fwd", "cols_inv", "tridag", "rows_r2c_pad",
-                "rows_c2r_pad", "cols_unpad", "cols_pad", "lag_rows", "lag_prep_rows")
+                "rows_c2r_pad", "cols_unpad", "cols_pad", "lag_rows", "lag_prep_rows",
+                "rows_r2c_prod")
 
 
 class LesgoError(RuntimeError):

@@ -157,6 +157,16 @@ static bool lag_fused_for(int nx, int ny, int tsize) {
     return tsize == 4 ? lag_rows_ok<float>(nx, ny) : lag_rows_ok<double>(nx, ny);
 }
 
+// The convective products formed in the 3/2 grid's forward row pass
+// (k_rows_r2c_prod): where nx2 has a four-step split and
+// LESGO_AMD_PROD_ROWS is not 0.  The three product fields are then not
+// allocated.
+static bool prod_fused_for(int nx2, int ny2, int tsize) {
+    const char* pr = std::getenv("LESGO_AMD_PROD_ROWS");
+    if (pr && pr[0] == '0') return false;
+    return tsize == 4 ? prod_rows_ok<float>(nx2, ny2) : prod_rows_ok<double>(nx2, ny2);
+}
+
 // real slots of the filtered buffer: 42 fields, or 21 complex spectra (each
 // within one field) and four FP64 contraction fields
 static long lag_filt_slots(bool fused, int tsize) { return fused ? 21 + 4 * (8 / tsize) : 42; }
@@ -169,7 +179,8 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     LG_HIP(hipStreamCreate(&stream));
     for (int i = 0; i < LG_NFIELDS; ++i) fld[i] = dalloc<T>(M * d.field);
     for (int i = 0; i < NSPEC; ++i) spec[i] = dalloc<cplx<T>>(M * msp);
-    for (int i = 0; i < NBIG; ++i) big[i] = dalloc<T>(M * mbp);
+    prod_fused = prod_fused_for(d.nx2, d.ny2, (int)sizeof(T));
+    for (int i = 0; i < (prod_fused ? 6 : NBIG); ++i) big[i] = dalloc<T>(M * mbp);
     for (int i = 0; i < 6; ++i) bigspec[i] = dalloc<cplx<T>>(M * mbsp);
     for (int i = 0; i < NTMP; ++i) tmp[i] = dalloc<T>(M * d.field);
     sU = dalloc<cplx<T>>(M * msp);
@@ -443,7 +454,8 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
     size_t b = 0;
     b += (size_t)(LG_NFIELDS + 9 + 4) * M * d.field * T;        // fields, tmp, Ftmp
     b += (size_t)(12 + 3) * M * msp * 2 * T;                     // spec, sU/sV/sW
-    b += 9 * M * mbp * T + 6 * M * mbsp * 2 * T;                 // big, bigspec
+    const size_t nbig = prod_fused_for(d.nx2, d.ny2, tsize) ? 6 : 9;
+    b += nbig * M * mbp * T + 6 * M * mbsp * 2 * T;              // big, bigspec
     b += M * msp * (sizeof(double) + sizeof(double2));           // gam, usc
     const bool lf = lag_fused_for(d.nx, d.ny, tsize);
     b += ((lf ? 0 : 15) + lag_filt_slots(lf, tsize)) * M * chunk * d.plane * T +
@@ -602,6 +614,12 @@ void Case<T>::fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst,
                             int nlev) {
     launch_rows_r2c<T>(timer, pbx.plan, njobs, src, nullptr, d.bplane, d.ld_big, lev0, mbp,
                        bigspec, d.bsplane, lev0, mbsp, d.ny2, d.NK, nlev, Ma, stream, nullptr);
+    cols_unpad(njobs, dst, lev0, nlev);
+}
+
+// The forward column pass of fwd_big_unpad: bigspec[0..njobs) into dst.
+template <typename T>
+void Case<T>::cols_unpad(int njobs, cplx<T>* const* dst, int lev0, int nlev) {
     ColArgs<T> c{};
     c.plan = pby.plan;
     for (int i = 0; i < njobs; ++i) {
@@ -1197,14 +1215,22 @@ void Case<T>::convec(bool reuse_spectra, bool spectral_out) {
         fwd2d(3, vs, nullptr, d.plane, 1, d.field, S + 3, d.splane, 1, msp, nl);
     }
     pad_inv_big(6, ps, 1, nl);
-    ProdArgs<T> pa{big[0], big[1], big[2], big[3], big[4], big[5], big[6], big[7], big[8],
-                   1.0 / (d.nx2 * d.ny2)};
-    const int pch = (d.nz + PROD_L - 1) / PROD_L;
-    hipLaunchKernelGGL(k_products<T>, grid_ij((d.nx2 + 1) / 2, d.ny2, pch, Ma), dim3(BLK), 0, stream, pa,
-                       d.nx2, d.ny2, d.ld_big, d.nz, mbp);
-    const T* cs[3] = {big[6], big[7], big[8]};
     cplx<T>* cd[3] = {spec[0], spec[1], spec[2]};
-    fwd_big_unpad(3, cs, cd, 1, nl);
+    const double pc = 1.0 / (d.nx2 * d.ny2);
+    if (prod_fused) {
+        const ProdSrc<T> p{big[0], big[1], big[2], big[3], big[4], big[5], pc, d.nz};
+        if (!launch_rows_r2c_prod<T>(timer, pbx.plan, p, d.bplane, d.ld_big, 1, mbp, bigspec,
+                                     d.bsplane, mbsp, d.ny2, d.NK, nl, Ma, stream))
+            throw std::runtime_error("convec: no fused products pass for nx2");
+        cols_unpad(3, cd, 1, nl);
+    } else {
+        ProdArgs<T> pa{big[0], big[1], big[2], big[3], big[4], big[5], big[6], big[7], big[8], pc};
+        const int pch = (d.nz + PROD_L - 1) / PROD_L;
+        hipLaunchKernelGGL(k_products<T>, grid_ij((d.nx2 + 1) / 2, d.ny2, pch, Ma), dim3(BLK), 0,
+                           stream, pa, d.nx2, d.ny2, d.ld_big, d.nz, mbp);
+        const T* cs[3] = {big[6], big[7], big[8]};
+        fwd_big_unpad(3, cs, cd, 1, nl);
+    }
     if (spectral_out) return;          // the advection spectra stay in spec[0..2]
     const cplx<T>* is[3] = {spec[0], spec[1], spec[2]};
     const int ops[3] = {OP_NONE, OP_NONE, OP_NONE};

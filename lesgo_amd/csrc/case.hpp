@@ -126,6 +126,10 @@ struct Case {
     // itself (k_lag_rows) where the grid allows it; LESGO_AMD_LAG_ROWS=0
     // keeps the 42 filtered fields
     bool lag_fused = true;
+    // the convective products are formed in the 3/2 grid's forward row pass
+    // (k_rows_r2c_prod) where that length has a four-step split;
+    // LESGO_AMD_PROD_ROWS=0 keeps k_products and the three product fields
+    bool prod_fused = true;
     std::vector<char> mcold;
     // members whose last step started from their physical fields (their
     // RHS_f high part is in the physical RHS_f only); the others take it
@@ -214,6 +218,7 @@ struct Case {
                cplx<T>* const* tmpspec, const cplx<T>* const* src2 = nullptr);
     void pad_inv_big(int njobs, const cplx<T>* const* src, int lev0, int nlev);
     void fwd_big_unpad(int njobs, const T* const* src, cplx<T>* const* dst, int lev0, int nlev);
+    void cols_unpad(int njobs, cplx<T>* const* dst, int lev0, int nlev);
 
     // subsystems
     void filt_da(int f, int dfdx, int dfdy, cplx<T>* keep = nullptr);

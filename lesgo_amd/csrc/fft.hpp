@@ -233,6 +233,15 @@ struct RowR2CArgs {
                            // reads no more; the four-step and generic kernels honour it)
 };
 
+// The fields the convective products are formed from (prod_rows.hpp): u, v,
+// w and the vorticity on the 3/2 grid, [member][0..nz][ny2][ld_big].
+template <typename T>
+struct ProdSrc {
+    const T *ub, *vb, *wb, *o1, *o2, *o3;
+    double c;              // 1 / (nx2 ny2)
+    int nz;
+};
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_rows_r2c(RowR2CArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_r2c[];

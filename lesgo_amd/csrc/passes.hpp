@@ -113,6 +113,16 @@ template <typename T>
 bool launch_lag_rows(KernelTimer& tm, const LagRowArgs<T>& a, int M, hipStream_t s);
 template <typename T>
 bool lag_rows_ok(int N, int nrows);
+// The 3/2 grid's forward row pass with the convective products formed in
+// its loads (prod_rows.hpp): the x, y, z products of levels lev0 ..
+// lev0 + nlev - 1 into dst[0..2].  False: no four-step split for this length.
+template <typename T>
+bool launch_rows_r2c_prod(KernelTimer& tm, const FftPlan& pl, const ProdSrc<T>& p,
+                          long src_plane, int src_ld, int lev0, long src_ms,
+                          cplx<T>* const* dst, long dst_plane, long dst_ms, int nrows, int ncols,
+                          int nlev, int M, hipStream_t s);
+template <typename T>
+bool prod_rows_ok(int N, int nrows);
 // The Lagrangian model's forward row pass forming its 21 quantities.
 template <typename T>
 bool launch_lag_prep_rows(KernelTimer& tm, const LagPrepArgs<T>& a, int M, hipStream_t s);

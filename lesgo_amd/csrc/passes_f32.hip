@@ -20,6 +20,10 @@ template void launch_cols<float, +1>(KernelTimer&, ColArgs<float>&, int, int, in
                                    const PassChoice*);
 template bool launch_lag_rows<float>(KernelTimer&, const LagRowArgs<float>&, int, hipStream_t);
 template bool lag_rows_ok<float>(int, int);
+template bool launch_rows_r2c_prod<float>(KernelTimer&, const FftPlan&, const ProdSrc<float>&, long,
+                                          int, int, long, cplx<float>* const*, long, long, int,
+                                          int, int, int, hipStream_t);
+template bool prod_rows_ok<float>(int, int);
 template bool launch_lag_prep_rows<float>(KernelTimer&, const LagPrepArgs<float>&, int, hipStream_t);
 
 }  // namespace lg

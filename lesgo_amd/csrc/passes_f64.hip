@@ -22,6 +22,10 @@ template void launch_cols<double, +1>(KernelTimer&, ColArgs<double>&, int, int, 
 std::vector<std::pair<int, int>> pass_splits(int N) { return splits_for(N); }
 template bool launch_lag_rows<double>(KernelTimer&, const LagRowArgs<double>&, int, hipStream_t);
 template bool lag_rows_ok<double>(int, int);
+template bool launch_rows_r2c_prod<double>(KernelTimer&, const FftPlan&, const ProdSrc<double>&, long,
+                                          int, int, long, cplx<double>* const*, long, long, int,
+                                          int, int, int, hipStream_t);
+template bool prod_rows_ok<double>(int, int);
 template bool launch_lag_prep_rows<double>(KernelTimer&, const LagPrepArgs<double>&, int, hipStream_t);
 
 }  // namespace lg

@@ -9,6 +9,7 @@
 #include "fft4.hpp"
 #include "lag_rows.hpp"
 #include "passes.hpp"
+#include "prod_rows.hpp"
 
 namespace lg {
 
@@ -420,7 +421,8 @@ void launch_cols(KernelTimer& tm, ColArgs<T>& c, int nj, int nlev, int M, int ny
         cmax = std::max(cmax, nzc);
         const int band = c.ykeep[i] > 0 ? std::min(2 * c.ykeep[i] - 1, ny_in) : 0;
         const double rin = (DIR > 0 && band) ? band : ny_in;
-        const double rout = (DIR < 0 && band) ? band : N;
+        // MAP_UNPAD writes the small grid's ny rows, not the N it transforms
+        const double rout = c.map == MAP_UNPAD ? c.ny : ((DIR < 0 && band) ? band : N);
         cols_rw += (double)nzc * (rin + rout);
     }
     dim3 grid(nj, (cmax + c.TC - 1) / c.TC, nlev * M);
@@ -499,6 +501,80 @@ bool launch_lag_prep_rows(KernelTimer& tm, const LagPrepArgs<T>& a, int M, hipSt
 #undef LG_CASE
     });
     return true;
+}
+
+// The split of the fused products pass (prod_rows.hpp) for length N: the
+// forward row pass's recorded or heuristic four-step split, else the first
+// instantiated one; one row pair per block (measured on the 384-point 3/2
+// grid of 256 x 256 x 192: 234, 236 and 245 us per pass in mixed precision
+// for 1, 2 and 3 pairs, 406, 415 and 435 us in FP64).  False: N has no
+// four-step split.
+template <typename T>
+static bool prod_rows_choice(int N, int nrows, int& n1, int& n2, int& P) {
+    const PassChoice ch = choice_for<T>(PK_R2C, N, nrows);
+    n1 = ch.n1;
+    n2 = ch.n2;
+    if (!n1) {
+        const auto sp = splits_for(N);
+        if (sp.empty()) return false;
+        n1 = sp[0].first;
+        n2 = sp[0].second;
+    }
+    P = 1;
+    return 3 * P * std::max(n1, n2) <= (int)four_step_block(3 * P, n1, n2);
+}
+
+// Depends on N only (an instantiated split exists), not on the launch table.
+template <typename T>
+bool prod_rows_ok(int N, int nrows) {
+    int n1, n2, P;
+    return !splits_for(N).empty() && prod_rows_choice<T>(N, nrows, n1, n2, P);
+}
+
+template <typename T>
+bool launch_rows_r2c_prod(KernelTimer& tm, const FftPlan& pl, const ProdSrc<T>& p,
+                          long src_plane, int src_ld, int lev0, long src_ms,
+                          cplx<T>* const* dst, long dst_plane, long dst_ms, int nrows, int ncols,
+                          int nlev, int M, hipStream_t s) {
+    int n1, n2, P;
+    if (!prod_rows_choice<T>(pl.N, nrows, n1, n2, P)) return false;
+    RowR2CArgs<T> a{};
+    a.plan = pl;
+    for (int i = 0; i < 3; ++i) a.dst[i] = dst[i];
+    a.src_plane = src_plane;
+    a.dst_plane = dst_plane;
+    a.src_ld = src_ld;
+    a.nrows = nrows;
+    a.ncols = ncols;
+    a.lev0_src = lev0;
+    a.lev0_dst = lev0;
+    a.nlev = nlev;
+    a.src_mstride = src_ms;
+    a.dst_mstride = dst_ms;
+    a.S = P;
+    a.stride = n1 * (n2 + 1) + 1;
+    const size_t lds = (size_t)3 * P * a.stride * sizeof(cplx<T>);
+    const bool twl = lds + (size_t)pl.N * sizeof(cplx<T>) <= LDS_DEFAULT;
+    const size_t l4 = lds + (twl ? (size_t)pl.N * sizeof(cplx<T>) : 0);
+    const dim3 grid((nrows / 2 + P - 1) / P, nlev * M);
+    const dim3 blk(four_step_block(3 * P, n1, n2));
+    // algorithmic bytes: the six fields read once on the levels that have a
+    // product (1..nz-1 of lev0..lev0+nlev-1), the three half spectra written
+    const int kr = std::max(0, std::min(lev0 + nlev - 1, p.nz - 1) - std::max(lev0, 1) + 1);
+    const double bytes = (double)M * nrows *
+                         (6.0 * kr * pl.N * sizeof(T) + 3.0 * nlev * ncols * sizeof(cplx<T>));
+    bool ok = false;
+    tm.run(s, KK_ROWS_R2C_PROD, bytes, [&] {
+#define LG_CASE(A, B)                                                                      \
+    if (n1 == A && n2 == B) {                                                              \
+        if (twl) LG_LAUNCH((k_rows_r2c_prod<T, A, B, true>), grid, blk, l4, s, a, p);      \
+        else LG_LAUNCH((k_rows_r2c_prod<T, A, B, false>), grid, blk, l4, s, a, p);         \
+        ok = true;                                                                         \
+    }
+        LG_SPLITS(LG_CASE)
+#undef LG_CASE
+    });
+    return ok;
 }
 
 }  // namespace lg

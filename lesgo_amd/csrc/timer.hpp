@@ -42,7 +42,8 @@ enum KernelKind : int {
     KK_COLS_PAD = 8,          // inverse columns small -> 3/2 grid
     KK_LAG_ROWS = 9,          // the Lagrangian model's inverse rows + contractions
     KK_LAG_PREP = 10,         // the Lagrangian model's forward rows of its quantities
-    KK_NKINDS = 11,
+    KK_ROWS_R2C_PROD = 11,    // forward rows on the 3/2 grid forming the convective products
+    KK_NKINDS = 12,
 };
 
 struct KernelTimer {
