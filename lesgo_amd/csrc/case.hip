@@ -918,22 +918,36 @@ void Case<T>::sgs_stag() {
 // ---------------------------------------------------------------------------
 // interpolag_Sdep.f90 / lagrange_Sdep.f90
 // ---------------------------------------------------------------------------
+// in_place: F_* hold the interpolated values afterwards (the reference's
+// interpolag_Sdep; the F_* are copied to Ftmp and gathered back).  Else the
+// interpolated values go to Ftmp (F_* unchanged), where lagrange_Sdep's
+// update reads them: no copies.
 template <typename T>
-void Case<T>::interpolag_sdep() {
+void Case<T>::interpolag_sdep(bool in_place) {
     const int ids[4] = {LG_F_LM, LG_F_MM, LG_F_QN, LG_F_NN};
     const long Fq = (long)M * d.field;
-    for (int q = 0; q < 4; ++q) copy_dev<T>(fld[ids[q]], Ftmp + q * Fq, Ma * d.field, stream);
+    const T* src[4];
+    T* dst[4];
+    for (int q = 0; q < 4; ++q) {
+        if (in_place) {
+            copy_dev<T>(fld[ids[q]], Ftmp + q * Fq, Ma * d.field, stream);
+            src[q] = Ftmp + q * Fq;
+            dst[q] = fld[ids[q]];
+        } else {
+            src[q] = fld[ids[q]];
+            dst[q] = Ftmp + q * Fq;
+        }
+    }
     InterpGeo g{d.nx, d.ny, d.nz, d.ld, d.plane, d.dx, d.dy, d.dz,
                 d.prm.lx, d.prm.ly, d.prm.lz, d.lagran_dt, d.field, clk()};
     hipLaunchKernelGGL(k_interpolag<T>, grid_ij(d.nx, d.ny, d.nz, Ma), dim3(BLK), 0, stream,
-                       fld[LG_U], fld[LG_V], fld[LG_W], Ftmp, Ftmp + Fq, Ftmp + 2 * Fq,
-                       Ftmp + 3 * Fq, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN], fld[LG_F_NN], g,
-                       (const int*)(d_flags + 2 * M));
+                       fld[LG_U], fld[LG_V], fld[LG_W], src[0], src[1], src[2], src[3], dst[0],
+                       dst[1], dst[2], dst[3], g, (const int*)(d_flags + 2 * M));
 }
 
 template <typename T>
 void Case<T>::lagrange_sdep() {
-    interpolag_sdep();
+    interpolag_sdep(false);
     const Geo g = GEO(1, d.nz);
     const double tf1 = 2.0, tf2 = 4.0;
     LagUpd up{};
@@ -1091,7 +1105,8 @@ void Case<T>::lagrange_sdep() {
         hipLaunchKernelGGL(k_lag_update<T>, grid_ij(d.ld, d.ny, nc, Ma), dim3(BLK), 0, stream,
                            (const T*)lag_filt, fld[LG_F_LM], fld[LG_F_MM], fld[LG_F_QN],
                            fld[LG_F_NN], fld[LG_CS_OPT2], fld[LG_BETA], fld[LG_TN_ALL], g, k0, nc,
-                           Ma, up, (const int*)(d_flags + 2 * M), (const double*)contr);
+                           Ma, up, (const int*)(d_flags + 2 * M), (const double*)contr,
+                           (const T*)Ftmp, (long)M * d.field);
     }
     if (cfl_mode)
         hipLaunchKernelGGL(k_clock_lag_reset, dim3(1), dim3(64), 0, stream, d_clock,

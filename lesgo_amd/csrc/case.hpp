@@ -185,7 +185,7 @@ struct Case {
     T* lag_phys = nullptr;           // 15 * chunk planes
     cplx<T>* lag_spec = nullptr;     // 21 * chunk planes
     T* lag_filt = nullptr;           // 42 * chunk planes
-    T* Ftmp = nullptr;               // 4 fields (F_* copies for interpolation)
+    T* Ftmp = nullptr;               // 4 fields (interpolated F_*, or F_* copies)
     double* red = nullptr;           // reduction scratch
     T* G1 = nullptr;                 // [ny][NK]
     T* G2 = nullptr;
@@ -232,7 +232,7 @@ struct Case {
     void wallstress(const cplx<T>* su = nullptr, const cplx<T>* sv = nullptr);
     void calc_sij();
     void sgs_stag();
-    void interpolag_sdep();
+    void interpolag_sdep(bool in_place = true);
     void lagrange_sdep();
     void divstress_uv();
     void divstress_w();

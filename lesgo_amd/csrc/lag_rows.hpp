@@ -93,29 +93,32 @@ __global__ void __launch_bounds__(LAGROW_BLK) k_lag_rows(LagRowArgs<T> a) {
         __syncthreads();
     }
     // the contractions at each point of the two rows (k_lag_update's
-    // formulas, FP64 from the filtered values)
+    // formulas, FP64 from the filtered values); a thread takes both rows of
+    // a point, so each slot element is read from LDS once
     const double cn = a.cn, tf2 = a.tf2;
-    for (int t = tid; t < 2 * N; t += blockDim.x) {
-        const int r = t / N, n = t - r * N;
-        auto V = [&](int q) -> double {
-            const cplx<T> v = lds[q * YS + n];
-            return (double)(r ? v.y : v.x);
-        };
-        const double ub = V(0), vb = V(1), wb = V(2);
-        const double L11 = V(3) - ub * ub, L12 = V(4) - ub * vb, L13 = V(5) - ub * wb;
-        const double L22 = V(6) - vb * vb, L23 = V(7) - vb * wb, L33 = V(8) - wb * wb;
-        const double Sn = smag_norm(V(9), V(10), V(11), V(12), V(13), V(14));
-        const double M11 = cn * (V(15) - tf2 * Sn * V(9));
-        const double M12 = cn * (V(16) - tf2 * Sn * V(10));
-        const double M13 = cn * (V(17) - tf2 * Sn * V(11));
-        const double M22 = cn * (V(18) - tf2 * Sn * V(12));
-        const double M23 = cn * (V(19) - tf2 * Sn * V(13));
-        const double M33 = cn * (V(20) - tf2 * Sn * V(14));
-        const double LM = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
-        const double MM = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
-        const long o = (long)mem * a.omstride + (long)c * a.oplane + (long)(j + r) * a.ld + n;
-        a.out0[o] = LM;
-        a.out1[o] = MM;
+    for (int n = tid; n < N; n += blockDim.x) {
+        cplx<T> val[LAGROW_Q];
+#pragma unroll
+        for (int q = 0; q < LAGROW_Q; ++q) val[q] = lds[q * YS + n];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            auto V = [&](int q) -> double { return (double)(r ? val[q].y : val[q].x); };
+            const double ub = V(0), vb = V(1), wb = V(2);
+            const double L11 = V(3) - ub * ub, L12 = V(4) - ub * vb, L13 = V(5) - ub * wb;
+            const double L22 = V(6) - vb * vb, L23 = V(7) - vb * wb, L33 = V(8) - wb * wb;
+            const double Sn = smag_norm(V(9), V(10), V(11), V(12), V(13), V(14));
+            const double M11 = cn * (V(15) - tf2 * Sn * V(9));
+            const double M12 = cn * (V(16) - tf2 * Sn * V(10));
+            const double M13 = cn * (V(17) - tf2 * Sn * V(11));
+            const double M22 = cn * (V(18) - tf2 * Sn * V(12));
+            const double M23 = cn * (V(19) - tf2 * Sn * V(13));
+            const double M33 = cn * (V(20) - tf2 * Sn * V(14));
+            const double LM = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
+            const double MM = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
+            const long o = (long)mem * a.omstride + (long)c * a.oplane + (long)(j + r) * a.ld + n;
+            a.out0[o] = LM;
+            a.out1[o] = MM;
+        }
     }
 }
 
@@ -142,35 +145,38 @@ __global__ void __launch_bounds__(LAGROW_BLK) k_lag_prep_rows(LagPrepArgs<T> a) 
     const int tid = threadIdx.x;
     const long P = a.fplane;
     // the quantities at every point of the two rows, into slot q (row j in
-    // .x, row j + 1 in .y), natural order
-    for (int t = tid; t < 2 * N; t += blockDim.x) {
-        const int r = t / N, i = t - r * N;
-        const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + r) * a.ld + i;
-        T ub, vb, wb;
-        if (k == 1) {
-            ub = a.u[o];
-            vb = a.v[o];
-            wb = (T)0.25 * a.w[o + P];
-        } else if (k == a.nz) {
-            ub = a.u[o - P];
-            vb = a.v[o - P];
-            wb = (T)0;
-        } else {
-            ub = (T)0.5 * (a.u[o] + a.u[o - P]);
-            vb = (T)0.5 * (a.v[o] + a.v[o - P]);
-            wb = a.w[o];
-        }
-        const T s11 = a.S11[o], s12 = a.S12[o], s13 = a.S13[o], s22 = a.S22[o], s23 = a.S23[o],
-                s33 = a.S33[o];
-        const T S = smag_norm(s11, s12, s13, s22, s23, s33);
-        const T q[LAGROW_Q] = {ub,      vb,      wb,      ub * ub, ub * vb, ub * wb, vb * vb,
-                               vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23,
-                               s33,     S * s11, S * s12, S * s13, S * s22, S * s23, S * s33};
+    // .x, row j + 1 in .y), natural order; a thread takes both rows of a
+    // point and writes each slot element once
+    for (int i = tid; i < N; i += blockDim.x) {
+        T qv[2][LAGROW_Q];
 #pragma unroll
-        for (int qq = 0; qq < LAGROW_Q; ++qq) {
-            T* e = reinterpret_cast<T*>(lds + qq * YS + i) + r;
-            *e = q[qq];
+        for (int r = 0; r < 2; ++r) {
+            const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + r) * a.ld + i;
+            T ub, vb, wb;
+            if (k == 1) {
+                ub = a.u[o];
+                vb = a.v[o];
+                wb = (T)0.25 * a.w[o + P];
+            } else if (k == a.nz) {
+                ub = a.u[o - P];
+                vb = a.v[o - P];
+                wb = (T)0;
+            } else {
+                ub = (T)0.5 * (a.u[o] + a.u[o - P]);
+                vb = (T)0.5 * (a.v[o] + a.v[o - P]);
+                wb = a.w[o];
+            }
+            const T s11 = a.S11[o], s12 = a.S12[o], s13 = a.S13[o], s22 = a.S22[o],
+                    s23 = a.S23[o], s33 = a.S33[o];
+            const T S = smag_norm(s11, s12, s13, s22, s23, s33);
+            const T q[LAGROW_Q] = {ub,      vb,      wb,      ub * ub, ub * vb, ub * wb, vb * vb,
+                                   vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23,
+                                   s33,     S * s11, S * s12, S * s13, S * s22, S * s23, S * s33};
+#pragma unroll
+            for (int qq = 0; qq < LAGROW_Q; ++qq) qv[r][qq] = q[qq];
         }
+#pragma unroll
+        for (int qq = 0; qq < LAGROW_Q; ++qq) lds[qq * YS + i] = cplx<T>{qv[0][qq], qv[1][qq]};
     }
     __syncthreads();
     // forward transforms: step 1 from natural order into the four-step

@@ -230,7 +230,7 @@ struct LagUpd {
 template <typename T>
 __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, T* Cs,
                              T* Beta, T* Tn_all, Geo g, int k0, int nc, int M, LagUpd p,
-                             const int* mode, const double* contr) {
+                             const int* mode, const double* contr, const T* Fin, long Fq) {
     int j, i;
     if (!ij_of(g.ld, g.ny, j, i)) return;
     const int m = blockIdx.y / nc;
@@ -292,8 +292,20 @@ __global__ void k_lag_update(const T* filt, T* F_LM, T* F_MM, T* F_QN, T* F_NN, 
     }
     const double zero = p.zero;
     // 2-delta running averages (lagrange_Sdep.f90:297-315)
-    double fLM = F_LM[o], fMM = F_MM[o];
-    double fQN = F_QN[o], fNN = F_NN[o];
+    // the interpolated F_* (interpolag): in place, or Fin + q Fq when the
+    // interpolation wrote them there
+    double fLM, fMM, fQN, fNN;
+    if (Fin) {
+        fLM = Fin[o];
+        fMM = Fin[o + Fq];
+        fQN = Fin[o + 2 * Fq];
+        fNN = Fin[o + 3 * Fq];
+    } else {
+        fLM = F_LM[o];
+        fMM = F_MM[o];
+        fQN = F_QN[o];
+        fNN = F_NN[o];
+    }
     if (md == 2) {
         fMM = MM;
         fLM = 0.03 * MM;
