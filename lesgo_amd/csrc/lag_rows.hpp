@@ -145,38 +145,35 @@ __global__ void __launch_bounds__(LAGROW_BLK) k_lag_prep_rows(LagPrepArgs<T> a) 
     const int tid = threadIdx.x;
     const long P = a.fplane;
     // the quantities at every point of the two rows, into slot q (row j in
-    // .x, row j + 1 in .y), natural order; a thread takes both rows of a
-    // point and writes each slot element once
-    for (int i = tid; i < N; i += blockDim.x) {
-        T qv[2][LAGROW_Q];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + r) * a.ld + i;
-            T ub, vb, wb;
-            if (k == 1) {
-                ub = a.u[o];
-                vb = a.v[o];
-                wb = (T)0.25 * a.w[o + P];
-            } else if (k == a.nz) {
-                ub = a.u[o - P];
-                vb = a.v[o - P];
-                wb = (T)0;
-            } else {
-                ub = (T)0.5 * (a.u[o] + a.u[o - P]);
-                vb = (T)0.5 * (a.v[o] + a.v[o - P]);
-                wb = a.w[o];
-            }
-            const T s11 = a.S11[o], s12 = a.S12[o], s13 = a.S13[o], s22 = a.S22[o],
-                    s23 = a.S23[o], s33 = a.S33[o];
-            const T S = smag_norm(s11, s12, s13, s22, s23, s33);
-            const T q[LAGROW_Q] = {ub,      vb,      wb,      ub * ub, ub * vb, ub * wb, vb * vb,
-                                   vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23,
-                                   s33,     S * s11, S * s12, S * s13, S * s22, S * s23, S * s33};
-#pragma unroll
-            for (int qq = 0; qq < LAGROW_Q; ++qq) qv[r][qq] = q[qq];
+    // .x, row j + 1 in .y), natural order
+    for (int t = tid; t < 2 * N; t += blockDim.x) {
+        const int r = t / N, i = t - r * N;
+        const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + r) * a.ld + i;
+        T ub, vb, wb;
+        if (k == 1) {
+            ub = a.u[o];
+            vb = a.v[o];
+            wb = (T)0.25 * a.w[o + P];
+        } else if (k == a.nz) {
+            ub = a.u[o - P];
+            vb = a.v[o - P];
+            wb = (T)0;
+        } else {
+            ub = (T)0.5 * (a.u[o] + a.u[o - P]);
+            vb = (T)0.5 * (a.v[o] + a.v[o - P]);
+            wb = a.w[o];
         }
+        const T s11 = a.S11[o], s12 = a.S12[o], s13 = a.S13[o], s22 = a.S22[o], s23 = a.S23[o],
+                s33 = a.S33[o];
+        const T S = smag_norm(s11, s12, s13, s22, s23, s33);
+        const T q[LAGROW_Q] = {ub,      vb,      wb,      ub * ub, ub * vb, ub * wb, vb * vb,
+                               vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23,
+                               s33,     S * s11, S * s12, S * s13, S * s22, S * s23, S * s33};
 #pragma unroll
-        for (int qq = 0; qq < LAGROW_Q; ++qq) lds[qq * YS + i] = cplx<T>{qv[0][qq], qv[1][qq]};
+        for (int qq = 0; qq < LAGROW_Q; ++qq) {
+            T* e = reinterpret_cast<T*>(lds + qq * YS + i) + r;
+            *e = q[qq];
+        }
     }
     __syncthreads();
     // forward transforms: step 1 from natural order into the four-step
