@@ -16,29 +16,14 @@ struct InterpGeo {
 };
 
 // cell_indx_w for 'i'/'j' (functions.f90:188-237): px is wrapped with
-// Fortran MODULO (the wrapped value is also the one used for xdiff).  A
-// departure point lies within a cell of its node, so px is almost always in
-// (-L, 2L): there the wrap is px, px + L or px - L, the values fmod and the
-// sign correction give (|px| < L: fmod returns px; px in [L, 2L): px - L is
-// exact); elsewhere fmod.  The end thresholds divide only when |r| or
-// |r - L| is within twice the threshold, where the quotient can fall below
-// it.
+// Fortran MODULO (the wrapped value is also the one used for xdiff).
 __device__ __forceinline__ int cell_xy(double& px, double L, double d, int n) {
-    double r;
-    if (px >= 0.0 && px < L) {
-        r = px;
-    } else if (px < 0.0 && px > -L) {
-        r = px + L;
-    } else if (px >= L && px < 2.0 * L) {
-        r = px - L;
-    } else {
-        r = fmod(px, L);
-        if (r != 0.0 && r < 0.0) r += L;
-    }
+    double r = fmod(px, L);
+    if (r != 0.0 && r < 0.0) r += L;
     px = r;
     const double thresh = 1.0e-9;
-    if (fabs(r) < 2.0 * thresh * L && fabs(r) / L < thresh) return 1;
-    if (fabs(r - L) < 2.0 * thresh * L && fabs(r - L) / L < thresh) return n;
+    if (fabs(r) / L < thresh) return 1;
+    if (fabs(r - L) / L < thresh) return n;
     if (!(r >= 0.0 && r < L)) return 1;     // non-finite input: keep the gather in bounds
     return min(max((int)floor(r / d) + 1, 1), n);
 }
@@ -50,7 +35,6 @@ struct TriStencil {
     long o00, o01, o10, o11;   // (j0, i0), (j0, i1), (j1, i0), (j1, i1) at level k0
     long dk;                   // plane offset from k0 to k1
     double xdiff, ydiff, zdiff;
-    double fx, fy, fz;         // xdiff / dx, ydiff / dy, zdiff / dz
 };
 
 __device__ __forceinline__ TriStencil tri_stencil(const InterpGeo& g, double px, double py,
@@ -78,7 +62,7 @@ __device__ __forceinline__ TriStencil tri_stencil(const InterpGeo& g, double px,
     } else {
         const double zw1 = ((1 - 0.5) * g.dz) - g.dz / 2.0;
         const double zwn = ((g.nz - 0.5) * g.dz) - g.dz / 2.0;
-        if (fabs(pz - zwn) < 2.0e-9 * g.lz && fabs(pz - zwn) / g.lz < 1.0e-9) k0 = g.nz - 1;
+        if (fabs(pz - zwn) / g.lz < 1.0e-9) k0 = g.nz - 1;
         else if (!(pz == pz)) k0 = 1;
         else k0 = (int)fmin(fmax(floor((pz - zw1) / g.dz) + 1.0, 0.0), (double)g.nz);
         k1 = k0 + 1;
@@ -94,9 +78,6 @@ __device__ __forceinline__ TriStencil tri_stencil(const InterpGeo& g, double px,
     t.o10 = base + (long)(j1 - 1) * g.ld + (i0 - 1);
     t.o11 = base + (long)(j1 - 1) * g.ld + (i1 - 1);
     t.dk = (long)(k1 - k0) * g.plane;
-    t.fx = t.xdiff / g.dx;
-    t.fy = t.ydiff / g.dy;
-    t.fz = t.zdiff / g.dz;
     return t;
 }
 
@@ -107,15 +88,13 @@ __device__ __forceinline__ double tri_apply(const T* var, const TriStencil& t,
     const double a10 = (double)var[t.o10], a11 = (double)var[t.o11];
     const double b00 = (double)var[t.o00 + t.dk], b01 = (double)var[t.o01 + t.dk];
     const double b10 = (double)var[t.o10 + t.dk], b11 = (double)var[t.o11 + t.dk];
-    // xdiff (b - a) / dx as (xdiff / dx) (b - a): one division per axis and
-    // point instead of one per term and field (the same value to rounding)
-    const double u1 = a00 + t.fx * (a01 - a00);
-    const double u2 = a10 + t.fx * (a11 - a10);
-    const double u3 = b00 + t.fx * (b01 - b00);
-    const double u4 = b10 + t.fx * (b11 - b10);
-    const double u5 = u1 + t.fy * (u2 - u1);
-    const double u6 = u3 + t.fy * (u4 - u3);
-    return u5 + t.fz * (u6 - u5);
+    const double u1 = a00 + t.xdiff * (a01 - a00) / g.dx;
+    const double u2 = a10 + t.xdiff * (a11 - a10) / g.dx;
+    const double u3 = b00 + t.xdiff * (b01 - b00) / g.dx;
+    const double u4 = b10 + t.xdiff * (b11 - b10) / g.dx;
+    const double u5 = u1 + t.ydiff * (u2 - u1) / g.dy;
+    const double u6 = u3 + t.ydiff * (u4 - u3) / g.dy;
+    return u5 + t.zdiff * (u6 - u5) / g.dz;
 }
 
 // interpolag_Sdep (interpolag_Sdep.f90:73-241): departure points and the
