@@ -186,7 +186,15 @@ Case<T>::Case(const lg_params& p, int members) : d(p), M(members), Ma(members) {
     sU = dalloc<cplx<T>>(M * msp);
     sV = dalloc<cplx<T>>(M * msp);
     sW = dalloc<cplx<T>>(M * msp);
-    gam = dalloc<double>(M * msp);
+    gam = dalloc<double>(msp);
+    {
+        // the elimination coefficients of the pressure systems (grid only)
+        const TriGeo tg{d.ny, d.NK, d.nz, d.splane, msp, 1, 2.0 * PI_D / d.prm.lx,
+                        2.0 * PI_D / d.prm.ly, d.dz, 1.0 / (d.dz * d.dz), 1.0 / d.dz};
+        const int n = d.ny * d.NK;
+        hipLaunchKernelGGL(k_tridag_gam, dim3((n + 255) / 256), dim3(256), 0, stream, gam, tg);
+        LG_HIP(hipGetLastError());
+    }
     usc = dalloc<double2>(M * msp);
     dev_err = dalloc<int>(1);
     lag_chunk = std::min(d.nz, 64);
@@ -456,7 +464,7 @@ size_t case_bytes(const lg_params& p, int members, int tsize) {
     b += (size_t)(12 + 3) * M * msp * 2 * T;                     // spec, sU/sV/sW
     const size_t nbig = prod_fused_for(d.nx2, d.ny2, tsize) ? 6 : 9;
     b += nbig * M * mbp * T + 6 * M * mbsp * 2 * T;              // big, bigspec
-    b += M * msp * (sizeof(double) + sizeof(double2));           // gam, usc
+    b += msp * sizeof(double) + M * msp * sizeof(double2);       // gam table, usc
     const bool lf = lag_fused_for(d.nx, d.ny, tsize);
     b += ((lf ? 0 : 15) + lag_filt_slots(lf, tsize)) * M * chunk * d.plane * T +
          21 * M * chunk * d.splane * 2 * T;

@@ -177,7 +177,7 @@ struct Case {
     cplx<T>* sU = nullptr;           // filtered u, v, w spectra of the current step
     cplx<T>* sV = nullptr;
     cplx<T>* sW = nullptr;
-    double* gam = nullptr;           // (nz+1) * ny * NK
+    double* gam = nullptr;           // (nz+1) * ny * NK: the pressure systems' gam table
     double2* usc = nullptr;          // (nz+1) * ny * NK
     int* dev_err = nullptr;
     // Lagrangian model scratch (chunked over levels)

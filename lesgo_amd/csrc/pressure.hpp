@@ -17,10 +17,46 @@ struct TriGeo {
     double hsc = 1.0;    // factor on Hx, Hy, Hz (1/(tadv1 dt) for velocity spectra)
 };
 
+// The Thomas elimination coefficients gam(m) = c(m-1) / bet(m-1) of every
+// (ky, kx) system: they depend on the grid only (kx, ky and dz), not on the
+// member or the step, so they are formed once per case, by the recurrence
+// k_tridag's forward sweep runs (the same operations, so the same values),
+// into gam[(nz+1) * ny * NK] (FP64).
+static __global__ void __launch_bounds__(256) k_tridag_gam(double* gam, TriGeo g) {
+    // no FMA contraction here or in k_tridag: both form bet = b - a gam with
+    // two roundings (as the reference does), so the table holds the values
+    // the forward sweep uses
+#pragma clang fp contract(off)
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.ny * g.NK) return;
+    const int jy = idx / g.NK;
+    const int jx = idx - jy * g.NK;
+    const long S = g.splane;
+    const double kx = g.kxs * (double)jx;
+    const int my = (jy < g.ny / 2) ? jy : jy - g.ny;
+    const double ky = g.kys * (double)my;
+    const double a_in = g.const3, c_in = g.const3;
+    const double b_in = -(kx * kx + ky * ky + 2.0 * g.const3);
+    double bet = -1.0, cprev = 1.0;
+    gam[idx] = 0.0;
+    for (int m = 1; m <= g.nz; ++m) {
+        const double a = m < g.nz ? a_in : -1.0;
+        const double b = m < g.nz ? b_in : 1.0;
+        const double cc = m < g.nz ? c_in : 0.0;
+        const double gm = cprev / bet;
+        gam[(long)m * S + idx] = gm;
+        bet = b - a * gm;
+        cprev = cc;
+    }
+}
+
 // One thread per (member, ky, kx < nx/2).  Hx, Hy hold levels 1..nz-1, Hz levels
 // 1..nz (all already scaled by 1/(nx ny tadv1 dt)); rb, rt the bottom/top
 // Neumann planes scaled by 1/(nx ny).  Writes p_hat for rows 0..nz and
-// uses gam and usc (FP64, (nz+1) * ny * NK each) as the Thomas scratch.
+// uses usc (FP64 complex, (nz+1) * ny * NK per member) as the Thomas
+// scratch; the forward sweep forms gam in registers and the back
+// substitution reads it from the case's table (k_tridag_gam, shared by the
+// members).
 // One column per thread leaves few waves per SIMD (ny * NK / 64 in all), so
 // each sweep issues the loads of a block of TRI_BLK levels together and then
 // runs the recurrence through them: one memory round trip per block, not
@@ -32,8 +68,9 @@ struct TriBlk {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_tridag(const cplx<T>* Hx, const cplx<T>* Hy, const cplx<T>* Hz,
-                         const cplx<T>* rb, const cplx<T>* rt, cplx<T>* ph, double* gam,
+                         const cplx<T>* rb, const cplx<T>* rt, cplx<T>* ph, const double* gam,
                          double2* usc, TriGeo g, int* err) {
+#pragma clang fp contract(off)
     constexpr int B = TriBlk<T>::n;
     const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (gidx >= g.M * g.ny * g.NK) return;
@@ -46,7 +83,6 @@ __global__ void __launch_bounds__(256) k_tridag(const cplx<T>* Hx, const cplx<T>
     rb += mb;
     rt += mb;
     ph += mb;
-    gam += mb;
     usc += mb;
     const int jy = idx / g.NK;
     const int jx = idx - jy * g.NK;
@@ -137,7 +173,6 @@ __global__ void __launch_bounds__(256) k_tridag(const cplx<T>* Hx, const cplx<T>
                 r = make_double2(-g.dz * t.x, -g.dz * t.y);
             }
             const double gm = cprev / bet;
-            gam[(long)m * S + idx] = gm;
             bet = b - a * gm;
             if (bet == 0.0) atomicExch(err, 1);
             u = make_double2((r.x - a * u.x) / bet, (r.y - a * u.y) / bet);

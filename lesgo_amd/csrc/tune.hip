@@ -129,7 +129,7 @@ PassChoice pick(int kind, int N, int other, hipStream_t s, int reps, F&& trial) 
 }  // namespace
 
 // Tune every pass of c's step; returns the number of passes tuned.  Trials
-// write only scratch arrays (tmp, spec, big, bigspec, gam/usc and the
+// write only scratch arrays (tmp, spec, big, bigspec, usc and the
 // spectra of the pressure solve), never the case's fields.
 template <typename T>
 int autotune(Case<T>& c, int reps) {
