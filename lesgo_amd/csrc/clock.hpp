@@ -35,7 +35,6 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_cfl_max(const T* u, const T* v, const T* w, int nx,
                                                  int ny, int ld, long plane, long mstride, int nz,
                                                  unsigned long long* cmax) {
-    __shared__ double red[3][256];
     const int m = blockIdx.y;
     const long base = (long)m * mstride + plane;   // level 1 of member m
     const long n = (long)(nz - 1) * ny * nx;
@@ -48,18 +47,17 @@ __global__ void __launch_bounds__(256) k_cfl_max(const T* u, const T* v, const T
         mv = fmax(mv, fabs((double)v[o]));
         mw = fmax(mw, fabs((double)w[o]));
     }
-    const int t = threadIdx.x;
-    red[0][t] = mu;
-    red[1][t] = mv;
-    red[2][t] = mw;
-    __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-        if (t < h)
-            for (int c = 0; c < 3; ++c) red[c][t] = fmax(red[c][t], red[c][t + h]);
-        __syncthreads();
+    // the block's maxima by wave shuffles (block_reduce), then one 64-bit
+    // atomic max per component on the bit patterns of the non-negative
+    // doubles
+    mu = block_reduce(mu, RedMax{}, 0.0);
+    mv = block_reduce(mv, RedMax{}, 0.0);
+    mw = block_reduce(mw, RedMax{}, 0.0);
+    if (threadIdx.x == 0) {
+        atomicMax(cmax + 3 * m, (unsigned long long)__double_as_longlong(mu));
+        atomicMax(cmax + 3 * m + 1, (unsigned long long)__double_as_longlong(mv));
+        atomicMax(cmax + 3 * m + 2, (unsigned long long)__double_as_longlong(mw));
     }
-    if (t < 3)
-        atomicMax(cmax + 3 * m + t, (unsigned long long)__double_as_longlong(red[t][0]));
 }
 
 // The head of the time loop for every stepped member: dt_f = dt,

@@ -58,4 +58,41 @@ struct HipError {};
 struct ArgError {};
 struct PivotError {};
 
+// Reductions over a 64-lane wave by cross-lane shuffles (no LDS): after the
+// five halving steps lane 0 holds the result.
+struct RedSum {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct RedMax {
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+
+template <typename Op>
+__device__ __forceinline__ double wave_reduce(double v, Op op) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+// Block reduction (at most 1024 threads): each wave reduces by shuffles,
+// the waves' partials meet in LDS and wave 0 reduces them by shuffles; the
+// result is in thread 0.  The order is fixed by the block shape, so a
+// member's sums are the same alone and in a batch.  Ends with a barrier, so
+// the call can be repeated in one kernel.
+template <typename Op>
+__device__ double block_reduce(double v, Op op, double identity) {
+    __shared__ double part[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_reduce(v, op);
+    if (lane == 0) part[wid] = v;
+    __syncthreads();
+    double r = identity;
+    if (wid == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        r = wave_reduce(lane < nw ? part[lane] : identity, op);
+    }
+    __syncthreads();
+    return r;
+}
+
 }  // namespace lg

@@ -835,10 +835,10 @@ __global__ void k_project(T* u, T* v, T* w, const T* px, const T* py, const T* p
     if (k >= 2) w[o] = w[o] + dt * (-(t1 * pz[o]));
 }
 
-// Block-partial sums for the diagnostics (rmsdiv.f90:39-50, io.f90:115-124).
+// Block-partial sums for the diagnostics (rmsdiv.f90:39-50, io.f90:115-124),
+// reduced by wave shuffles (block_reduce).
 template <typename T>
 __global__ void k_rmsdiv_part(const T* dudx, const T* dvdy, const T* dwdz, Geo g, double* part) {
-    __shared__ double sh[256];
     double acc = 0.0;
     const long n = (long)(g.nz - 1) * g.ny * g.nx;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
@@ -849,18 +849,12 @@ __global__ void k_rmsdiv_part(const T* dudx, const T* dvdy, const T* dwdz, Geo g
         const long o = k * g.plane + j * g.ld + i;
         acc += fabs((double)dudx[o] + (double)dvdy[o] + (double)dwdz[o]);
     }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    acc = block_reduce(acc, RedSum{}, 0.0);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 template <typename T>
 __global__ void k_energy_part(const T* u, const T* v, const T* w, Geo g, double* part) {
-    __shared__ double sh[256];
     double acc = 0.0;
     const long n = (long)(g.nz - 1) * g.ny * g.nx;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
@@ -872,18 +866,12 @@ __global__ void k_energy_part(const T* u, const T* v, const T* w, Geo g, double*
         const double tw = 0.5 * ((double)w[o] + (double)w[o + g.plane]);
         acc += (double)u[o] * u[o] + (double)v[o] * v[o] + tw * tw;
     }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    acc = block_reduce(acc, RedSum{}, 0.0);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 template <typename T>
 __global__ void k_maxabs_part(const T* f, Geo g, double* part) {
-    __shared__ double sh[256];
     double acc = 0.0;
     const long n = (long)(g.nz - 1) * g.ny * g.nx;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
@@ -893,13 +881,8 @@ __global__ void k_maxabs_part(const T* f, Geo g, double* part) {
         const long j = r / g.nx, i = r % g.nx;
         acc = fmax(acc, fabs((double)f[k * g.plane + j * g.ld + i]));
     }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    acc = block_reduce(acc, RedMax{}, 0.0);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 }  // namespace lg

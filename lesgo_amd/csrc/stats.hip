@@ -258,7 +258,6 @@ void Case<T>::stats_spectrum(int m, int c, double* host) {
 template <typename TA, typename TB>
 __global__ void k_diff_l2(const TA* a, const TB* b, int nx, int ny, int ld, int k0, int nlev,
                           double* part) {
-    __shared__ double s0[SBLK], s1[SBLK];
     const long n = (long)nlev * ny * nx;
     double d2 = 0.0, b2 = 0.0;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
@@ -270,19 +269,11 @@ __global__ void k_diff_l2(const TA* a, const TB* b, int nx, int ny, int ld, int 
         d2 += (x - y) * (x - y);
         b2 += y * y;
     }
-    s0[threadIdx.x] = d2;
-    s1[threadIdx.x] = b2;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            s0[threadIdx.x] += s0[threadIdx.x + s];
-            s1[threadIdx.x] += s1[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
+    d2 = block_reduce(d2, RedSum{}, 0.0);
+    b2 = block_reduce(b2, RedSum{}, 0.0);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s0[0];
-        part[2 * blockIdx.x + 1] = s1[0];
+        part[2 * blockIdx.x] = d2;
+        part[2 * blockIdx.x + 1] = b2;
     }
 }
 

@@ -117,7 +117,6 @@ __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* no
                               const double* node_ind, const int* off, const double* coef,
                               double* state, long plane, double dV, TurbFilter tf,
                               TurbCenter tc) {
-    __shared__ double sh[256];
     const int s = blockIdx.x;
     const double n1 = coef[TC_N * s + 0], n2 = coef[TC_N * s + 1], n3 = coef[TC_N * s + 2];
     double acc = 0.0;
@@ -126,16 +125,12 @@ __global__ void k_turb_reduce(const T* u, const T* v, const T* w, const long* no
         const double wuv = 0.5 * ((double)w[o + plane] + (double)w[o]);
         acc += dV * node_ind[l] * (n1 * (double)u[o] + n2 * (double)v[o] + n3 * wuv);
     }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-        __syncthreads();
-    }
+    // the disk's sum by wave shuffles, then across the block's waves
+    acc = block_reduce(acc, RedSum{}, 0.0);
     if (threadIdx.x == 0) {
         const double ct = coef[TC_N * s + 3], dia = coef[TC_N * s + 4],
                      corr = coef[TC_N * s + 5];
-        double ud = sh[0];
+        double ud = acc;
         ud = ud / corr;
         double eps = 1.0;
         if (tf.T_avg_dim > 0.0) {
