@@ -44,24 +44,32 @@ __global__ void __launch_bounds__(LAGROW_BLK) k_lag_rows(LagRowArgs<T> a) {
         const int q = t / N2, n2 = t - q * N2;
         const cplx<T>* ra = base + (long)q * a.qstride;
         const cplx<T>* rb = ra + nc;
+        // every in-band load is issued before any is used (the band is a
+        // runtime value: a use inside its branch would wait for each load in
+        // turn), then the Hermitian extension
+        cplx<T> A[N1], B[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int k = n2 + N2 * n1;
+            const int kk = k < half ? k : N - k;
+            A[n1] = B[n1] = cplx<T>{(T)0, (T)0};
+            if (k != half && kk < nce) {
+                A[n1] = gld(ra + kk);
+                B[n1] = gld(rb + kk);
+            }
+        }
         cplx<T> x[N1];
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) {
             const int k = n2 + N2 * n1;
+            const int kk = k < half ? k : N - k;
             cplx<T> z{(T)0, (T)0};
             if (k == 0) {
-                z = cplx<T>{gld(ra).x, gld(rb).x};
+                z = cplx<T>{A[n1].x, B[n1].x};
             } else if (k < half) {
-                if (k < nce) {
-                    const cplx<T> A = gld(ra + k), B = gld(rb + k);
-                    z = cplx<T>{A.x - B.y, A.y + B.x};
-                }
+                if (kk < nce) z = cplx<T>{A[n1].x - B[n1].y, A[n1].y + B[n1].x};
             } else if (k > half) {
-                const int kk = N - k;
-                if (kk < nce) {
-                    const cplx<T> A = gld(ra + kk), B = gld(rb + kk);
-                    z = cplx<T>{A.x + B.y, B.x - A.y};
-                }
+                if (kk < nce) z = cplx<T>{A[n1].x + B[n1].y, B[n1].x - A[n1].y};
             }
             x[n1] = z;
         }
