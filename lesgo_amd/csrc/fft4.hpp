@@ -80,10 +80,16 @@ __device__ __forceinline__ void cols_inv_load(cplx<T> (&x)[N1], const cplx<T>* s
     constexpr int N = N1 * N2;
     const unsigned nc = (unsigned)a.ncols;
     const T kxv = a.kxs * (T)c;
+    // every load is issued before any is used (the band and the Nyquist
+    // choice are runtime values: a use inside its branch would wait for each
+    // load in turn), then the multiplier
+    int jr[N1];
+    bool ld[N1], nq[N1];
+    cplx<T> raw[N1];
+    T gv[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int jb = n2 + N2 * n1;
-        cplx<T> v{(T)0, (T)0};
         int j = jb;          // small-grid row
         bool load = true;
         bool nyq = false;    // the small grid's Nyquist row (MAP_ID only)
@@ -113,9 +119,24 @@ __device__ __forceinline__ void cols_inv_load(cplx<T> (&x)[N1], const cplx<T>* s
                 else load = false;
             }
         }
+        jr[n1] = j;
+        ld[n1] = load;
+        nq[n1] = nyq;
+        raw[n1] = cplx<T>{(T)0, (T)0};
+        gv[n1] = (T)0;
         if (load) {
             const unsigned off = ((unsigned)j * nc + (unsigned)c) * (unsigned)sizeof(cplx<T>);
-            v = at_bytes(sb, off);
+            raw[n1] = at_bytes(sb, off);
+            if constexpr (K == K_G || K == K_GB) gv[n1] = gt[(unsigned)j * nc + (unsigned)c];
+        }
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        cplx<T> v{(T)0, (T)0};
+        if (ld[n1]) {
+            v = raw[n1];
+            const int j = jr[n1];
+            const bool nyq = nq[n1];
             if constexpr (K == K_IKX) {
                 const T k = nyq ? (T)0 : kxv;
                 v = cplx<T>{-v.y * k, v.x * k};
@@ -123,12 +144,14 @@ __device__ __forceinline__ void cols_inv_load(cplx<T> (&x)[N1], const cplx<T>* s
                 const T k = nyq ? (T)0 : a.kys * (T)ky_index(j, a.ny);
                 v = cplx<T>{-v.y * k, v.x * k};
             } else if constexpr (K == K_G) {
-                const T g = gt[(unsigned)j * nc + (unsigned)c];
+                const T g = gv[n1];
                 v = cplx<T>{g * v.x, g * v.y};
             } else if constexpr (K == K_GB) {
-                const bool in = gt[(unsigned)j * nc + (unsigned)c] != (T)0 && c != a.ncols - 1;
+                const bool in = gv[n1] != (T)0 && c != a.ncols - 1;
                 if (!in) v = cplx<T>{(T)0, (T)0};
             }
+            (void)j;
+            (void)nyq;
         }
         x[n1] = v;
     }
@@ -142,6 +165,8 @@ __device__ __forceinline__ void cols_inv_load_divxy(cplx<T> (&x)[N1], const cplx
     constexpr int N = N1 * N2;
     constexpr int NH = N / 2;
     const unsigned nc = (unsigned)a.ncols;
+    bool ld[N1];
+    cplx<T> ra[N1], rb[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int jb = n2 + N2 * n1;
@@ -153,12 +178,19 @@ __device__ __forceinline__ void cols_inv_load_divxy(cplx<T> (&x)[N1], const cplx
             load = !nyq && m < ykeep;
         }
         if (nyq && nyq0) load = false;
-        cplx<T> v{(T)0, (T)0};
+        ld[n1] = load;
+        ra[n1] = rb[n1] = cplx<T>{(T)0, (T)0};
         if (load) {
+            // (issued before any is used: see cols_inv_load)
             const unsigned off = ((unsigned)jb * nc + (unsigned)c) * (unsigned)sizeof(cplx<T>);
-            v = divxy<T>(at_bytes(sb, off), at_bytes(sb2, off), c, jb, a);
+            ra[n1] = at_bytes(sb, off);
+            rb[n1] = at_bytes(sb2, off);
         }
-        x[n1] = v;
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int jb = n2 + N2 * n1;
+        x[n1] = ld[n1] ? divxy<T>(ra[n1], rb[n1], c, jb, a) : cplx<T>{(T)0, (T)0};
     }
 }
 
