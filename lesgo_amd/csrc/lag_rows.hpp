@@ -153,30 +153,65 @@ __global__ void __launch_bounds__(LAGROW_BLK) k_lag_prep_rows(LagPrepArgs<T> a) 
     const int tid = threadIdx.x;
     const long P = a.fplane;
     // the quantities at every point of the two rows, into slot q (row j in
-    // .x, row j + 1 in .y), natural order
-    for (int t = tid; t < 2 * N; t += blockDim.x) {
-        const int r = t / N, i = t - r * N;
+    // .x, row j + 1 in .y), natural order.  A thread takes points t and
+    // t + blockDim.x (2N <= 2 blockDim.x) and loads both before forming
+    // either, so the second round's loads do not wait for the first's stores
+    static_assert(2 * N <= 2 * LAGROW_BLK, "two rounds at most");
+    T in[2][9];
+    bool has[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int t = tid + h * (int)blockDim.x;
+        has[h] = t < 2 * N;
+        const int tt = has[h] ? t : tid;
+        const int r = tt / N, i = tt - r * N;
         const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + r) * a.ld + i;
+        // u, v at k and k - 1 (level 1: k only), w at k + 1 (level 1) or k
+        const long ou0 = o, ou1 = k == 1 ? o : o - P;
+        const long ow = k == 1 ? o + P : o;
+        in[h][0] = a.u[ou0];
+        in[h][1] = a.u[ou1];
+        in[h][2] = a.v[ou0];
+        in[h][3] = a.v[ou1];
+        in[h][4] = a.w[ow];
+        in[h][5] = a.S11[o];
+        in[h][6] = a.S12[o];
+        in[h][7] = a.S13[o];
+        in[h][8] = a.S22[o];
+    }
+    T s23[2], s33[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int t = has[h] ? tid + h * (int)blockDim.x : tid;
+        const long o = (long)mem * a.fmstride + (long)k * P + (long)(j + t / N) * a.ld + (t % N);
+        s23[h] = a.S23[o];
+        s33[h] = a.S33[o];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!has[h]) continue;
+        const int t = tid + h * (int)blockDim.x;
+        const int r = t / N, i = t - r * N;
         T ub, vb, wb;
         if (k == 1) {
-            ub = a.u[o];
-            vb = a.v[o];
-            wb = (T)0.25 * a.w[o + P];
+            ub = in[h][0];
+            vb = in[h][2];
+            wb = (T)0.25 * in[h][4];
         } else if (k == a.nz) {
-            ub = a.u[o - P];
-            vb = a.v[o - P];
+            ub = in[h][1];
+            vb = in[h][3];
             wb = (T)0;
         } else {
-            ub = (T)0.5 * (a.u[o] + a.u[o - P]);
-            vb = (T)0.5 * (a.v[o] + a.v[o - P]);
-            wb = a.w[o];
+            ub = (T)0.5 * (in[h][0] + in[h][1]);
+            vb = (T)0.5 * (in[h][2] + in[h][3]);
+            wb = in[h][4];
         }
-        const T s11 = a.S11[o], s12 = a.S12[o], s13 = a.S13[o], s22 = a.S22[o], s23 = a.S23[o],
-                s33 = a.S33[o];
-        const T S = smag_norm(s11, s12, s13, s22, s23, s33);
+        const T s11 = in[h][5], s12 = in[h][6], s13 = in[h][7], s22 = in[h][8],
+                s23v = s23[h], s33v = s33[h];
+        const T S = smag_norm(s11, s12, s13, s22, s23v, s33v);
         const T q[LAGROW_Q] = {ub,      vb,      wb,      ub * ub, ub * vb, ub * wb, vb * vb,
-                               vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23,
-                               s33,     S * s11, S * s12, S * s13, S * s22, S * s23, S * s33};
+                               vb * wb, wb * wb, s11,     s12,     s13,     s22,     s23v,
+                               s33v,    S * s11, S * s12, S * s13, S * s22, S * s23v, S * s33v};
 #pragma unroll
         for (int qq = 0; qq < LAGROW_Q; ++qq) {
             T* e = reinterpret_cast<T*>(lds + qq * YS + i) + r;
