@@ -16,8 +16,8 @@ figure and the ratio are reported beside it, with the whole-step roofline
 (algorithmic bytes and flops per step against the HBM and vector peaks).
 The ensemble, serving and CPU legs follow.
 
-Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs one case
-per GPU with no sharding.
+Prints one JSON line.  `--gpus N` with N > 1 exits: the library runs a case
+on one GPU, with no sharding.
 """
 from __future__ import annotations
 
@@ -410,7 +410,7 @@ def main():
         print(json.dumps({"tune_file": TUNE_FILE}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs one case per GPU with no sharding; "
+        print(json.dumps({"error": "the library runs a case on one GPU, with no sharding; "
                                    "run with --gpus 1 (multi-GPU scaling is not built)"}))
         sys.exit(2)
     nx, ny, Nz, dt, seed, turb = WORKLOADS[args.workload]

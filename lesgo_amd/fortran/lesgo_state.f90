@@ -10,7 +10,7 @@ end module types
 module param
 !*******************************************************************************
 ! The subset of the reference's param module (param.f90) the time step reads,
-! with the same names; set by the driver's input, derived sizes follow
+! with the same names; set from the main program's input, derived sizes follow
 ! input_util.f90:196-235 for one process (lbz = 0, nz = Nz + 1).
 use types, only : rprec
 implicit none
