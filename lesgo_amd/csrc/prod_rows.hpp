@@ -10,9 +10,9 @@
 // (the blocks of the levels either side, launched nrows / 2P blocks before
 // and after on the same XCD, read the same planes, so those loads mostly hit
 // the caches), and writes the three products of its points into LDS in
-// natural order: sequence q * P + s holds product q
-// (0: x, 1: y, 2: z) of pair s, row a in .x and row b in .y.  The 3P
-// sequences then take k_rows_r2c_v4's four-step arithmetic.  The products are
+// natural order: sequence q * P + s holds product q (0: x, 1: y, 2: z) of
+// pair s, row a in .x and row b in .y.  The 3P sequences then take
+// k_rows_r2c_v4's four-step arithmetic.  The products are
 // k_products' expressions (same operands, same order, FP64 with no
 // contraction, rounded once to T), so the transformed values are those of
 // the unfused passes.
@@ -22,7 +22,6 @@
 #include "pointwise.hpp"
 
 namespace lg {
-
 
 // One point's values at level k: u, v, o3 at k; w, o1, o2 at k + 1; u, v at
 // k - 1 and w, o1, o2 at k.
