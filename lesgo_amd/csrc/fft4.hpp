@@ -354,18 +354,55 @@ __device__ __forceinline__ void rows_c2r_v4_body(const RowC2RArgs<T>& a) {
                         z = cplx<T>{A.x + B.y, B.x - A.y};
                     }
                 };
+                // With a band known at compile time, the elements whose case
+                // differs between lanes (k = 0, the Nyquist column, the band's
+                // edge) load unconditionally from a valid column and select
+                // the result: a load and its use inside a per-lane branch
+                // would hold the loads after it until it returns.
+                // (n1 = 0 is inside the band whenever the band is known:
+                // N2 - 1 < N / 3 for every split of a 3/2-grid length)
+                constexpr bool CT = BAND == BAND_FULL || (BAND == BAND_THIRD && N2 - 1 < N / 3);
                 if (hi < half) {
-                    if (lo == 0 && k == 0) {
+                    if (lo == 0 && CT) {
+                        cplx<T> A, B;
+                        ld2(k, A, B);
+                        const cplx<T> zb{A.x - B.y, A.y + B.x};
+                        const cplx<T> z0 = pass ? cplx<T>{(T)0, (T)0} : cplx<T>{A.x, B.x};
+                        z = k == 0 ? z0 : zb;
+                    } else if (lo == 0 && k == 0) {
                         if (!pass) {
                             const cplx<T> A = at_bytes(sb, ra * (unsigned)sizeof(cplx<T>));
                             const cplx<T> B = at_bytes(sb, (ra + nc) * (unsigned)sizeof(cplx<T>));
                             z = cplx<T>{A.x, B.x};
                         }
+                    } else if (BAND == BAND_THIRD && lo < N / 3 && hi >= N / 3) {
+                        // the band's edge from below: column k, kept when below nce
+                        cplx<T> A, B;
+                        ld2(k < nce ? k : nce - 1, A, B);
+                        const cplx<T> zb{A.x - B.y, A.y + B.x};
+                        z = k < nce ? zb : cplx<T>{(T)0, (T)0};
                     } else {
                         below(k, lo, hi);
                     }
                 } else if (lo > half) {
-                    above(N - k, N - hi, N - lo);
+                    if (BAND == BAND_THIRD && N - hi < N / 3 && N - lo >= N / 3) {
+                        // the band's edge: column N - k, kept when below nce
+                        const int kk = N - k;
+                        cplx<T> A, B;
+                        ld2(kk < nce ? kk : nce - 1, A, B);
+                        const cplx<T> za{A.x + B.y, B.x - A.y};
+                        z = kk < nce ? za : cplx<T>{(T)0, (T)0};
+                    } else {
+                        above(N - k, N - hi, N - lo);
+                    }
+                } else if (BAND == BAND_FULL) {
+                    // the Nyquist column's element: below, above or zero by lane
+                    const int kk = k < half ? k : (k > half ? N - k : 0);
+                    cplx<T> A, B;
+                    ld2(kk, A, B);
+                    const cplx<T> zb{A.x - B.y, A.y + B.x};
+                    const cplx<T> za{A.x + B.y, B.x - A.y};
+                    z = k < half ? zb : (k > half ? za : cplx<T>{(T)0, (T)0});
                 } else {
                     if (k < half) below(k, lo, half - 1);
                     else if (k > half) above(N - k, N - hi, half - 1);
