@@ -354,16 +354,17 @@ __device__ __forceinline__ void rows_c2r_v4_body(const RowC2RArgs<T>& a) {
                         z = cplx<T>{A.x + B.y, B.x - A.y};
                     }
                 };
-                // With a band known at compile time, the elements whose case
-                // differs between lanes (k = 0, the Nyquist column, the band's
-                // edge) load unconditionally from a valid column and select
-                // the result: a load and its use inside a per-lane branch
-                // would hold the loads after it until it returns.
-                // (n1 = 0 is inside the band whenever the band is known:
-                // N2 - 1 < N / 3 for every split of a 3/2-grid length)
-                constexpr bool CT = BAND == BAND_FULL || (BAND == BAND_THIRD && N2 - 1 < N / 3);
+                // FP32 over the 3/2 grid's third: the elements whose case
+                // differs between lanes (k = 0 and the third's edges) load
+                // unconditionally from a valid column and select the result,
+                // so that no load and its use sit in a per-lane branch, which
+                // would hold the loads after it until it returns (measured
+                // faster there; the FP64 instance and the whole half spectrum
+                // ran slower this way and keep the branches).  n1 = 0 lies
+                // inside the band: N2 - 1 < N / 3.
+                constexpr bool SEL = BAND == BAND_THIRD && sizeof(T) == 4 && N2 - 1 < N / 3;
                 if (hi < half) {
-                    if (lo == 0 && CT) {
+                    if (lo == 0 && SEL) {
                         cplx<T> A, B;
                         ld2(k, A, B);
                         const cplx<T> zb{A.x - B.y, A.y + B.x};
@@ -375,8 +376,8 @@ __device__ __forceinline__ void rows_c2r_v4_body(const RowC2RArgs<T>& a) {
                             const cplx<T> B = at_bytes(sb, (ra + nc) * (unsigned)sizeof(cplx<T>));
                             z = cplx<T>{A.x, B.x};
                         }
-                    } else if (BAND == BAND_THIRD && lo < N / 3 && hi >= N / 3) {
-                        // the band's edge from below: column k, kept when below nce
+                    } else if (SEL && lo < N / 3 && hi >= N / 3) {
+                        // the third's edge from below: column k, kept below nce
                         cplx<T> A, B;
                         ld2(k < nce ? k : nce - 1, A, B);
                         const cplx<T> zb{A.x - B.y, A.y + B.x};
@@ -385,8 +386,8 @@ __device__ __forceinline__ void rows_c2r_v4_body(const RowC2RArgs<T>& a) {
                         below(k, lo, hi);
                     }
                 } else if (lo > half) {
-                    if (BAND == BAND_THIRD && N - hi < N / 3 && N - lo >= N / 3) {
-                        // the band's edge: column N - k, kept when below nce
+                    if (SEL && N - hi < N / 3 && N - lo >= N / 3) {
+                        // the third's edge from above: column N - k, kept below nce
                         const int kk = N - k;
                         cplx<T> A, B;
                         ld2(kk < nce ? kk : nce - 1, A, B);
@@ -395,14 +396,6 @@ __device__ __forceinline__ void rows_c2r_v4_body(const RowC2RArgs<T>& a) {
                     } else {
                         above(N - k, N - hi, N - lo);
                     }
-                } else if (BAND == BAND_FULL) {
-                    // the Nyquist column's element: below, above or zero by lane
-                    const int kk = k < half ? k : (k > half ? N - k : 0);
-                    cplx<T> A, B;
-                    ld2(kk, A, B);
-                    const cplx<T> zb{A.x - B.y, A.y + B.x};
-                    const cplx<T> za{A.x + B.y, B.x - A.y};
-                    z = k < half ? zb : (k > half ? za : cplx<T>{(T)0, (T)0});
                 } else {
                     if (k < half) below(k, lo, half - 1);
                     else if (k > half) above(N - k, N - hi, half - 1);
