@@ -95,14 +95,12 @@ __global__ void __launch_bounds__(256) k_rows_r2c_prod(RowR2CArgs<T> a, ProdSrc<
     tw_stage<T, TWL>(ltw, gtw, N);
     {
         const int tid = threadIdx.x;
-        // products of the level into LDS, natural order
-#pragma unroll 1
-        for (int task = tid; task < ntask; task += blockDim.x) {
-            // task: (row r of the block's 2P, point pair ip)
+        // products of the level into LDS, natural order; a round's loads
+        // are issued before the round ahead of it forms its products (the
+        // blocks are bound by LDS, so the registers this holds cost no waves)
+        auto load = [&](int task, ProdPt<T>& q0, ProdPt<T>& q1) {
             const int r = task / NH, ip = task - r * NH;
-            const int s = r >> 1, e = r & 1;
             const long o = lk + (long)r * sld + 2 * ip;
-            ProdPt<T> q0, q1;
             ld_pair(p.ub + o, q0.u, q1.u);
             ld_pair(p.vb + o, q0.v, q1.v);
             ld_pair(p.o3 + o, q0.o3, q1.o3);
@@ -117,6 +115,16 @@ __global__ void __launch_bounds__(256) k_rows_r2c_prod(RowR2CArgs<T> a, ProdSrc<
                 ld_pair(p.o1 + o + PL, q0.o11, q1.o11);
                 ld_pair(p.o2 + o + PL, q0.o21, q1.o21);
             }
+        };
+        ProdPt<T> n0{}, n1{};
+        if (tid < ntask) load(tid, n0, n1);
+#pragma unroll 1
+        for (int task = tid; task < ntask; task += blockDim.x) {
+            // task: (row r of the block's 2P, point pair ip)
+            const int r = task / NH, ip = task - r * NH;
+            const int s = r >> 1, e = r & 1;
+            const ProdPt<T> q0 = n0, q1 = n1;
+            if (task + (int)blockDim.x < ntask) load(task + blockDim.x, n0, n1);
             T px[2], py[2], pz[2];
             conv_products(q0, k, nz, p.c, px[0], py[0], pz[0]);
             conv_products(q1, k, nz, p.c, px[1], py[1], pz[1]);
