@@ -95,9 +95,10 @@ __global__ void __launch_bounds__(256) k_rows_r2c_prod(RowR2CArgs<T> a, ProdSrc<
     tw_stage<T, TWL>(ltw, gtw, N);
     {
         const int tid = threadIdx.x;
-        // products of the level into LDS, natural order; a round's loads
-        // are issued before the round ahead of it forms its products (the
-        // blocks are bound by LDS, so the registers this holds cost no waves)
+        // products of the level into LDS, natural order; a thread issues the
+        // loads of its next task before it forms the products of the current
+        // one (the blocks are bound by LDS, so the registers this holds cost
+        // no waves)
         auto load = [&](int task, ProdPt<T>& q0, ProdPt<T>& q1) {
             const int r = task / NH, ip = task - r * NH;
             const long o = lk + (long)r * sld + 2 * ip;
